@@ -1,0 +1,101 @@
+// StereoMapper: the per-keyframe depth pipeline behind the reference's interface
+// (motion_stereo/stereo_mapper.h:26-63).  Same class name, method set, argument order and
+// semantics; cv::Mat / cv::cuda::GpuMat / Eigen arguments become the look-alikes of
+// pm/types.h.  Differences a caller can observe:
+//   * shapes are runtime (the default constructor uses WIDTH x HEIGHT x DEP_CNT);
+//   * update() stages the measurement frame; the cost of all staged frames is computed in one
+//     pass when an output method runs (the running mean of the reference is order-free);
+//   * raw_cost may be stored as fp16 / bf16 (set_cost_type);
+//   * no GUI calls: outputFusion's imshow (stereo_mapper.cpp:232-235) is dropped;
+//   * errors raise pm::Error (a std::runtime_error with a pm_status) instead of exiting.
+#ifndef PM_STEREO_MAPPER_H
+#define PM_STEREO_MAPPER_H
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pm/parameters.h"
+#include "pm/types.h"
+#include "pm_capi.h"
+
+namespace pm {
+
+class Error : public std::runtime_error {
+ public:
+  Error(pm_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+  pm_status status;
+};
+
+class StereoMapper {
+ public:
+  StereoMapper();
+  StereoMapper(int width, int height, int dep_cnt);
+  ~StereoMapper();
+  StereoMapper(const StereoMapper&) = delete;
+  StereoMapper& operator=(const StereoMapper&) = delete;
+
+  void initIntrinsic(const Mat& _K1, float bf, float dep_sample);
+  void initReference(const Mat& _img_l);
+  void update(const Mat& _img_r, const Mat& R_l, const Mat& T_l, const Mat& R_r,
+              const Mat& T_r);
+  void output(Mat& result);
+  void outputFusion(Mat& result, Matrix3d R, Vector3d t, float fx, float fy, float cx,
+                    float cy);
+  // CASE 0 / 1 / 2 ablation outputs (stereo_mapper.cpp:625-719).  For CASE >= 3 this runs
+  // the GPU fusion of outputFusion (the reference's CPU-constant fusion is not provided).
+  void outputFusionCPU(Mat& result, Matrix3d R, Vector3d t, float fx, float fy, float cx,
+                       float cy);
+
+  // extensions
+  void set_cost_type(pm_cost_type t);
+  pm_cost_type cost_type() const { return cost_type_; }
+  int width() const { return W_; }
+  int height() const { return H_; }
+  int planes() const { return D_; }
+  int num_sources() const { return (int)homs_.size(); }
+  void synchronize() const;
+  // runs the pending cost, SGM and filter stages without downloading anything
+  void compute_depth();
+  // CASE 0 / 1 winner-take-all maps (host result [H][W])
+  void wta(int case_mode, Mat& result);
+
+  // members of the reference (stereo_mapper.h:44-62); device buffers
+  DeviceMat img_l;                         // uint8 reference image
+  DeviceMat raw_cost, sgm_cost;            // [H][W][D]
+  DeviceMat dep;                           // filtered depth [H][W]
+  DeviceMat propogate_table;               // int32 [H][W], filled table of the last fusion
+  DeviceMat alpha0, beta0, mu0, sigma0;
+  DeviceMat alpha1, beta1, mu1, sigma1;
+  DeviceMat fuseDepth;                     // fused depth [H][W]
+
+  int pre_index, cur_index;
+  bool firstDepth;
+  Mat nK1;
+  Mat R, T;                                // last homography setup (H1, H2)
+  Matrix3d R_b_2_w;
+  Vector3d t_b_2_w;
+
+  float DEP_SAMPLE;
+
+ private:
+  void alloc();
+  void release();
+  void ensure_cost();
+  void run_sgm_filter();
+
+  int W_, H_, D_;
+  pm_cost_type cost_type_ = PM_COST_F32;
+  pm_cost_type raw_cost_type_ = PM_COST_F32;
+  void* stream_ = nullptr;
+  std::vector<DeviceMat> quads_;           // per staged source, uint32 [H][W]
+  std::vector<pm_homography> homs_;
+  DeviceMat zbuf_, med_tmp_, wta_tmp_;
+  bool cost_ready_ = false;
+  bool depth_ready_ = false;
+  bool have_reference_ = false;
+};
+
+}  // namespace pm
+
+#endif  // PM_STEREO_MAPPER_H

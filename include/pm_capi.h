@@ -1,0 +1,201 @@
+/* pm_capi.h -- C ABI of the MI355X depth pipeline (libpm_hip.so).
+ *
+ * Replaces the kernel-call layer of HKUST-Aerial-Robotics/probabilistic_mapping
+ * (motion_stereo/{calc_cost,sgm,filter_cost,depth_fusion,convert}.h).  Those were C++-linkage
+ * functions that took untyped device pointers (unsigned char*), read shapes from compile-time
+ * constants and knobs from globals, synchronised the device, and called exit() on error.
+ * Here:
+ *   - every entry point is extern "C", takes typed device pointers plus explicit shapes,
+ *     and returns a pm_status (never exits);
+ *   - work is enqueued on the caller's stream (NULL = default stream) and is asynchronous;
+ *   - stage entry points allocate nothing: the caller owns every buffer;
+ *   - pm_last_error() returns a message for the last failure on the calling thread.
+ *
+ * Data layouts (device memory):
+ *   image   : uint8 [H][W]                  (row-major, pitch = W)
+ *   quad    : uint32 [H][W]  texel (x,y) packs I(x,y), I(x+1,y), I(x,y+1), I(x+1,y+1) in bytes
+ *             0..3, with 0 outside the image -- one load per bilinear sample
+ *   cost    : [H][W][D] pixel-major, D contiguous (the reference's INDEX(y,x,d),
+ *             motion_stereo/cuda_utils.h:34), element type per pm_cost_type
+ *   agg     : float [H][W][D], same layout
+ *   depth, filter state : float [H][W]
+ */
+#ifndef PM_CAPI_H
+#define PM_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pm_status {
+  PM_OK = 0,
+  PM_ERR_INVALID_ARGUMENT = 1,
+  PM_ERR_OUT_OF_MEMORY = 2,
+  PM_ERR_LAUNCH = 3,
+  PM_ERR_INVALID_STATE = 4,
+  PM_ERR_DEVICE = 5
+} pm_status;
+
+typedef enum pm_cost_type {
+  PM_COST_F32 = 0,
+  PM_COST_F16 = 1,
+  PM_COST_BF16 = 2
+} pm_cost_type;
+
+typedef void* pm_stream; /* hipStream_t; NULL = the default stream */
+
+/* Largest supported plane count and source count per keyframe. */
+#define PM_MAX_PLANES 256
+#define PM_MAX_SOURCES 16
+
+typedef struct pm_shape {
+  int32_t width, height, planes;
+} pm_shape;
+
+/* Plane-sweep homography of one measurement frame, FP64 setup rounded to float:
+ * h = K R_r^T R_l K^-1 (row-major), t = K R_r^T (T_l - T_r).
+ * Reference: StereoMapper::update, motion_stereo/stereo_mapper.cpp:79-90. */
+typedef struct pm_homography {
+  float h[9];
+  float t[3];
+} pm_homography;
+
+/* SGM knobs (motion_stereo/parameters.cpp:10-14) plus the path count (4 = the reference's
+ * L->R, R->L, T->B, B->T set, sgm.cu:119-131; 8 adds the four diagonals).  Penalties and
+ * tau_so must be finite and non-negative. */
+typedef struct pm_sgm_params {
+  float pi1, pi2, tau_so, sgm_q1, sgm_q2;
+  int32_t paths;
+} pm_sgm_params;
+
+typedef struct pm_camera {
+  float fx, fy, cx, cy;
+} pm_camera;
+
+/* Gaussian x Beta filter state planes, each float [H][W]
+ * (alpha0..sigma1 of motion_stereo/stereo_mapper.h:48-49). */
+typedef struct pm_fusion_state {
+  float *alpha, *beta, *mu, *sig;
+} pm_fusion_state;
+
+/* Message describing the last error on this thread ("" if none). */
+const char* pm_last_error(void);
+
+/* Library version string. */
+const char* pm_version(void);
+
+/* FP64 homography setup from camera->world poses (R row-major 3x3, T 3-vector) and K
+ * (row-major 3x3), rounded to float as the reference passes it to ad_calc_cost.
+ * Reference: motion_stereo/stereo_mapper.cpp:79-90. */
+pm_status pm_homography_setup(const double K[9], const double R_l[9], const double T_l[3],
+                              const double R_r[9], const double T_r[3], pm_homography* out);
+
+/* Builds the quad texture of an image (border 0).  Replaces the texture binding of
+ * ad_calc_cost (motion_stereo/calc_cost.cu:221-244) and the float upload of
+ * StereoMapper::update (stereo_mapper.cpp:74-77). */
+pm_status pm_make_quad(const uint8_t* image, uint32_t* quad, int32_t width, int32_t height,
+                       pm_stream stream);
+
+/* Plane-sweep matching cost fused over all measurement frames of one keyframe.
+ * Replaces num_sources calls of ad_calc_cost (motion_stereo/calc_cost.h:13-20,
+ * calc_cost.cu:11-253) on a volume reset to -1 (stereo_mapper.cpp:69-70).
+ * idep of plane d = idep_min + d * dep_sample (the reference has idep_min = 0).
+ * src_quads and homs are HOST arrays of num_sources entries (the quads are device buffers).
+ * cost: device [H][W][D] of cost_type; -1 where no source saw the (pixel, plane). */
+pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t num_sources,
+                       const uint32_t* const* src_quads, const pm_homography* homs,
+                       float idep_min, float dep_sample, void* cost, pm_cost_type cost_type,
+                       pm_stream stream);
+
+/* Multi-path semi-global aggregation into agg (fully overwritten; no pre-zeroing needed).
+ * Replaces sgm2 (motion_stereo/sgm.h:6-7, sgm.cu:5-135) on a zeroed output
+ * (stereo_mapper.cpp:118-120). */
+pm_status pm_sgm(pm_shape shape, const uint8_t* ref, const void* cost, pm_cost_type cost_type,
+                 pm_sgm_params params, float* agg, pm_stream stream);
+
+/* Winner-take-all, rejection rules and parabola subpixel fit -> depth [H][W].
+ * Replaces filter_cost (motion_stereo/filter_cost.h:7-8, filter_cost.cu:3-70); depth is a
+ * dense [H][W] map instead of a pitched one.  depth = 1 / (idep_min + s * dep_sample). */
+pm_status pm_filter_cost(pm_shape shape, const float* agg, float var_scale, float idep_min,
+                         float dep_sample, float* depth, pm_stream stream);
+
+/* Winner-take-all over a cost volume without rejection rules or subpixel fit: the CASE 0
+ * (raw cost) and CASE 1 (aggregate) outputs of StereoMapper::outputFusionCPU
+ * (motion_stereo/stereo_mapper.cpp:628-706), computed on the GPU instead of after a
+ * download.  Per pixel: m = argmin (lowest index on ties, starting from 100000);
+ * min < FLOAT_EPS -> 10000, m at the first or last plane -> 0, else 1/(idep_min + m*dep_sample). */
+pm_status pm_wta(pm_shape shape, const void* cost, pm_cost_type cost_type, float idep_min,
+                 float dep_sample, float* depth, pm_stream stream);
+
+/* Optional 3x3 median over valid depths (0 < z < 1000).  No reference counterpart: the
+ * median stage of the 752x480 workload.  in and out must not alias. */
+pm_status pm_median3x3(const float* in, float* out, int32_t width, int32_t height,
+                       pm_stream stream);
+
+/* One Gaussian x Beta fusion step.  Replaces depth_fuse (motion_stereo/depth_fusion.h:13-20,
+ * depth_fusion.cu:8-280) and the convert calls around it (stereo_mapper.cpp:149-152,185-193).
+ *   depth  : [H][W] filtered depth in, fused depth out (in place, like tmpDepthMap)
+ *   zbuf   : [H][W] uint64 workspace (z-buffer; no initialisation needed)
+ *   table  : optional [H][W] int32 output of the filled propagation table (NULL to skip)
+ *   pre    : previous state; mu and sig of moved pixels are rewritten in place
+ *   cur    : current state
+ *   r_p2c, t_p2c : previous->current camera transform (stereo_mapper.cpp:155-156), row-major */
+pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam, const float r_p2c[9],
+                        const float t_p2c[3], float* depth, uint64_t* zbuf, int32_t* table,
+                        pm_fusion_state pre, pm_fusion_state cur, pm_stream stream);
+
+/* Fills n floats with value (used for the state resets of outputFusion,
+ * stereo_mapper.cpp:143-144). */
+pm_status pm_fill_f32(float* dst, size_t n, float value, pm_stream stream);
+
+/* ---- StereoMapper handle (motion_stereo/stereo_mapper.h:26-63) ------------------------
+ * The C++ class lives in include/pm/stereo_mapper.h; these wrappers expose it to C and
+ * Python.  Images and results are HOST arrays; the mapper owns its device buffers. */
+typedef struct pm_mapper pm_mapper;
+
+pm_status pm_mapper_create(int32_t width, int32_t height, int32_t planes, pm_mapper** out);
+void pm_mapper_destroy(pm_mapper* m);
+/* StereoMapper::initIntrinsic, stereo_mapper.cpp:51-61 (bf is unused, as in the reference) */
+pm_status pm_mapper_init_intrinsic(pm_mapper* m, const double K[9], float bf, float dep_sample);
+/* StereoMapper::initReference, stereo_mapper.cpp:63-71 */
+pm_status pm_mapper_init_reference(pm_mapper* m, const uint8_t* image);
+/* StereoMapper::update, stereo_mapper.cpp:73-91 */
+pm_status pm_mapper_update(pm_mapper* m, const uint8_t* image, const double R_l[9],
+                           const double T_l[3], const double R_r[9], const double T_r[3]);
+/* StereoMapper::output, stereo_mapper.cpp:93-109; result: host float [H][W] */
+pm_status pm_mapper_output(pm_mapper* m, float* result);
+/* StereoMapper::outputFusion, stereo_mapper.cpp:111-244; result: host float [H][W] holds the
+ * filtered (un-fused) depth, as in the reference; the fused depth is the "fuse_depth" buffer. */
+pm_status pm_mapper_output_fusion(pm_mapper* m, float* result, const double R[9],
+                                  const double t[3], float fx, float fy, float cx, float cy);
+/* StereoMapper::outputFusionCPU CASE modes 0-2, stereo_mapper.cpp:625-719 (WTA on the raw
+ * cost, WTA on the aggregate, filtered depth); result: host float [H][W]. */
+pm_status pm_mapper_output_case(pm_mapper* m, int32_t case_mode, float* result);
+/* Copies a member buffer to host memory.  Names: "raw_cost", "sgm_cost" (float [H][W][D],
+ * raw_cost converted from its storage type), "dep", "fuse_depth", "table" (int32),
+ * "alpha", "beta", "mu", "sig" (the current state after the last fusion step),
+ * "img_l" (uint8).  bytes must equal the buffer size. */
+pm_status pm_mapper_download(pm_mapper* m, const char* name, void* host, size_t bytes);
+/* Storage type of the cost volume (default PM_COST_F32).  Takes effect at the next
+ * initReference. */
+pm_status pm_mapper_set_cost_type(pm_mapper* m, pm_cost_type type);
+
+/* ---- knobs (motion_stereo/parameters.cpp:10-18, plus additive ones) ------------------- */
+typedef struct pm_knobs {
+  float pi1, pi2, tau_so, sgm_q1, sgm_q2, var_scale;
+  int32_t case_mode;   /* CASE */
+  float idep_min;      /* additive: first plane's inverse depth (reference: 0) */
+  int32_t sgm_paths;   /* additive: 4 (reference) or 8 */
+  int32_t median;      /* additive: 1 = 3x3 median after the filter (reference: 0) */
+} pm_knobs;
+void pm_get_knobs(pm_knobs* out);
+void pm_set_knobs(const pm_knobs* in);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PM_CAPI_H */

@@ -1,0 +1,410 @@
+// pm_oracle.hpp -- CPU restatement of the reference's per-keyframe depth pipeline.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke check and the
+// cpu_baseline leg of bench.py may call it; the product (probabilistic_mapping_amd/) never does.
+//
+// One templated source serves two purposes:
+//   T = double : the FP64 oracle that GPU results are compared against;
+//   T = float  : the C++/OpenMP CPU baseline timed beside the GPU.
+//
+// Every stage follows a reference file; citations are path:line inside
+// HKUST-Aerial-Robotics/probabilistic_mapping.
+//
+// Layouts (identical to the GPU library, and to the reference's INDEX macro,
+// motion_stereo/cuda_utils.h:34):
+//   images        : uint8, row-major [H][W]
+//   cost / agg    : pixel-major [H][W][D] (D contiguous)
+//   depth, state  : row-major [H][W] float
+//
+// Deliberate deviations from the CUDA reference (all documented in DESIGN.md):
+//   * bilinear sampling is exact (the reference's tex2D uses hardware fixed-point weights);
+//   * a non-finite warp (0/0) counts as invalid (the reference's comparisons let NaN through);
+//   * min reductions are exact for every D (the reference's tree skips elements for
+//     non-power-of-two D, never reached there since DEP_CNT = 64);
+//   * WTA ties go to the lowest plane index;
+//   * the fusion z-buffer and hole filling are deterministic: the nearest transformed
+//     depth wins (ties: lowest source index; source pixel 0, the empty sentinel, never
+//     inserts), and holes are filled from the pre-fill table, taking the
+//     lowest-linear-index interior neighbour.  Each is an admissible outcome of the
+//     reference's racy kernels (depth_fusion.cu:54-77, 91-104).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace pmo {
+
+constexpr double kFloatEps = 1e-5;    // FLOAT_EPS, motion_stereo/parameters.h:6
+constexpr double kDepInf = 1000.0;    // DEP_INF, parameters.h:28
+constexpr double kCostInf = 0.0;      // COST_INF, parameters.h:29
+
+struct Knobs {                        // runtime globals, motion_stereo/parameters.cpp:10-18
+  double pi1 = 8, pi2 = 24, tau_so = 4, sgm_q1 = 2, sgm_q2 = 3, var_scale = 1.02;
+  int paths = 4;                      // 4 = the reference's set (sgm.cu:119-131); 8 adds diagonals
+};
+
+// Scanline directions in aggregation order.  0-3 are the reference's four paths
+// (sgm.cu:119-131: L->R, R->L, T->B, B->T); 4-7 are the diagonal extension.
+constexpr int kPathDx[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+constexpr int kPathDy[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+
+// Texel read at an integer position with border addressing (cudaAddressModeBorder,
+// calc_cost.cu:230-231): outside the image reads 0.  tex2D at (x+0.5, y+0.5) with linear
+// filtering lands exactly on texel (x, y).
+template <typename T>
+inline T texel(const uint8_t* img, int W, int H, long x, long y) {
+  if (x < 0 || y < 0 || x >= W || y >= H) return T(0);
+  return T(img[y * (long)W + x]);
+}
+
+// Bilinear sample at continuous texel coordinates (U, V): tex2D(tex, U+0.5, V+0.5) with
+// cudaFilterModeLinear and border 0 (calc_cost.cu:93,230-237), with exact weights.
+template <typename T>
+inline T bilinear(const uint8_t* img, int W, int H, T U, T V) {
+  T fx = std::floor(U), fy = std::floor(V);
+  long x0 = (long)fx, y0 = (long)fy;
+  T ax = U - fx, ay = V - fy;
+  T i00 = texel<T>(img, W, H, x0, y0), i10 = texel<T>(img, W, H, x0 + 1, y0);
+  T i01 = texel<T>(img, W, H, x0, y0 + 1), i11 = texel<T>(img, W, H, x0 + 1, y0 + 1);
+  T top = i00 + ax * (i10 - i00);
+  T bot = i01 + ax * (i11 - i01);
+  return top + ay * (bot - top);
+}
+
+// One measurement frame: h = row-major 3x3 K R_r^T R_l K^-1 and t = K R_r^T (T_l - T_r)
+// (stereo_mapper.cpp:79-80), already rounded to float as the reference passes them
+// (stereo_mapper.cpp:82-90); the oracle widens them to T.
+struct Source {
+  const uint8_t* img;
+  double h[9];
+  double t[3];
+};
+
+// Plane-sweep cost, fused over all sources.  Follows ADCalcCostKernel (calc_cost.cu:11-210)
+// and the running mean of ad_calc_cost calls (calc_cost.cu:200-207, stereo_mapper.cpp:73-91).
+//   * 9 warps per (pixel, plane), with the reference's exact warp points, including the
+//     garbled right / up-right triples (calc_cost.cu:50-51, 70-71) and the swapped up/down
+//     reference samples (calc_cost.cu:106, 119);
+//   * a source contributes only if all 9 warps are valid (the `continue`s, calc_cost.cu:89-91);
+//   * the result is the mean over contributing sources of (sum of 9 |diffs|)/9, or -1 if no
+//     source contributed (initReference sets -1, stereo_mapper.cpp:69).
+// idep_d = idep_min + d * dep_sample (calc_cost.cu:81 with idep_min = 0).
+template <typename T>
+void calc_cost(int W, int H, int D, const uint8_t* ref, const Source* src, int S,
+               T idep_min, T dep_sample, T* cost /* [H][W][D] */) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int v = 0; v < H; ++v) {
+    std::vector<T> sum(D), cnt(D);
+    for (int u = 0; u < W; ++u) {
+      std::fill(sum.begin(), sum.end(), T(0));
+      std::fill(cnt.begin(), cnt.end(), T(0));
+      // reference samples (calc_cost.cu:93-198)
+      const T r[9] = {texel<T>(ref, W, H, u, v),         texel<T>(ref, W, H, u, v + 1),
+                      texel<T>(ref, W, H, u, v - 1),     texel<T>(ref, W, H, u - 1, v),
+                      texel<T>(ref, W, H, u + 1, v),     texel<T>(ref, W, H, u - 1, v - 1),
+                      texel<T>(ref, W, H, u + 1, v + 1), texel<T>(ref, W, H, u - 1, v + 1),
+                      texel<T>(ref, W, H, u + 1, v - 1)};
+      for (int s = 0; s < S; ++s) {
+        const Source& sc = src[s];
+        const T h11 = T(sc.h[0]), h12 = T(sc.h[1]), h13 = T(sc.h[2]);
+        const T h21 = T(sc.h[3]), h22 = T(sc.h[4]), h23 = T(sc.h[5]);
+        const T h31 = T(sc.h[6]), h32 = T(sc.h[7]), h33 = T(sc.h[8]);
+        const T t1 = T(sc.t[0]), t2 = T(sc.t[1]), t3 = T(sc.t[2]);
+        const T x = h11 * u + h12 * v + h13;   // calc_cost.cu:28-30
+        const T y = h21 * u + h22 * v + h23;
+        const T z = h31 * u + h32 * v + h33;
+        const T xr = x + h11;
+        // warp triples, calc_cost.cu:34-71
+        const T X[9] = {x, x - h12, x + h12, x - h11, xr, x - h12 - h11, x + h12 + h11,
+                        x - h11 + h12, xr - h12};
+        const T Y[9] = {y, y - h22, y + h22, y - h21, x + h21, y - h22 - h21, y + h22 + h21,
+                        y - h21 + h22, xr - h22};
+        const T Z[9] = {z, z - h32, z + h32, z - h31, x + h31, z - h32 - h31, z + h32 + h31,
+                        z - h31 + h32, xr - h32};
+        for (int d = 0; d < D; ++d) {
+          const T idep = idep_min + T(d) * dep_sample;
+          T tmp = T(0);
+          bool ok = true;
+          for (int k = 0; k < 9 && ok; ++k) {
+            const T w = Z[k] + t3 * idep;
+            const T U = (X[k] + t1 * idep) / w;
+            const T V = (Y[k] + t2 * idep) / w;
+            // validity, calc_cost.cu:89 (written positively so NaN is invalid)
+            ok = (w >= T(0)) && (U >= T(0)) && (U <= T(W - 1)) && (V >= T(0)) &&
+                 (V <= T(H - 1));
+            if (ok) tmp += std::fabs(r[k] - bilinear<T>(sc.img, W, H, U, V));
+          }
+          if (ok) {
+            sum[d] += tmp / T(9);
+            cnt[d] += T(1);
+          }
+        }
+      }
+      T* out = cost + ((long)v * W + u) * D;
+      for (int d = 0; d < D; ++d) out[d] = cnt[d] > T(0) ? sum[d] / cnt[d] : T(-1);
+    }
+  }
+}
+
+// Enumerate the scanlines of path direction (dx, dy): the start pixels p with p-(dx,dy)
+// outside the image.  Returns the count; start_of(i) gives the i-th start.
+inline long scanline_count(int W, int H, int dx, int dy) {
+  if (dx != 0 && dy != 0) return (long)W + H - 1;
+  return dx != 0 ? H : W;
+}
+inline void scanline_start(int W, int H, int dx, int dy, long i, int* x, int* y) {
+  if (dx != 0) {
+    const int xs = dx > 0 ? 0 : W - 1;
+    if (i < H) { *x = xs; *y = (int)i; return; }
+    const long j = i - H;                      // columns other than xs along the entry row
+    *x = dx > 0 ? (int)(1 + j) : (int)j;
+    *y = dy > 0 ? 0 : H - 1;
+    return;
+  }
+  *x = (int)i;
+  *y = dy > 0 ? 0 : H - 1;
+}
+
+// Semi-global aggregation.  Follows sgm2<> (sgm.cu:5-113) and its host driver
+// (sgm.cu:115-135), with the path set extended to 8 directions.
+//   * a pixel is invalid if any plane's cost is < 0 (sgm.cu:24-38, 55-71); there the
+//     recurrence uses cost 0 and the aggregate is set (not added) to 0 on every path;
+//   * L(p,d) = C'(p,d) + min(L(q,d), L(q,d+-1) + P1, min L(q,.) + P2) - min L(q,.)
+//     (sgm.cu:85-98), with P1, P2 multiplied by q1, q2 where |I(p) - I(q)| < tau_so
+//     (sgm.cu:75-82);
+//   * the aggregate is the sum of L over the paths, accumulated in path order into a
+//     buffer that starts at 0 (stereo_mapper.cpp:118).
+template <typename T>
+void sgm(int W, int H, int D, const uint8_t* ref, const T* cost, const Knobs& kn,
+         T* agg /* [H][W][D] */) {
+  const long N = (long)W * H;
+  std::vector<uint8_t> invalid(N);
+#pragma omp parallel for
+  for (long p = 0; p < N; ++p) {
+    T m = cost[p * D];
+    for (int d = 1; d < D; ++d) m = std::min(m, cost[p * D + d]);
+    invalid[p] = m < T(0);
+  }
+  std::fill(agg, agg + N * D, T(0));
+  const T inf = std::numeric_limits<T>::infinity();
+  for (int path = 0; path < kn.paths; ++path) {
+    const int dx = kPathDx[path], dy = kPathDy[path];
+    const long nlines = scanline_count(W, H, dx, dy);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long li = 0; li < nlines; ++li) {
+      std::vector<T> prev(D), cur(D);
+      int x, y;
+      scanline_start(W, H, dx, dy, li, &x, &y);
+      long p = (long)y * W + x;
+      for (int d = 0; d < D; ++d) {
+        prev[d] = invalid[p] ? T(kCostInf) : cost[p * D + d];
+        agg[p * D + d] = invalid[p] ? T(kCostInf) : agg[p * D + d] + prev[d];
+      }
+      for (int k = 1;; ++k) {
+        const int px = x + k * dx, py = y + k * dy;
+        if (px < 0 || py < 0 || px >= W || py >= H) break;
+        const long q = p;
+        p = (long)py * W + px;
+        T m = inf;
+        for (int d = 0; d < D; ++d) m = std::min(m, prev[d]);
+        const T D1 = std::fabs(T(ref[p]) - T(ref[q]));
+        T P1 = T(kn.pi1), P2 = T(kn.pi2);
+        if (D1 < T(kn.tau_so)) {
+          P1 *= T(kn.sgm_q1);
+          P2 *= T(kn.sgm_q2);
+        }
+        for (int d = 0; d < D; ++d) {
+          T val = std::min(prev[d], m + P2);
+          if (d > 0) val = std::min(val, prev[d - 1] + P1);
+          if (d + 1 < D) val = std::min(val, prev[d + 1] + P1);
+          const T c = invalid[p] ? T(kCostInf) : cost[p * D + d];
+          cur[d] = c + val - m;
+          agg[p * D + d] = invalid[p] ? T(kCostInf) : agg[p * D + d] + cur[d];
+        }
+        std::swap(prev, cur);
+      }
+    }
+  }
+}
+
+// Winner-take-all, rejection rules and parabola subpixel fit.  Follows filterCostKernel
+// (filter_cost.cu:3-55):
+//   rule 1: |min| < FLOAT_EPS -> DEP_INF (filter_cost.cu:37-38)
+//   rule 2: argmin at the first or last plane -> 0 (:39-40)
+//   rule 3: c[m-1] + c[m+1] < 2 c[m] var_scale -> 0 (:41-42)
+//   else  : s = m - (c+ - c-) / (2 (c- - 2 c0 + c+)), depth = 1 / (idep_min + s dep_sample)
+//           (:45-51, with idep_min = 0 in the reference)
+template <typename T>
+void filter_cost(int W, int H, int D, const T* agg, T var_scale, T idep_min, T dep_sample,
+                 T* depth) {
+  const long N = (long)W * H;
+#pragma omp parallel for
+  for (long p = 0; p < N; ++p) {
+    const T* c = agg + p * D;
+    int mi = 0;
+    for (int d = 1; d < D; ++d)
+      if (c[d] < c[mi]) mi = d;  // lowest index wins ties
+    const T c0 = c[mi];
+    T out;
+    if (c0 < T(kFloatEps) && c0 > T(-kFloatEps)) {
+      out = T(kDepInf);
+    } else if (mi == 0 || mi == D - 1) {
+      out = T(0);
+    } else if (c[mi - 1] + c[mi + 1] < T(2) * c0 * var_scale) {
+      out = T(0);
+    } else {
+      const T a = c[mi - 1] - T(2) * c0 + c[mi + 1];
+      const T b = -c[mi - 1] + c[mi + 1];
+      const T s = T(mi) - b / (T(2) * a);
+      out = T(1) / (idep_min + s * dep_sample);
+    }
+    depth[p] = out;
+  }
+}
+
+// Optional 3x3 median over valid depths (0 < z < DEP_INF).  Not in the reference; it is the
+// "median/consistency filter" of the 752x480 workload.  A pixel with a valid depth takes the
+// lower median of the valid depths in its 3x3 window (itself included); other pixels pass.
+template <typename T>
+void median3x3(int W, int H, const T* in, T* out) {
+#pragma omp parallel for
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const T z = in[(long)y * W + x];
+      if (!(z > T(0) && z < T(kDepInf))) { out[(long)y * W + x] = z; continue; }
+      T v[9];
+      int n = 0;
+      for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) {
+          const int xx = x + i, yy = y + j;
+          if (xx < 0 || yy < 0 || xx >= W || yy >= H) continue;
+          const T w = in[(long)yy * W + xx];
+          if (w > T(0) && w < T(kDepInf)) v[n++] = w;
+        }
+      std::sort(v, v + n);
+      out[(long)y * W + x] = v[(n - 1) / 2];
+    }
+}
+
+// Gaussian x Beta filter state, one plane per quantity, row-major [H][W].
+template <typename T>
+struct State {
+  T *alpha, *beta, *mu, *sig;
+};
+
+// normpdf, depth_fusion.cu:3-6
+template <typename T>
+inline T normpdf(T x, T mu, T sigma_sq) {
+  return std::exp(-(x - mu) * (x - mu) / (T(2) * sigma_sq)) /
+         std::sqrt(T(2) * T(M_PI) * sigma_sq);
+}
+
+// One fusion step, depth_fuse (depth_fusion.cu:229-280): transformMap (:8-78),
+// hole_filling (:80-105), depth_predict (:107-137), depthUpdate (:139-225).
+//   depth  : [N] measurement in, fused depth out (in place, like tmpDepthMap)
+//   table  : [N] int32 scratch; after the call it holds the filled propagation table
+//            (linear source index, 0 = empty; source pixel 0 can never propagate)
+//   pre    : previous state; transformMap rewrites mu and sig of moved pixels in place
+//   cur    : current state (written)
+// r, t    : R_p2c and t_p2c (stereo_mapper.cpp:155-156), rounded to float by the caller.
+template <typename T>
+void depth_fuse(int W, int H, T fx, T fy, T cx, T cy, const double* r, const double* t,
+                T* depth, int32_t* table, State<T> pre, State<T> cur) {
+  const long N = (long)W * H;
+  const T r11 = T(r[0]), r12 = T(r[1]), r13 = T(r[2]), r21 = T(r[3]), r22 = T(r[4]),
+          r23 = T(r[5]), r31 = T(r[6]), r32 = T(r[7]), r33 = T(r[8]);
+  const T t1 = T(t[0]), t2 = T(t[1]), t3 = T(t[2]);
+  // z-buffer: smallest transformed depth, ties -> lowest source index
+  std::vector<long> win(N, -1);
+  std::vector<T> winz(N);
+  for (long i = 0; i < N; ++i) {
+    const T a = pre.alpha[i];
+    if (a < T(kFloatEps)) continue;                           // :26
+    const T b = pre.beta[i];
+    if (double(a / (a + b)) < 0.4) continue;                  // :30
+    const int x = int(i % W), y = int(i / W);
+    const T dep = pre.mu[i];
+    const T px = (T(x) - cx) / fx * dep, py = (T(y) - cy) / fy * dep, pz = dep;
+    const T px2 = r11 * px + r12 * py + r13 * pz + t1;
+    const T py2 = r21 * px + r22 * py + r23 * pz + t2;
+    const T pz2 = r31 * px + r32 * py + r33 * pz + t3;
+    if (pz2 < T(kFloatEps)) continue;                         // :40
+    const int x2 = int(double(px2 / pz2 * fx + cx) + 0.5);    // :43-44, truncation
+    const int y2 = int(double(py2 / pz2 * fy + cy) + 0.5);
+    if (x2 < 0 || x2 >= W || y2 < 0 || y2 >= H) continue;     // :45
+    pre.mu[i] = pz2;                                          // :48-49
+    pre.sig[i] = T(double(pre.sig[i]) + double(pz2) * 0.05);
+    // source pixel 0 is the table's empty sentinel and never inserts (:54, :61)
+    const long j = (long)y2 * W + x2;
+    if (i != 0 && (win[j] < 0 || pz2 < winz[j])) { win[j] = i; winz[j] = pz2; }
+  }
+  std::vector<int32_t> snap(N);
+  for (long j = 0; j < N; ++j) snap[j] = win[j] < 0 ? 0 : int32_t(win[j]);
+  // hole filling from the pre-fill table: an empty pixel takes the entry of its
+  // lowest-linear-index non-empty interior 3x3 neighbour (:87-104)
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const long j = (long)y * W + x;
+      int32_t v = snap[j];
+      if (v == 0) {
+        for (int dy = -1; dy <= 1 && v == 0; ++dy)
+          for (int dx = -1; dx <= 1 && v == 0; ++dx) {
+            const int nx = x + dx, ny = y + dy;
+            if (nx <= 0 || nx >= W - 1 || ny <= 0 || ny >= H - 1) continue;
+            v = snap[(long)ny * W + nx];
+          }
+      }
+      table[j] = v;
+    }
+  // predict (:121-136) and update (:150-224)
+#pragma omp parallel for
+  for (long j = 0; j < N; ++j) {
+    const int32_t src = table[j];
+    if (src == 0) {
+      cur.alpha[j] = T(-1);
+    } else {
+      cur.alpha[j] = pre.alpha[src];
+      cur.beta[j] = pre.beta[src];
+      cur.mu[j] = pre.mu[src];
+      cur.sig[j] = pre.sig[src];
+    }
+    const T dest = depth[j];
+    const T unc = T(double(dest * dest) * 0.01);
+    if (cur.alpha[j] < T(kFloatEps)) {
+      if (dest < T(kFloatEps)) {
+        depth[j] = T(0);
+      } else {
+        cur.alpha[j] = T(10); cur.beta[j] = T(10); cur.mu[j] = dest; cur.sig[j] = T(2500);
+      }
+      continue;
+    }
+    const T a = cur.alpha[j], b = cur.beta[j];
+    if (dest < T(kFloatEps)) {
+      depth[j] = double(a / (a + b)) > 0.6 ? cur.mu[j] : T(0);
+      continue;
+    }
+    const T miu = cur.mu[j], sq = cur.sig[j];
+    const T new_sq = unc * sq / (unc + sq);
+    const T new_miu = (dest * sq + miu * unc) / (unc + sq);
+    T c1 = (a / (a + b)) * normpdf<T>(dest, miu, unc + sq);
+    T c2 = (b / (a + b)) * T(1) / T(50);
+    const T nc = c1 + c2;
+    c1 = c1 / nc;
+    c2 = c2 / nc;
+    const T f = c1 * ((a + T(1)) / (a + b + T(1))) + c2 * (a / (a + b + T(1)));
+    const T e = c1 * (((a + T(1)) * (a + T(2))) / ((a + b + T(1)) * (a + b + T(2)))) +
+                c2 * (a * (a + T(1)) / ((a + b + T(1)) * (a + b + T(2))));
+    const T mu_p = c1 * new_miu + c2 * miu;
+    const T sig_p = c1 * (new_sq + new_miu * new_miu) + c2 * (sq + miu * miu) - mu_p * mu_p;
+    const T a_p = (e - f) / (f - e / f);
+    const T b_p = a_p * (T(1) - f) / f;
+    cur.alpha[j] = a_p; cur.beta[j] = b_p; cur.mu[j] = mu_p; cur.sig[j] = sig_p;
+    depth[j] = double(a / (a + b)) > 0.6 ? mu_p : T(0);
+  }
+}
+
+}  // namespace pmo

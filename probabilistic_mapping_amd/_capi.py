@@ -1,0 +1,136 @@
+"""ctypes binding of ``libpm_hip.so`` (declared in ``include/pm_capi.h``).
+
+The library is built in-tree by ``probabilistic_mapping_amd.build``; loading fails loudly if it
+is missing -- there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpm_hip.so")
+
+PM_OK = 0
+PM_ERR_INVALID_ARGUMENT = 1
+PM_ERR_OUT_OF_MEMORY = 2
+PM_ERR_LAUNCH = 3
+PM_ERR_INVALID_STATE = 4
+PM_ERR_DEVICE = 5
+
+PM_COST_F32 = 0
+PM_COST_F16 = 1
+PM_COST_BF16 = 2
+
+PM_MAX_PLANES = 256
+PM_MAX_SOURCES = 16
+
+STATUS_NAMES = {0: "PM_OK", 1: "PM_ERR_INVALID_ARGUMENT", 2: "PM_ERR_OUT_OF_MEMORY",
+                3: "PM_ERR_LAUNCH", 4: "PM_ERR_INVALID_STATE", 5: "PM_ERR_DEVICE"}
+
+
+class PmError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Shape(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("planes", ctypes.c_int32)]
+
+
+class Homography(ctypes.Structure):
+    _fields_ = [("h", ctypes.c_float * 9), ("t", ctypes.c_float * 3)]
+
+
+class SgmParams(ctypes.Structure):
+    _fields_ = [("pi1", ctypes.c_float), ("pi2", ctypes.c_float), ("tau_so", ctypes.c_float),
+                ("sgm_q1", ctypes.c_float), ("sgm_q2", ctypes.c_float), ("paths", ctypes.c_int32)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("fx", ctypes.c_float), ("fy", ctypes.c_float), ("cx", ctypes.c_float),
+                ("cy", ctypes.c_float)]
+
+
+class FusionState(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_void_p), ("beta", ctypes.c_void_p), ("mu", ctypes.c_void_p),
+                ("sig", ctypes.c_void_p)]
+
+
+class Knobs(ctypes.Structure):
+    _fields_ = [("pi1", ctypes.c_float), ("pi2", ctypes.c_float), ("tau_so", ctypes.c_float),
+                ("sgm_q1", ctypes.c_float), ("sgm_q2", ctypes.c_float),
+                ("var_scale", ctypes.c_float), ("case_mode", ctypes.c_int32),
+                ("idep_min", ctypes.c_float), ("sgm_paths", ctypes.c_int32),
+                ("median", ctypes.c_int32)]
+
+
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_i32 = ctypes.c_int32
+
+# name -> (restype, argtypes); every symbol declared in include/pm_capi.h
+SIGNATURES = {
+    "pm_last_error": (ctypes.c_char_p, []),
+    "pm_version": (ctypes.c_char_p, []),
+    "pm_homography_setup": (_i32, [_dp, _dp, _dp, _dp, _dp, ctypes.POINTER(Homography)]),
+    "pm_make_quad": (_i32, [_vp, _vp, _i32, _i32, _vp]),
+    "pm_calc_cost": (_i32, [Shape, _vp, _i32, ctypes.POINTER(_vp), ctypes.POINTER(Homography),
+                            ctypes.c_float, ctypes.c_float, _vp, _i32, _vp]),
+    "pm_sgm": (_i32, [Shape, _vp, _vp, _i32, SgmParams, _vp, _vp]),
+    "pm_filter_cost": (_i32, [Shape, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp,
+                              _vp]),
+    "pm_wta": (_i32, [Shape, _vp, _i32, ctypes.c_float, ctypes.c_float, _vp, _vp]),
+    "pm_median3x3": (_i32, [_vp, _vp, _i32, _i32, _vp]),
+    "pm_depth_fuse": (_i32, [_i32, _i32, Camera, _fp, _fp, _vp, _vp, _vp, FusionState,
+                             FusionState, _vp]),
+    "pm_fill_f32": (_i32, [_vp, ctypes.c_size_t, ctypes.c_float, _vp]),
+    "pm_mapper_create": (_i32, [_i32, _i32, _i32, ctypes.POINTER(_vp)]),
+    "pm_mapper_destroy": (None, [_vp]),
+    "pm_mapper_init_intrinsic": (_i32, [_vp, _dp, ctypes.c_float, ctypes.c_float]),
+    "pm_mapper_init_reference": (_i32, [_vp, _vp]),
+    "pm_mapper_update": (_i32, [_vp, _vp, _dp, _dp, _dp, _dp]),
+    "pm_mapper_output": (_i32, [_vp, _vp]),
+    "pm_mapper_output_fusion": (_i32, [_vp, _vp, _dp, _dp, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_float]),
+    "pm_mapper_output_case": (_i32, [_vp, _i32, _vp]),
+    "pm_mapper_download": (_i32, [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]),
+    "pm_mapper_set_cost_type": (_i32, [_vp, _i32]),
+    "pm_get_knobs": (None, [ctypes.POINTER(Knobs)]),
+    "pm_set_knobs": (None, [ctypes.POINTER(Knobs)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load ``libpm_hip.so`` and bind every entry point (raises if the library is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m "
+                              "probabilistic_mapping_amd.build` (or __graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != PM_OK:
+        raise PmError(status, load().pm_last_error().decode())
+
+
+def last_error() -> str:
+    return load().pm_last_error().decode()
+
+
+def dptr(a) -> ctypes.POINTER(ctypes.c_double):
+    import numpy as np
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)

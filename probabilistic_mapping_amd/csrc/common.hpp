@@ -1,0 +1,97 @@
+// Shared device helpers and host-side error plumbing for the gfx950 depth pipeline.
+#pragma once
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "pm_capi.h"
+
+namespace pm {
+
+constexpr float kFloatEps = 1e-5f;   // FLOAT_EPS, motion_stereo/parameters.h:6
+constexpr float kDepInf = 1000.0f;   // DEP_INF, parameters.h:28
+constexpr float kCostInf = 0.0f;     // COST_INF, parameters.h:29
+constexpr int kWave = 64;
+
+// Scanline directions in aggregation order: 0-3 are the reference's paths
+// (sgm.cu:119-131), 4-7 the diagonal extension.
+constexpr int kPathDx[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+constexpr int kPathDy[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+
+// ---- host error plumbing -------------------------------------------------------------
+void set_error(const std::string& msg);
+void clear_error();
+pm_status fail(pm_status code, const std::string& msg);
+// Checks the last launch on this thread; returns PM_OK or PM_ERR_LAUNCH with a message.
+pm_status check_launch(const char* what);
+
+// FP64 plane-sweep homography H = K R_r^T R_l K^-1, t = K R_r^T (T_l - T_r)
+// (stereo_mapper.cpp:79-80).  Returns false for a singular K or non-finite input.
+bool homography_fp64(const double K[9], const double R_l[9], const double T_l[3],
+                     const double R_r[9], const double T_r[3], double H[9], double t[3]);
+
+// ---- cost storage ---------------------------------------------------------------------
+template <typename CT>
+__device__ __forceinline__ float load_cost(const CT* p);
+template <>
+__device__ __forceinline__ float load_cost<float>(const float* p) { return *p; }
+template <>
+__device__ __forceinline__ float load_cost<__half>(const __half* p) { return __half2float(*p); }
+template <>
+__device__ __forceinline__ float load_cost<__hip_bfloat16>(const __hip_bfloat16* p) {
+  return __bfloat162float(*p);
+}
+
+template <typename CT>
+__device__ __forceinline__ CT store_cost(float v);
+template <>
+__device__ __forceinline__ float store_cost<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ __half store_cost<__half>(float v) { return __float2half_rn(v); }
+template <>
+__device__ __forceinline__ __hip_bfloat16 store_cost<__hip_bfloat16>(float v) {
+  return __float2bfloat16(v);
+}
+
+// ---- cross-lane helpers (wave64, DPP) ---------------------------------------------------
+// Values handled here are non-negative floats (or +inf), whose IEEE bit patterns order like
+// unsigned integers; integer min avoids the NaN-canonicalising moves fminf needs.
+__device__ __forceinline__ float umin(float a, float b) {
+  return __uint_as_float(min(__float_as_uint(a), __float_as_uint(b)));
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t src) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, 0xF, 0xF, false);
+}
+
+// lane i receives lane i-1's value; lane 0 receives `fill`
+__device__ __forceinline__ float wave_shr1(float v, float fill) {
+  return __uint_as_float(dpp_u32<0x138>(__float_as_uint(fill), __float_as_uint(v)));
+}
+// lane i receives lane i+1's value; lane 63 receives `fill`
+__device__ __forceinline__ float wave_shl1(float v, float fill) {
+  return __uint_as_float(dpp_u32<0x130>(__float_as_uint(fill), __float_as_uint(v)));
+}
+
+// Minimum over the 64 lanes of a non-negative float, returned wave-uniform.
+__device__ __forceinline__ float wave_min_nonneg(float v) {
+  uint32_t u = __float_as_uint(v);
+  u = min(u, dpp_u32<0xB1>(u, u));    // quad_perm [1,0,3,2]
+  u = min(u, dpp_u32<0x4E>(u, u));    // quad_perm [2,3,0,1]
+  u = min(u, dpp_u32<0x124>(u, u));   // row_ror:4
+  u = min(u, dpp_u32<0x128>(u, u));   // row_ror:8
+  const uint32_t a = __builtin_amdgcn_readlane(u, 0), b = __builtin_amdgcn_readlane(u, 16);
+  const uint32_t c = __builtin_amdgcn_readlane(u, 32), d = __builtin_amdgcn_readlane(u, 48);
+  return __uint_as_float(min(min(a, b), min(c, d)));
+}
+
+__device__ __forceinline__ float texel_u8(const uint8_t* img, int W, int H, int x, int y) {
+  return (x < 0 || y < 0 || x >= W || y >= H) ? 0.0f : (float)img[(long)y * W + x];
+}
+
+}  // namespace pm

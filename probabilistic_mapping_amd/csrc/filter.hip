@@ -1,0 +1,165 @@
+// Winner-take-all with rejection rules and parabola subpixel fit, and the optional 3x3 median.
+//
+// Reference: filterCostKernel + filter_cost, motion_stereo/filter_cost.cu:3-70.  The reference
+// launches one 64-thread block per pixel and reduces (value, index) pairs through shared memory
+// with log2(D) barriers.  Here one thread owns a pixel and scans its D contiguous aggregate
+// values with 16-byte loads; the strict '<' scan gives the lowest plane index on ties.
+#include "common.hpp"
+
+namespace pm {
+
+__global__ __launch_bounds__(256) void filter_kernel(long N, int D, bool vec4,
+                                                     const float* __restrict__ agg,
+                                                     float var_scale, float idep_min,
+                                                     float dep_sample, float* __restrict__ depth) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const float* c = agg + p * D;
+  float best = c[0];
+  int mi = 0;
+  if (vec4) {
+    const float4* c4 = reinterpret_cast<const float4*>(c);
+    for (int g = 0; g < D / 4; ++g) {
+      const float4 v = c4[g];
+      if (v.x < best) { best = v.x; mi = 4 * g; }
+      if (v.y < best) { best = v.y; mi = 4 * g + 1; }
+      if (v.z < best) { best = v.z; mi = 4 * g + 2; }
+      if (v.w < best) { best = v.w; mi = 4 * g + 3; }
+    }
+  } else {
+    for (int d = 1; d < D; ++d)
+      if (c[d] < best) { best = c[d]; mi = d; }
+  }
+  float out;
+  if ((double)best < 1e-5 && (double)best > -1e-5) {          // rule 1, filter_cost.cu:37-38
+    out = kDepInf;
+  } else if (mi == 0 || mi == D - 1) {                        // rule 2, :39-40
+    out = 0.0f;
+  } else {
+    const float cm = c[mi - 1], cp = c[mi + 1];
+    if (cm + cp < 2.0f * best * var_scale) {                  // rule 3, :41-42
+      out = 0.0f;
+    } else {                                                  // parabola, :45-51
+      const float A = cm - 2.0f * best + cp;
+      const float Bq = -cm + cp;
+      const float s = (float)mi - Bq / (2.0f * A);
+      out = 1.0f / (idep_min + s * dep_sample);
+    }
+  }
+  depth[p] = out;
+}
+
+__device__ __forceinline__ bool valid_depth(float z) { return z > 0.0f && z < kDepInf; }
+
+__global__ __launch_bounds__(256) void median3x3_kernel(const float* __restrict__ in,
+                                                        float* __restrict__ out, int W, int H) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= W) return;
+  const float z = in[(long)y * W + x];
+  if (!valid_depth(z)) { out[(long)y * W + x] = z; return; }
+  float v[9];
+  int n = 0;
+#pragma unroll
+  for (int j = -1; j <= 1; ++j)
+#pragma unroll
+    for (int i = -1; i <= 1; ++i) {
+      const int xx = x + i, yy = y + j;
+      if (xx < 0 || yy < 0 || xx >= W || yy >= H) continue;
+      const float w = in[(long)yy * W + xx];
+      if (valid_depth(w)) v[n++] = w;
+    }
+  // insertion sort of at most 9 values; lower median
+  for (int a = 1; a < n; ++a) {
+    const float t = v[a];
+    int b = a - 1;
+    while (b >= 0 && v[b] > t) { v[b + 1] = v[b]; --b; }
+    v[b + 1] = t;
+  }
+  out[(long)y * W + x] = v[(n - 1) / 2];
+}
+
+// CASE 0 / 1 winner-take-all (stereo_mapper.cpp:637-660, 681-704): the running minimum
+// starts at 100000 (a double) with index 0, so a pixel whose costs are all >= 100000 keeps
+// index 0 (the reference leaves the index uninitialised there).
+template <typename CT>
+__global__ __launch_bounds__(256) void wta_kernel(long N, int D, const CT* __restrict__ cost,
+                                                  float idep_min, float dep_sample,
+                                                  float* __restrict__ depth) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  const CT* c = cost + p * D;
+  double best = 100000.0;
+  int mi = 0;
+  for (int d = 0; d < D; ++d) {
+    const double v = (double)load_cost<CT>(c + d);
+    if (v < best) { best = v; mi = d; }
+  }
+  float out;
+  if (best < 1e-5) out = 10000.0f;
+  else if (mi == 0 || mi == D - 1) out = 0.0f;
+  else out = 1.0f / (idep_min + (float)mi * dep_sample);
+  depth[p] = out;
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" pm_status pm_wta(pm_shape shape, const void* cost, pm_cost_type cost_type,
+                            float idep_min, float dep_sample, float* depth, pm_stream stream) {
+  clear_error();
+  if (!cost || !depth) return fail(PM_ERR_INVALID_ARGUMENT, "pm_wta: null pointer");
+  if (shape.width <= 0 || shape.height <= 0 || shape.width > 65535 || shape.height > 65535)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_wta: bad image size");
+  if (shape.planes <= 0 || shape.planes > PM_MAX_PLANES)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_wta: planes must be in 1..256");
+  const long N = (long)shape.width * shape.height;
+  const unsigned blocks = (unsigned)((N + 255) / 256);
+  hipStream_t st = (hipStream_t)stream;
+  switch (cost_type) {
+    case PM_COST_F32:
+      wta_kernel<float><<<blocks, 256, 0, st>>>(N, shape.planes, (const float*)cost, idep_min,
+                                                dep_sample, depth);
+      break;
+    case PM_COST_F16:
+      wta_kernel<__half><<<blocks, 256, 0, st>>>(N, shape.planes, (const __half*)cost,
+                                                 idep_min, dep_sample, depth);
+      break;
+    case PM_COST_BF16:
+      wta_kernel<__hip_bfloat16><<<blocks, 256, 0, st>>>(
+          N, shape.planes, (const __hip_bfloat16*)cost, idep_min, dep_sample, depth);
+      break;
+    default:
+      return fail(PM_ERR_INVALID_ARGUMENT, "pm_wta: unknown cost type");
+  }
+  return check_launch("wta_kernel");
+}
+
+extern "C" pm_status pm_filter_cost(pm_shape shape, const float* agg, float var_scale,
+                                    float idep_min, float dep_sample, float* depth,
+                                    pm_stream stream) {
+  clear_error();
+  if (!agg || !depth) return fail(PM_ERR_INVALID_ARGUMENT, "pm_filter_cost: null pointer");
+  if (shape.width <= 0 || shape.height <= 0 || shape.width > 65535 || shape.height > 65535)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_filter_cost: bad image size");
+  if (shape.planes <= 0 || shape.planes > PM_MAX_PLANES)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_filter_cost: planes must be in 1..256");
+  const long N = (long)shape.width * shape.height;
+  const bool vec4 = (shape.planes % 4 == 0) && ((uintptr_t)agg % 16 == 0);
+  filter_kernel<<<(unsigned)((N + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      N, shape.planes, vec4, agg, var_scale, idep_min, dep_sample, depth);
+  return check_launch("filter_kernel");
+}
+
+extern "C" pm_status pm_median3x3(const float* in, float* out, int32_t width, int32_t height,
+                                  pm_stream stream) {
+  clear_error();
+  if (!in || !out) return fail(PM_ERR_INVALID_ARGUMENT, "pm_median3x3: null pointer");
+  if (in == out) return fail(PM_ERR_INVALID_ARGUMENT, "pm_median3x3: in and out alias");
+  if (width <= 0 || height <= 0 || width > 65535 || height > 65535)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_median3x3: bad image size");
+  dim3 grid((width + 255) / 256, height);
+  median3x3_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(in, out, width, height);
+  return check_launch("median3x3_kernel");
+}

@@ -1,0 +1,532 @@
+// StereoMapper host shell: owns the device buffers and sequences the stage kernels.
+// Reference: motion_stereo/stereo_mapper.cpp:27-244 (constructor, initIntrinsic,
+// initReference, update, output, outputFusion) and :625-719 (outputFusionCPU CASE 0-2).
+#include "pm/stereo_mapper.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "common.hpp"
+
+namespace pm {
+
+namespace {
+
+void check(pm_status s) {
+  if (s != PM_OK) throw Error(s, pm_last_error());
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw Error(e == hipErrorOutOfMemory ? PM_ERR_OUT_OF_MEMORY : PM_ERR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DeviceMat dalloc(int rows, int cols, int type) {
+  DeviceMat m;
+  m.rows = rows; m.cols = cols; m.type = type; m.step = (size_t)cols * elem_size(type);
+  void* p = nullptr;
+  hip_check(hipMalloc(&p, m.bytes()), "hipMalloc");
+  m.data = static_cast<uint8_t*>(p);
+  return m;
+}
+
+void dfree(DeviceMat& m) {
+  if (m.data) (void)hipFree(m.data);
+  m.data = nullptr;
+}
+
+int storage_type(pm_cost_type t) {
+  return t == PM_COST_F16 ? PM_16F : (t == PM_COST_BF16 ? PM_16BF : CV_32F);
+}
+
+// Host copy of a dense rows x cols image into a device buffer of the same shape.
+void upload_image(const Mat& img, const DeviceMat& dst, hipStream_t st, const char* who) {
+  if (img.empty() || img.rows != dst.rows || img.cols != dst.cols)
+    throw Error(PM_ERR_INVALID_ARGUMENT, std::string(who) + ": image size must be " +
+                                             std::to_string(dst.cols) + "x" +
+                                             std::to_string(dst.rows));
+  if (img.type != CV_8U)
+    throw Error(PM_ERR_INVALID_ARGUMENT, std::string(who) + ": image must be 8-bit grayscale");
+  hip_check(hipMemcpy2DAsync(dst.data, dst.step, img.data, img.step, (size_t)img.cols,
+                             (size_t)img.rows, hipMemcpyHostToDevice, st),
+            "upload");
+}
+
+void mat_to_array(const Mat& m, int rows, int cols, double* out, const char* what) {
+  if (m.rows != rows || m.cols != cols || m.type != CV_64F)
+    throw Error(PM_ERR_INVALID_ARGUMENT, std::string(what) + " must be a " +
+                                             std::to_string(rows) + "x" +
+                                             std::to_string(cols) + " CV_64F matrix");
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < cols; ++j) out[i * cols + j] = m.at<double>(i, j);
+}
+
+}  // namespace
+
+void DeviceMat::download(Mat& m) const {
+  if (m.rows != rows || m.cols != cols || m.type != type) m.create(rows, cols, type);
+  hip_check(hipMemcpy(m.data, data, bytes(), hipMemcpyDeviceToHost), "download");
+}
+
+void DeviceMat::setTo(float v) const {
+  const size_t n = (size_t)rows * cols;
+  if (type == CV_32F) {
+    check(pm_fill_f32(reinterpret_cast<float*>(data), n, v, nullptr));
+    hip_check(hipDeviceSynchronize(), "setTo");
+  } else if (v == 0.0f) {
+    hip_check(hipMemset(data, 0, bytes()), "setTo");
+  } else {
+    throw Error(PM_ERR_INVALID_ARGUMENT, "DeviceMat::setTo: non-zero fill of a non-float buffer");
+  }
+}
+
+StereoMapper::StereoMapper() : StereoMapper(WIDTH, HEIGHT, DEP_CNT) {}
+
+StereoMapper::StereoMapper(int width, int height, int dep_cnt)
+    : pre_index(0), cur_index(1), firstDepth(true), DEP_SAMPLE(0.0f), W_(width), H_(height),
+      D_(dep_cnt) {
+  if (W_ <= 0 || H_ <= 0 || W_ > 65535 || H_ > 65535)
+    throw Error(PM_ERR_INVALID_ARGUMENT, "StereoMapper: bad image size");
+  if (D_ < 1 || D_ > PM_MAX_PLANES)
+    throw Error(PM_ERR_INVALID_ARGUMENT, "StereoMapper: dep_cnt must be in 1..256");
+  hipStream_t st = nullptr;
+  hip_check(hipStreamCreate(&st), "hipStreamCreate");
+  stream_ = st;
+  try {
+    alloc();
+  } catch (...) {
+    release();
+    (void)hipStreamDestroy(st);
+    throw;
+  }
+}
+
+StereoMapper::~StereoMapper() {
+  if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
+  release();
+  if (stream_) (void)hipStreamDestroy((hipStream_t)stream_);
+}
+
+void StereoMapper::alloc() {
+  const long N = (long)W_ * H_;
+  img_l = dalloc(H_, W_, CV_8U);
+  raw_cost_type_ = cost_type_;
+  raw_cost = dalloc(1, (int)(N * D_), storage_type(cost_type_));
+  sgm_cost = dalloc(1, (int)(N * D_), CV_32F);
+  dep = dalloc(H_, W_, CV_32F);
+  fuseDepth = dalloc(H_, W_, CV_32F);
+  propogate_table = dalloc(H_, W_, CV_32S);
+  DeviceMat* st[8] = {&alpha0, &beta0, &mu0, &sigma0, &alpha1, &beta1, &mu1, &sigma1};
+  for (DeviceMat* m : st) *m = dalloc(1, (int)N, CV_32F);
+  zbuf_ = dalloc(1, (int)N, PM_64U);
+  med_tmp_ = dalloc(H_, W_, CV_32F);
+  wta_tmp_ = dalloc(H_, W_, CV_32F);
+  hip_check(hipMemset(propogate_table.data, 0, propogate_table.bytes()), "hipMemset");
+  for (DeviceMat* m : st) hip_check(hipMemset(m->data, 0, m->bytes()), "hipMemset");
+}
+
+void StereoMapper::release() {
+  DeviceMat* all[] = {&img_l, &raw_cost, &sgm_cost, &dep, &fuseDepth, &propogate_table,
+                      &alpha0, &beta0, &mu0, &sigma0, &alpha1, &beta1, &mu1, &sigma1,
+                      &zbuf_, &med_tmp_, &wta_tmp_};
+  for (DeviceMat* m : all) dfree(*m);
+  for (DeviceMat& q : quads_) dfree(q);
+  quads_.clear();
+}
+
+void StereoMapper::set_cost_type(pm_cost_type t) {
+  if (t != PM_COST_F32 && t != PM_COST_F16 && t != PM_COST_BF16)
+    throw Error(PM_ERR_INVALID_ARGUMENT, "set_cost_type: unknown type");
+  cost_type_ = t;
+}
+
+void StereoMapper::synchronize() const {
+  hip_check(hipStreamSynchronize((hipStream_t)stream_), "hipStreamSynchronize");
+}
+
+// stereo_mapper.cpp:51-61 (the depth listing it prints is not reproduced)
+void StereoMapper::initIntrinsic(const Mat& _K1, float bf, float dep_sample) {
+  (void)bf;   // unused by the reference as well
+  double k[9];
+  mat_to_array(_K1, 3, 3, k, "initIntrinsic: K");
+  nK1 = _K1.clone();
+  DEP_SAMPLE = dep_sample;
+}
+
+// stereo_mapper.cpp:63-71: new reference frame; the cost volume restarts at -1
+void StereoMapper::initReference(const Mat& _img_l) {
+  hipStream_t st = (hipStream_t)stream_;
+  upload_image(_img_l, img_l, st, "initReference");
+  if (raw_cost_type_ != cost_type_) {
+    synchronize();
+    dfree(raw_cost);
+    raw_cost = dalloc(1, (int)((long)W_ * H_ * D_), storage_type(cost_type_));
+    raw_cost_type_ = cost_type_;
+  }
+  homs_.clear();
+  have_reference_ = true;
+  cost_ready_ = false;
+  depth_ready_ = false;
+}
+
+// stereo_mapper.cpp:73-91: stage a measurement frame (image + homography)
+void StereoMapper::update(const Mat& _img_r, const Mat& R_l, const Mat& T_l, const Mat& R_r,
+                          const Mat& T_r) {
+  if (!have_reference_)
+    throw Error(PM_ERR_INVALID_STATE, "update: initReference has not been called");
+  if (nK1.empty()) throw Error(PM_ERR_INVALID_STATE, "update: initIntrinsic has not been called");
+  if ((int)homs_.size() >= PM_MAX_SOURCES)
+    throw Error(PM_ERR_INVALID_STATE, "update: at most 16 measurement frames per keyframe");
+  double K[9], rl[9], tl[3], rr[9], tr[3], Hm[9], t[3];
+  mat_to_array(nK1, 3, 3, K, "K");
+  mat_to_array(R_l, 3, 3, rl, "update: R_l");
+  mat_to_array(T_l, 3, 1, tl, "update: T_l");
+  mat_to_array(R_r, 3, 3, rr, "update: R_r");
+  mat_to_array(T_r, 3, 1, tr, "update: T_r");
+  if (!homography_fp64(K, rl, tl, rr, tr, Hm, t))
+    throw Error(PM_ERR_INVALID_ARGUMENT, "update: non-finite pose or singular K");
+  const size_t slot = homs_.size();
+  if (quads_.size() <= slot) quads_.push_back(dalloc(H_, W_, CV_32S));
+  hipStream_t st = (hipStream_t)stream_;
+  // the u8 frame goes through the filtered-depth buffer's memory, which is free until output
+  DeviceMat staging = wta_tmp_;
+  staging.type = CV_8U;
+  staging.step = (size_t)W_;
+  upload_image(_img_r, staging, st, "update");
+  check(pm_make_quad(staging.data, reinterpret_cast<uint32_t*>(quads_[slot].data), W_, H_,
+                     stream_));
+  pm_homography h;
+  for (int k = 0; k < 9; ++k) h.h[k] = (float)Hm[k];
+  for (int k = 0; k < 3; ++k) h.t[k] = (float)t[k];
+  homs_.push_back(h);
+  R.create(3, 3, CV_64F);
+  T.create(3, 1, CV_64F);
+  for (int k = 0; k < 9; ++k) R.at<double>(k / 3, k % 3) = Hm[k];
+  for (int k = 0; k < 3; ++k) T.at<double>(k, 0) = t[k];
+  cost_ready_ = false;
+  depth_ready_ = false;
+  synchronize();   // the caller may reuse its image buffer after update returns
+}
+
+void StereoMapper::ensure_cost() {
+  if (cost_ready_) return;
+  if (!have_reference_)
+    throw Error(PM_ERR_INVALID_STATE, "output: initReference has not been called");
+  std::vector<const uint32_t*> q(homs_.size());
+  for (size_t s = 0; s < homs_.size(); ++s) q[s] = reinterpret_cast<const uint32_t*>(quads_[s].data);
+  check(pm_calc_cost(pm_shape{W_, H_, D_}, img_l.data, (int32_t)homs_.size(), q.data(),
+                     homs_.data(), IDEP_MIN, DEP_SAMPLE, raw_cost.data, raw_cost_type_,
+                     stream_));
+  cost_ready_ = true;
+}
+
+// sgm (stereo_mapper.cpp:118-120) + filter (:125-126) (+ the optional median)
+void StereoMapper::run_sgm_filter() {
+  if (depth_ready_) return;
+  ensure_cost();
+  pm_sgm_params p{pi1, pi2, tau_so, sgm_q1, sgm_q2, SGM_PATHS};
+  const pm_shape sh{W_, H_, D_};
+  check(pm_sgm(sh, img_l.data, raw_cost.data, raw_cost_type_, p,
+               reinterpret_cast<float*>(sgm_cost.data), stream_));
+  float* fout = reinterpret_cast<float*>(MEDIAN_FILTER ? med_tmp_.data : dep.data);
+  check(pm_filter_cost(sh, reinterpret_cast<const float*>(sgm_cost.data), var_scale, IDEP_MIN,
+                       DEP_SAMPLE, fout, stream_));
+  if (MEDIAN_FILTER)
+    check(pm_median3x3(fout, reinterpret_cast<float*>(dep.data), W_, H_, stream_));
+  depth_ready_ = true;
+}
+
+void StereoMapper::compute_depth() { run_sgm_filter(); }
+
+// stereo_mapper.cpp:93-109
+void StereoMapper::output(Mat& result) {
+  run_sgm_filter();
+  synchronize();
+  dep.download(result);
+}
+
+// stereo_mapper.cpp:111-244
+void StereoMapper::outputFusion(Mat& result, Matrix3d Rw, Vector3d tw, float fx, float fy,
+                                float cx, float cy) {
+  run_sgm_filter();
+  hipStream_t st = (hipStream_t)stream_;
+  const size_t N = (size_t)W_ * H_;
+  if (firstDepth) {   // :128-146
+    firstDepth = false;
+    result.create(H_, W_, CV_32F);
+    result.setTo<float>(0.0f);
+    R_b_2_w = Rw;
+    t_b_2_w = tw;
+    pre_index = 0;
+    cur_index = 1;
+    check(pm_fill_f32(reinterpret_cast<float*>(alpha0.data), N, 0.0f, stream_));
+    check(pm_fill_f32(reinterpret_cast<float*>(alpha1.data), N, 0.0f, stream_));
+    synchronize();
+    return;
+  }
+  // :149-152 (dense layouts: the filtered depth is copied, not converted)
+  hip_check(hipMemcpyAsync(fuseDepth.data, dep.data, dep.bytes(), hipMemcpyDeviceToDevice, st),
+            "copy depth");
+  // :155-156, FP64 then cast to float (:167-170)
+  const Matrix3d R_p_2_c = Rw.transpose() * R_b_2_w;
+  const Vector3d t_p_2_c = Rw.transpose() * (t_b_2_w - tw);
+  float r[9], t[3];
+  for (int k = 0; k < 9; ++k) r[k] = (float)R_p_2_c.m[k];
+  for (int k = 0; k < 3; ++k) t[k] = (float)t_p_2_c.v[k];
+  DeviceMat* s0[4] = {&alpha0, &beta0, &mu0, &sigma0};
+  DeviceMat* s1[4] = {&alpha1, &beta1, &mu1, &sigma1};
+  DeviceMat** pre = pre_index == 0 ? s0 : s1;
+  DeviceMat** cur = pre_index == 0 ? s1 : s0;
+  auto state = [](DeviceMat** s) {
+    return pm_fusion_state{reinterpret_cast<float*>(s[0]->data), reinterpret_cast<float*>(s[1]->data),
+                           reinterpret_cast<float*>(s[2]->data), reinterpret_cast<float*>(s[3]->data)};
+  };
+  check(pm_depth_fuse(W_, H_, pm_camera{fx, fy, cx, cy}, r, t,
+                      reinterpret_cast<float*>(fuseDepth.data),
+                      reinterpret_cast<uint64_t*>(zbuf_.data),
+                      reinterpret_cast<int32_t*>(propogate_table.data), state(pre), state(cur),
+                      stream_));
+  synchronize();
+  dep.download(result);   // :229-230 -- the filtered, not the fused, depth
+  R_b_2_w = Rw;            // :237-241
+  t_b_2_w = tw;
+  pre_index = cur_index;
+  cur_index ^= 1;
+}
+
+void StereoMapper::wta(int case_mode, Mat& result) {
+  const pm_shape sh{W_, H_, D_};
+  if (case_mode == 0) {
+    ensure_cost();
+    check(pm_wta(sh, raw_cost.data, raw_cost_type_, IDEP_MIN, DEP_SAMPLE,
+                 reinterpret_cast<float*>(wta_tmp_.data), stream_));
+  } else {
+    ensure_cost();
+    pm_sgm_params p{pi1, pi2, tau_so, sgm_q1, sgm_q2, SGM_PATHS};
+    check(pm_sgm(sh, img_l.data, raw_cost.data, raw_cost_type_, p,
+                 reinterpret_cast<float*>(sgm_cost.data), stream_));
+    check(pm_wta(sh, sgm_cost.data, PM_COST_F32, IDEP_MIN, DEP_SAMPLE,
+                 reinterpret_cast<float*>(wta_tmp_.data), stream_));
+  }
+  synchronize();
+  wta_tmp_.download(result);
+}
+
+// stereo_mapper.cpp:625-719
+void StereoMapper::outputFusionCPU(Mat& result, Matrix3d Rw, Vector3d tw, float fx, float fy,
+                                   float cx, float cy) {
+  if (CASE == 0 || CASE == 1) {
+    wta(CASE, result);
+    return;
+  }
+  if (CASE == 2) {
+    output(result);
+    return;
+  }
+  outputFusion(result, Rw, tw, fx, fy, cx, cy);
+}
+
+}  // namespace pm
+
+// ---- C wrappers ---------------------------------------------------------------------------
+
+struct pm_mapper {
+  pm::StereoMapper* impl;
+};
+
+namespace {
+
+template <typename F>
+pm_status guard(F&& f) {
+  pm::clear_error();
+  try {
+    f();
+    return PM_OK;
+  } catch (const pm::Error& e) {
+    return pm::fail(e.status, e.what());
+  } catch (const std::bad_alloc&) {
+    return pm::fail(PM_ERR_OUT_OF_MEMORY, "host allocation failed");
+  } catch (const std::exception& e) {
+    return pm::fail(PM_ERR_DEVICE, e.what());
+  }
+}
+
+pm::Mat wrap_image(const uint8_t* img, int H, int W) {
+  return pm::Mat(H, W, pm::CV_8U, const_cast<uint8_t*>(img));
+}
+
+pm::Mat wrap_d(const double* v, int r, int c) {
+  pm::Mat m(r, c, pm::CV_64F);
+  std::memcpy(m.data, v, sizeof(double) * r * c);
+  return m;
+}
+
+float half_to_float(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h >> 15) << 31;
+  const int exp = (h >> 10) & 0x1F;
+  const uint32_t man = h & 0x3FF;
+  float f;
+  if (exp == 0) {
+    f = std::ldexp((float)man, -24);
+  } else if (exp == 31) {
+    f = man ? NAN : INFINITY;
+  } else {
+    f = std::ldexp((float)(man | 0x400), exp - 25);
+  }
+  uint32_t bits;
+  std::memcpy(&bits, &f, 4);
+  bits |= sign;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
+}  // namespace
+
+extern "C" pm_status pm_mapper_create(int32_t width, int32_t height, int32_t planes,
+                                      pm_mapper** out) {
+  if (!out) return pm::fail(PM_ERR_INVALID_ARGUMENT, "pm_mapper_create: null out");
+  *out = nullptr;
+  return guard([&] {
+    auto* m = new pm_mapper{nullptr};
+    try {
+      m->impl = new pm::StereoMapper(width, height, planes);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+extern "C" void pm_mapper_destroy(pm_mapper* m) {
+  if (!m) return;
+  delete m->impl;
+  delete m;
+}
+
+#define PM_MAPPER_CHECK(m) \
+  if (!(m) || !(m)->impl) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null mapper")
+
+extern "C" pm_status pm_mapper_init_intrinsic(pm_mapper* m, const double K[9], float bf,
+                                              float dep_sample) {
+  PM_MAPPER_CHECK(m);
+  if (!K) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null K");
+  return guard([&] { m->impl->initIntrinsic(wrap_d(K, 3, 3), bf, dep_sample); });
+}
+
+extern "C" pm_status pm_mapper_init_reference(pm_mapper* m, const uint8_t* image) {
+  PM_MAPPER_CHECK(m);
+  if (!image) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null image");
+  return guard([&] {
+    m->impl->initReference(wrap_image(image, m->impl->height(), m->impl->width()));
+  });
+}
+
+extern "C" pm_status pm_mapper_update(pm_mapper* m, const uint8_t* image, const double R_l[9],
+                                      const double T_l[3], const double R_r[9],
+                                      const double T_r[3]) {
+  PM_MAPPER_CHECK(m);
+  if (!image || !R_l || !T_l || !R_r || !T_r)
+    return pm::fail(PM_ERR_INVALID_ARGUMENT, "null argument");
+  return guard([&] {
+    m->impl->update(wrap_image(image, m->impl->height(), m->impl->width()), wrap_d(R_l, 3, 3),
+                    wrap_d(T_l, 3, 1), wrap_d(R_r, 3, 3), wrap_d(T_r, 3, 1));
+  });
+}
+
+extern "C" pm_status pm_mapper_output(pm_mapper* m, float* result) {
+  PM_MAPPER_CHECK(m);
+  if (!result) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null result");
+  return guard([&] {
+    pm::Mat r(m->impl->height(), m->impl->width(), pm::CV_32F, result);
+    m->impl->output(r);
+  });
+}
+
+extern "C" pm_status pm_mapper_output_fusion(pm_mapper* m, float* result, const double R[9],
+                                             const double t[3], float fx, float fy, float cx,
+                                             float cy) {
+  PM_MAPPER_CHECK(m);
+  if (!result || !R || !t) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null argument");
+  return guard([&] {
+    pm::Matrix3d Rm;
+    for (int k = 0; k < 9; ++k) Rm.m[k] = R[k];
+    pm::Vector3d tv(t[0], t[1], t[2]);
+    pm::Mat r(m->impl->height(), m->impl->width(), pm::CV_32F, result);
+    m->impl->outputFusion(r, Rm, tv, fx, fy, cx, cy);
+  });
+}
+
+extern "C" pm_status pm_mapper_output_case(pm_mapper* m, int32_t case_mode, float* result) {
+  PM_MAPPER_CHECK(m);
+  if (!result) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null result");
+  if (case_mode < 0 || case_mode > 2)
+    return pm::fail(PM_ERR_INVALID_ARGUMENT, "case_mode must be 0, 1 or 2");
+  return guard([&] {
+    pm::Mat r(m->impl->height(), m->impl->width(), pm::CV_32F, result);
+    if (case_mode == 2) m->impl->output(r);
+    else m->impl->wta(case_mode, r);
+  });
+}
+
+extern "C" pm_status pm_mapper_set_cost_type(pm_mapper* m, pm_cost_type type) {
+  PM_MAPPER_CHECK(m);
+  return guard([&] { m->impl->set_cost_type(type); });
+}
+
+extern "C" pm_status pm_mapper_download(pm_mapper* m, const char* name, void* host,
+                                        size_t bytes) {
+  PM_MAPPER_CHECK(m);
+  if (!name || !host) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null argument");
+  return guard([&] {
+    pm::StereoMapper& s = *m->impl;
+    const std::string n(name);
+    const bool cur0 = s.pre_index == 0;   // after a fusion step the new state sits at pre_index
+    const pm::DeviceMat* src = nullptr;
+    if (n == "raw_cost") src = &s.raw_cost;
+    else if (n == "sgm_cost") src = &s.sgm_cost;
+    else if (n == "dep") src = &s.dep;
+    else if (n == "fuse_depth") src = &s.fuseDepth;
+    else if (n == "table") src = &s.propogate_table;
+    else if (n == "img_l") src = &s.img_l;
+    else if (n == "alpha") src = cur0 ? &s.alpha0 : &s.alpha1;
+    else if (n == "beta") src = cur0 ? &s.beta0 : &s.beta1;
+    else if (n == "mu") src = cur0 ? &s.mu0 : &s.mu1;
+    else if (n == "sig") src = cur0 ? &s.sigma0 : &s.sigma1;
+    else if (n == "alpha0") src = &s.alpha0;
+    else if (n == "beta0") src = &s.beta0;
+    else if (n == "mu0") src = &s.mu0;
+    else if (n == "sigma0") src = &s.sigma0;
+    else if (n == "alpha1") src = &s.alpha1;
+    else if (n == "beta1") src = &s.beta1;
+    else if (n == "mu1") src = &s.mu1;
+    else if (n == "sigma1") src = &s.sigma1;
+    else throw pm::Error(PM_ERR_INVALID_ARGUMENT, "unknown buffer name: " + n);
+    s.synchronize();
+    const size_t count = (size_t)src->rows * src->cols;
+    if (src == &s.raw_cost && src->type != pm::CV_32F) {
+      if (bytes != count * sizeof(float))
+        throw pm::Error(PM_ERR_INVALID_ARGUMENT, "raw_cost: bytes must be H*W*D*4");
+      std::vector<uint16_t> tmp(count);
+      pm::hip_check(hipMemcpy(tmp.data(), src->data, count * 2, hipMemcpyDeviceToHost), "download");
+      float* out = static_cast<float*>(host);
+      for (size_t i = 0; i < count; ++i) {
+        if (src->type == pm::PM_16F) {
+          out[i] = half_to_float(tmp[i]);
+        } else {
+          const uint32_t b = (uint32_t)tmp[i] << 16;
+          std::memcpy(&out[i], &b, 4);
+        }
+      }
+      return;
+    }
+    if (bytes != src->bytes())
+      throw pm::Error(PM_ERR_INVALID_ARGUMENT,
+                      n + ": bytes must be " + std::to_string(src->bytes()));
+    pm::hip_check(hipMemcpy(host, src->data, src->bytes(), hipMemcpyDeviceToHost), "download");
+  });
+}

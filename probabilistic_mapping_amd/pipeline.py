@@ -1,0 +1,246 @@
+"""Device-resident stage calls and the keyframe sequence driver.
+
+Every function here enqueues HIP kernels from ``libpm_hip.so`` through the C ABI on torch's
+current stream; torch is used only to own device memory.  Layouts are those of
+``include/pm_capi.h`` (cost / aggregate pixel-major ``[H][W][D]``).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _capi as C
+
+COST_TYPES = {
+    "f32": (C.PM_COST_F32, torch.float32),
+    "f16": (C.PM_COST_F16, torch.float16),
+    "bf16": (C.PM_COST_BF16, torch.bfloat16),
+}
+
+STATE_KEYS = ("alpha", "beta", "mu", "sig")
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: torch.Tensor):
+    assert t.is_cuda and t.is_contiguous(), "device tensors must be contiguous CUDA tensors"
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def homography(K, R_l, T_l, R_r, T_r):
+    """FP64 plane-sweep homography rounded to float (stereo_mapper.cpp:79-90).
+
+    Returns ``(h, t)``: float32 arrays of 9 (row-major) and 3 entries.
+    """
+    lib = C.load()
+    arrs = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1)) for a in
+            (K, R_l, T_l, R_r, T_r)]
+    out = C.Homography()
+    C.check(lib.pm_homography_setup(*[C.dptr(a) for a in arrs], ctypes.byref(out)))
+    return np.array(out.h[:], np.float32), np.array(out.t[:], np.float32)
+
+
+def _hom_array(homs):
+    arr = (C.Homography * max(len(homs), 1))()
+    for i, (h, t) in enumerate(homs):
+        arr[i].h[:] = [float(v) for v in np.asarray(h, np.float32).reshape(9)]
+        arr[i].t[:] = [float(v) for v in np.asarray(t, np.float32).reshape(3)]
+    return arr
+
+
+def make_quad(img: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    """uint8 [H][W] image -> packed bilinear quad texture (int32 [H][W])."""
+    assert img.dtype == torch.uint8 and img.dim() == 2
+    H, W = img.shape
+    if out is None:
+        out = torch.empty((H, W), dtype=torch.int32, device=img.device)
+    C.check(C.load().pm_make_quad(_ptr(img), _ptr(out), W, H, _stream()))
+    return out
+
+
+def calc_cost(ref: torch.Tensor, quads, homs, idep_min: float, dep_sample: float, planes: int,
+              cost_type: str = "f32", out: torch.Tensor | None = None) -> torch.Tensor:
+    """Plane-sweep cost over all sources -> ``[H][W][D]`` of the chosen storage type."""
+    H, W = ref.shape
+    code, dt = COST_TYPES[cost_type]
+    if out is None:
+        out = torch.empty((H, W, planes), dtype=dt, device=ref.device)
+    assert out.dtype == dt and tuple(out.shape) == (H, W, planes)
+    S = len(quads)
+    qarr = (ctypes.c_void_p * max(S, 1))(*[q.data_ptr() for q in quads])
+    C.check(C.load().pm_calc_cost(C.Shape(W, H, planes), _ptr(ref), S, qarr, _hom_array(homs),
+                                  float(idep_min), float(dep_sample), _ptr(out), code, _stream()))
+    return out
+
+
+def _cost_code(cost: torch.Tensor) -> int:
+    for code, dt in COST_TYPES.values():
+        if cost.dtype == dt:
+            return code
+    raise TypeError(f"unsupported cost dtype {cost.dtype}")
+
+
+def sgm(ref: torch.Tensor, cost: torch.Tensor, knobs: dict, paths: int,
+        out: torch.Tensor | None = None) -> torch.Tensor:
+    """Multi-path SGM aggregation -> float32 ``[H][W][D]``."""
+    H, W, D = cost.shape
+    if out is None:
+        out = torch.empty((H, W, D), dtype=torch.float32, device=cost.device)
+    p = C.SgmParams(knobs["pi1"], knobs["pi2"], knobs["tau_so"], knobs["sgm_q1"],
+                    knobs["sgm_q2"], paths)
+    C.check(C.load().pm_sgm(C.Shape(W, H, D), _ptr(ref), _ptr(cost), _cost_code(cost), p,
+                            _ptr(out), _stream()))
+    return out
+
+
+def filter_cost(agg: torch.Tensor, var_scale: float, idep_min: float, dep_sample: float,
+                out: torch.Tensor | None = None) -> torch.Tensor:
+    H, W, D = agg.shape
+    if out is None:
+        out = torch.empty((H, W), dtype=torch.float32, device=agg.device)
+    C.check(C.load().pm_filter_cost(C.Shape(W, H, D), _ptr(agg), float(var_scale),
+                                    float(idep_min), float(dep_sample), _ptr(out), _stream()))
+    return out
+
+
+def wta(cost: torch.Tensor, idep_min: float, dep_sample: float) -> torch.Tensor:
+    H, W, D = cost.shape
+    out = torch.empty((H, W), dtype=torch.float32, device=cost.device)
+    C.check(C.load().pm_wta(C.Shape(W, H, D), _ptr(cost), _cost_code(cost), float(idep_min),
+                            float(dep_sample), _ptr(out), _stream()))
+    return out
+
+
+def median3x3(depth: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    H, W = depth.shape
+    if out is None:
+        out = torch.empty_like(depth)
+    C.check(C.load().pm_median3x3(_ptr(depth), _ptr(out), W, H, _stream()))
+    return out
+
+
+def new_state(H: int, W: int, device) -> dict:
+    return {k: torch.zeros((H, W), dtype=torch.float32, device=device) for k in STATE_KEYS}
+
+
+def _state(s: dict) -> C.FusionState:
+    return C.FusionState(*[s[k].data_ptr() for k in STATE_KEYS])
+
+
+def depth_fuse(depth: torch.Tensor, zbuf: torch.Tensor, pre: dict, cur: dict, r_p2c, t_p2c,
+               cam, table: torch.Tensor | None = None) -> None:
+    """One Gaussian x Beta fusion step, in place on ``depth``, ``pre`` and ``cur``."""
+    H, W = depth.shape
+    r = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(r_p2c, np.float32).reshape(9)])
+    t = (ctypes.c_float * 3)(*[float(v) for v in np.asarray(t_p2c, np.float32).reshape(3)])
+    tab = _ptr(table) if table is not None else ctypes.c_void_p(0)
+    C.check(C.load().pm_depth_fuse(W, H, C.Camera(*[float(c) for c in cam]), r, t, _ptr(depth),
+                                   _ptr(zbuf), tab, _state(pre), _state(cur), _stream()))
+
+
+def pose_p2c(R, t, R_prev, t_prev):
+    """R_p2c = R^T R_prev, t_p2c = R^T (t_prev - t) in FP64, cast to float
+    (stereo_mapper.cpp:155-156, 167-170)."""
+    R = np.asarray(R, np.float64); R_prev = np.asarray(R_prev, np.float64)
+    t = np.asarray(t, np.float64).reshape(3); t_prev = np.asarray(t_prev, np.float64).reshape(3)
+    return (R.T @ R_prev).astype(np.float32), (R.T @ (t_prev - t)).astype(np.float32)
+
+
+class KeyframeSequence:
+    """Device-resident driver of a keyframe sequence (the StereoMapper::outputFusion loop).
+
+    All frames are uploaded once and turned into quad textures; each keyframe then runs
+    cost (over its sources) -> SGM -> filter (-> median) -> fusion on the current stream with
+    no host synchronisation.  The first keyframe only resets the filter state
+    (stereo_mapper.cpp:128-146); later ones fuse into the ping-pong state (:149-243).
+    """
+
+    def __init__(self, images, poses, sources, K, planes, idep_min, idep_max, knobs, paths,
+                 cost_type="f32", median=False, device="cuda"):
+        self.device = torch.device(device)
+        self.H, self.W = np.asarray(images[0]).shape
+        self.D = planes
+        self.idep_min = float(idep_min)
+        self.dep_sample = float((idep_max - idep_min) / (planes - 1))
+        self.knobs = dict(knobs)
+        self.paths = paths
+        self.cost_type = cost_type
+        self.median = median
+        self.K = np.asarray(K, np.float64)
+        self.cam = (self.K[0, 0], self.K[1, 1], self.K[0, 2], self.K[1, 2])
+        self.poses = [(np.asarray(R, np.float64), np.asarray(T, np.float64).reshape(3))
+                      for R, T in poses]
+        self.sources = [list(s) for s in sources]
+        self.images = [torch.from_numpy(np.ascontiguousarray(im)).to(self.device)
+                       for im in images]
+        self.quads = [make_quad(im) for im in self.images]
+        self.homs = []
+        for i, srcs in enumerate(self.sources):
+            Rl, Tl = self.poses[i]
+            self.homs.append([homography(self.K, Rl, Tl, *self.poses[j]) for j in srcs])
+        H, W, D = self.H, self.W, self.D
+        dt = COST_TYPES[cost_type][1]
+        self.cost = torch.empty((H, W, D), dtype=dt, device=self.device)
+        self.agg = torch.empty((H, W, D), dtype=torch.float32, device=self.device)
+        self.dep = torch.empty((H, W), dtype=torch.float32, device=self.device)
+        self.dep_raw = torch.empty((H, W), dtype=torch.float32, device=self.device)
+        self.fused = torch.empty((H, W), dtype=torch.float32, device=self.device)
+        self.zbuf = torch.empty((H, W), dtype=torch.int64, device=self.device)
+        self.table = torch.empty((H, W), dtype=torch.int32, device=self.device)
+        self.states = [new_state(H, W, self.device), new_state(H, W, self.device)]
+        self.reset()
+
+    def reset(self):
+        self.first = True
+        self.pre = 0
+        self.prev_pose = None
+
+    def depth_stage(self, i: int) -> torch.Tensor:
+        """cost -> SGM -> filter (-> median) for keyframe i; returns the filtered depth."""
+        srcs = self.sources[i]
+        calc_cost(self.images[i], [self.quads[j] for j in srcs], self.homs[i], self.idep_min,
+                  self.dep_sample, self.D, self.cost_type, out=self.cost)
+        sgm(self.images[i], self.cost, self.knobs, self.paths, out=self.agg)
+        if self.median:
+            filter_cost(self.agg, self.knobs["var_scale"], self.idep_min, self.dep_sample,
+                        out=self.dep_raw)
+            median3x3(self.dep_raw, out=self.dep)
+        else:
+            filter_cost(self.agg, self.knobs["var_scale"], self.idep_min, self.dep_sample,
+                        out=self.dep)
+        return self.dep
+
+    def fusion_stage(self, i: int) -> None:
+        R, t = self.poses[i]
+        if self.first:
+            self.first = False
+            self.pre = 0
+            for s in self.states:
+                s["alpha"].zero_()
+            self.prev_pose = (R, t)
+            self.fused.zero_()
+            return
+        self.fused.copy_(self.dep)
+        r, tt = pose_p2c(R, t, *self.prev_pose)
+        depth_fuse(self.fused, self.zbuf, self.states[self.pre], self.states[self.pre ^ 1], r,
+                   tt, self.cam, table=self.table)
+        self.prev_pose = (R, t)
+        self.pre ^= 1
+
+    def run_keyframe(self, i: int) -> None:
+        self.depth_stage(i)
+        self.fusion_stage(i)
+
+    def run(self) -> None:
+        self.reset()
+        for i in range(len(self.images)):
+            self.run_keyframe(i)
+
+    @property
+    def state(self) -> dict:
+        """The state written by the last fusion step."""
+        return self.states[self.pre]

@@ -1,0 +1,160 @@
+"""End-to-end parity of the GPU pipeline against the FP64 oracle (needs a GPU).
+
+Bars (north-star acceptance for the plain FP32 form):
+  * WTA plane index identical on >= 99.9 % of pixels;
+  * filtered depth category (rejected / non-overlap / value) identical on >= 99.9 % of pixels;
+  * subpixel depth, fused mean, variance and inlier ratio: relative error <= 1e-4 on the
+    pixels where both sides made the same discrete decisions, judged at the 99.9th
+    percentile (the remaining 0.1 % sit on FP32-vs-FP64 decision boundaries of the
+    parabola's curvature and are bounded separately).
+"""
+import json
+
+import numpy as np
+import pytest
+
+from oracle.sequence import OracleSequence, homography as np_homography
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _report(name, stats):
+    print(f"[parity] {name}: {json.dumps(stats)}", flush=True)
+
+
+def _assert_depth(st):
+    assert st["category_mismatch"] <= 1e-3
+    assert st["rel"]["p999"] <= 1e-4
+
+
+def test_stereo_mapper_small_scene(small_scene):
+    """The drop-in StereoMapper on the 320x240, 2-source, 32-plane, 4-path workload."""
+    from probabilistic_mapping_amd.mapper import StereoMapper, set_knobs
+    import oracle
+    s = small_scene
+    K = s["scene"].K
+    D = s["planes"]
+    ds = (s["idep_max"] - s["idep_min"]) / (D - 1)
+    old = set_knobs(idep_min=s["idep_min"], sgm_paths=4, median=0, **s["knobs"])
+    try:
+        m = StereoMapper(320, 240, D)
+        m.initIntrinsic(K, 0.0, ds)
+        m.initReference(s["ref"])
+        for img, (R, T) in zip(s["srcs"], s["src_poses"]):
+            m.update(img, *s["ref_pose"], R, T)
+        g = m.output()
+        g_cost = m.download("raw_cost")
+        g_agg = m.download("sgm_cost")
+    finally:
+        set_knobs(**old)
+    homs = [np_homography(K, *s["ref_pose"], R, T) for R, T in s["src_poses"]]
+    o_cost = oracle.calc_cost(s["ref"], s["srcs"], homs, s["idep_min"], ds, D, "f64")
+    o_agg = oracle.sgm(s["ref"], o_cost, s["knobs"], 4, "f64")
+    o = oracle.filter_cost(o_agg, s["knobs"]["var_scale"], s["idep_min"], ds, "f64")
+    wta_agree = float((parity.wta_index(g_agg) == parity.wta_index(o_agg)).mean())
+    st = {"wta_agree": wta_agree, "cost": parity.cost_parity(g_cost, o_cost),
+          "depth": parity.depth_parity(g, o)}
+    _report("mapper small", st)
+    assert wta_agree >= 0.999
+    _assert_depth(st["depth"])
+
+
+def test_stereo_mapper_fusion_semantics(small_scene):
+    """outputFusion: first keyframe returns zeros and only resets; later calls return the
+    filtered (un-fused) depth, and the fused depth is a separate buffer."""
+    from probabilistic_mapping_amd.mapper import StereoMapper, set_knobs
+    s = small_scene
+    K = s["scene"].K
+    D = s["planes"]
+    ds = (s["idep_max"] - s["idep_min"]) / (D - 1)
+    old = set_knobs(idep_min=s["idep_min"], sgm_paths=4, median=0, **s["knobs"])
+    try:
+        m = StereoMapper(320, 240, D)
+        m.initIntrinsic(K, 0.0, ds)
+        fx, fy, cx, cy = s["scene"].intrinsics
+        outs = []
+        for k in range(3):
+            m.initReference(s["ref"])
+            for img, (R, T) in zip(s["srcs"], s["src_poses"]):
+                m.update(img, *s["ref_pose"], R, T)
+            outs.append(m.outputFusion(*s["ref_pose"], fx, fy, cx, cy))
+        dep = m.download("dep")
+        fused = m.download("fuse_depth")
+        alpha = m.download("alpha")
+    finally:
+        set_knobs(**old)
+    assert np.all(outs[0] == 0)
+    assert np.array_equal(outs[1], dep) and np.array_equal(outs[2], dep)
+    # identical keyframes at an identical pose: the third call fuses an initialised state
+    assert (alpha >= 1e-5).mean() > 0.3
+    assert not np.array_equal(fused, dep)
+
+
+def test_case_modes(small_scene):
+    """outputFusionCPU CASE 0 / 1 / 2 (stereo_mapper.cpp:628-719)."""
+    from probabilistic_mapping_amd.mapper import StereoMapper, set_knobs
+    import oracle
+    s = small_scene
+    K = s["scene"].K
+    D = s["planes"]
+    ds = (s["idep_max"] - s["idep_min"]) / (D - 1)
+    fx, fy, cx, cy = s["scene"].intrinsics
+    old = set_knobs(idep_min=s["idep_min"], sgm_paths=4, median=0, **s["knobs"])
+    res = {}
+    try:
+        m = StereoMapper(320, 240, D)
+        m.initIntrinsic(K, 0.0, ds)
+        m.initReference(s["ref"])
+        for img, (R, T) in zip(s["srcs"], s["src_poses"]):
+            m.update(img, *s["ref_pose"], R, T)
+        for mode in (0, 1, 2):
+            set_knobs(case_mode=mode)
+            res[mode] = m.outputFusionCPU(*s["ref_pose"], fx, fy, cx, cy)
+        raw = m.download("raw_cost")
+        agg = m.download("sgm_cost")
+        dep = m.download("dep")
+    finally:
+        set_knobs(**old)
+
+    def wta(c):
+        idx = np.argmin(c, axis=2)
+        mn = c.min(axis=2)
+        return np.where(mn < 1e-5, 10000.0, np.where((idx == 0) | (idx == D - 1), 0.0,
+                                                      1.0 / (s["idep_min"] + idx * ds)))
+    assert np.allclose(res[0], wta(raw), rtol=1e-6, atol=0)
+    assert np.allclose(res[1], wta(agg), rtol=1e-6, atol=0)
+    assert np.array_equal(res[2], dep)
+
+
+@pytest.mark.parametrize("cost_type", ["f32"])
+def test_trajectory_sequence(trajectory_scene, cost_type):
+    """752x480, 64 planes, 5 sources, 8 paths: the first 5 keyframes (4 fusion steps)."""
+    from probabilistic_mapping_amd.pipeline import KeyframeSequence
+    s = trajectory_scene
+    n = 5
+    args = (s["images"], s["poses"], s["sources"], s["scene"].K, s["planes"], s["idep_min"],
+            s["idep_max"], s["knobs"], s["paths"])
+    gseq = KeyframeSequence(*args, cost_type=cost_type)
+    oseq = OracleSequence(*args, precision="f64")
+    gseq.reset()
+    for i in range(n):
+        gseq.run_keyframe(i)
+        oseq.run_keyframe(i)
+        g_agg = gseq.agg.cpu().numpy()
+        wta_agree = float((parity.wta_index(g_agg) == parity.wta_index(oseq.agg)).mean())
+        st = {"wta_agree": wta_agree,
+              "cost": parity.cost_parity(gseq.cost.float().cpu().numpy(), oseq.cost),
+              "depth": parity.depth_parity(gseq.dep.cpu().numpy(), oseq.depth)}
+        if i > 0:
+            gstate = {k: v.cpu().numpy() for k, v in gseq.state.items()}
+            st["state"] = parity.state_parity(gstate, oseq.state)
+            st["fused"] = parity.depth_parity(gseq.fused.cpu().numpy(), oseq.fused)
+        _report(f"trajectory kf{i}", st)
+        assert wta_agree >= 0.999
+        _assert_depth(st["depth"])
+        if i > 0:
+            assert st["state"]["init_mismatch"] <= 1e-3
+            for k in ("mu", "sig", "inlier"):
+                assert st["state"][k]["p999"] <= 1e-4, k
