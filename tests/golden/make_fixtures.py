@@ -21,7 +21,7 @@ the mathematics fixes, so the oracle is checked against it rather than against i
   (mu', s2') = (E[z], Var[z]) are what depthUpdate must produce (depth_fusion.cu:195-216).
 * hole_fill.json -- a hand-made 6x5 propagation table and its filled form under the
   deterministic rule (empty pixels take the entry of their lowest-linear-index non-empty
-  interior 3x3 neighbour, read from the pre-fill table).
+  interior 3x3 neighbour, read from the pre-fill table), applied by hand here.
 """
 from __future__ import annotations
 
@@ -95,7 +95,7 @@ def fusion_posterior():
              (6.0, 9.0, 1.2, 0.5, 1.5), (40.0, 2.0, 5.0, 0.3, 5.2)]
     z = np.linspace(-30, 60, 180001)
     dz = z[1] - z[0]
-    p = np.linspace(0, 1, 20001)[1:-1]
+    p = np.linspace(0, 1, 200001)[1:-1]
     dp = p[1] - p[0]
     for a, b, mu, s2, d in cases:
         tau2 = 0.01 * d * d
@@ -127,12 +127,13 @@ def fusion_posterior():
 
 
 def hole_fill():
+    """Sources at their own pixel (what an identity pose produces), so the pre-fill table holds
+    t[y, x] = y*W + x at the chosen pixels; (5, 0) is on the border and cannot scatter, and
+    pixel (0, 0) would be index 0, the empty sentinel."""
     W, H = 6, 5
     t = np.zeros((H, W), np.int64)
-    t[1, 1] = 7          # interior, fills its whole 3x3
-    t[3, 4] = 29         # interior
-    t[2, 3] = 11         # interior, competes with 29 for (3,3)/(4..,) neighbours
-    t[0, 5] = 4          # on the border: never scatters
+    for x, y in [(1, 1), (3, 2), (4, 3), (5, 0), (2, 3)]:
+        t[y, x] = y * W + x
     filled = t.copy()
     for y in range(H):
         for x in range(W):
