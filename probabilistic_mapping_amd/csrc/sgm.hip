@@ -13,36 +13,50 @@
 //   with P1, P2 scaled by q1, q2 where |I(p)-I(q)| < tau_so.
 // A pixel is invalid if any plane's cost is < 0: there C' = 0 and the aggregate is set to 0
 // (sgm.cu:33-38, 67-71, 101-104).  Every L is >= 0, so mins run on the unsigned bit patterns.
+#include <algorithm>
+#include <cmath>
+
 #include "common.hpp"
+
+#ifndef PM_SGM_PREFETCH
+#define PM_SGM_PREFETCH 16
+#endif
 
 namespace pm {
 
 struct SgmArgs {
   int W, H, D, dx, dy;
-  long nlines;
+  int nlines;
   const uint8_t* ref;
-  float pi1, pi2, tau_so, q1, q2;
+  float P1n, P2n;   // penalties where |I(p) - I(q)| >= tau_so
+  float P1s, P2s;   // penalties scaled by q1, q2 where |I(p) - I(q)| < tau_so
+  int tau_i;        // integer form of the test: |dI| < tau_so  <=>  |dI| < ceil(tau_so)
 };
 
-__device__ __forceinline__ void scanline_start(const SgmArgs& a, long i, int& x, int& y) {
+__device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, int& y) {
   if (a.dx != 0) {
     const int xs = a.dx > 0 ? 0 : a.W - 1;
-    if (i < a.H) { x = xs; y = (int)i; return; }
-    const long j = i - a.H;
-    x = a.dx > 0 ? (int)(1 + j) : (int)j;
+    if (i < a.H) { x = xs; y = i; return; }
+    const int j = i - a.H;
+    x = a.dx > 0 ? 1 + j : j;
     y = a.dy > 0 ? 0 : a.H - 1;
     return;
   }
-  x = (int)i;
+  x = i;
   y = a.dy > 0 ? 0 : a.H - 1;
 }
 
-template <int NV, typename CT, bool FIRST, int B>
+// One wave per scanline.  All scanline geometry is wave-uniform (scalar registers); every
+// load address is clamped in bounds (past-the-end steps re-read the last pixel, lanes past D
+// re-read plane D-1 of the same pixel), so the prefetch has no branches and the B loads of
+// the next block stay in flight while the current block computes.
+template <int NV, typename CT, bool FIRST, bool FULL, int B>
 __global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
                                                        const CT* __restrict__ cost,
                                                        float* __restrict__ agg) {
   const int lane = threadIdx.x & 63;
-  const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int line = blockIdx.x * 4 + wave;
   if (line >= a.nlines) return;
   int x, y;
   scanline_start(a, line, x, y);
@@ -52,127 +66,150 @@ __global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
   if (a.dy > 0) n = min(n, a.H - y);
   if (a.dy < 0) n = min(n, y + 1);
   const int D = a.D;
-  const int d0 = lane * NV;
+  const int pstep = a.dy * a.W + a.dx;
+  const int p0 = y * a.W + x;
+  int off[NV];
   bool act[NV];
 #pragma unroll
-  for (int j = 0; j < NV; ++j) act[j] = d0 + j < D;
-  const long pstep = (long)a.dy * a.W + a.dx;
-  const long p0 = (long)y * a.W + x;
+  for (int j = 0; j < NV; ++j) {
+    const int d = lane * NV + j;
+    act[j] = FULL || d < D;
+    off[j] = FULL ? d : min(d, D - 1);
+  }
   const float INF = __uint_as_float(0x7F800000u);
 
-  // register double buffer of B steps of cost and aggregate
-  float cbuf[B][NV], abuf[B][NV];
-  uint32_t ibuf[B];
-  auto load_block = [&](int base, float (&cb)[B][NV], float (&ab)[B][NV], uint32_t (&ib)[B]) {
+  // two register buffers of B steps each, used ping-pong: while one block computes, the
+  // loads of the next are in flight (addresses past the end re-read the last pixel)
+  float c0[B][NV], g0[B][NV], c1[B][NV], g1[B][NV];
+  uint32_t r0, r1;
+  auto load_block = [&](int base, float (&c)[B][NV], float (&g)[B][NV], uint32_t& r) {
+    // lane i fetches the reference intensity of step base+i; issued first so that reading it
+    // does not wait for the block's cost and aggregate loads
+    r = a.ref[p0 + min(base + lane, n - 1) * pstep];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-      const int k = base + i;
-      const long p = p0 + (long)k * pstep;
-      const bool in = k < n;
-      ib[i] = in ? (uint32_t)a.ref[p] : 0u;
+      const long p = (long)(p0 + min(base + i, n - 1) * pstep) * D;
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
-        const bool ld = in && act[j];
-        cb[i][j] = ld ? load_cost<CT>(cost + p * D + d0 + j) : 0.0f;
-        if (!FIRST) ab[i][j] = ld ? agg[p * D + d0 + j] : 0.0f;
-        else ab[i][j] = 0.0f;
+        c[i][j] = load_cost<CT>(cost + p + off[j]);
+        if (!FIRST) g[i][j] = agg[p + off[j]];
       }
     }
   };
 
-  load_block(0, cbuf, abuf, ibuf);
   float L[NV];
-  uint32_t iprev = 0;
-  // step 0: L = C' (sgm.cu:21-43)
-  {
+  uint32_t iq;        // intensity of the last pixel of the previous block
+  uint64_t smooth;    // bit i: |I(step i) - I(step i-1)| < tau_so for the current block
+  // gradient test of a whole block at once (sgm.cu:75-78): lane i compares step i with step
+  // i-1 (lane 0 with the carried intensity) and a ballot packs the answers
+  auto smooth_mask = [&](uint32_t r) {
+    const int prev = (int)dpp_u32<0x138>(iq, r);   // wave_shr:1, lane 0 takes iq
+    const int d1 = abs((int)r - prev);
+    iq = __builtin_amdgcn_readlane(r, B - 1);
+    return (uint64_t)__ballot(d1 < a.tau_i);
+  };
+  // one step k >= 1 from slot i of a block buffer
+  auto step = [&](int k, int i, const float (&c)[B][NV], const float (&g)[B][NV]) {
+    const long p = (long)(p0 + k * pstep) * D;
     bool neg = false;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) neg = neg || (act[j] && cbuf[0][j] < 0.0f);
+    for (int j = 0; j < NV; ++j) neg = neg || c[i][j] < 0.0f;
+    const bool invalid = __ballot(neg) != 0;
+    // m = min_d L(q, d) (sgm.cu:55-66)
+    float lm = L[0];
+#pragma unroll
+    for (int j = 1; j < NV; ++j) lm = umin(lm, L[j]);
+    const float m = wave_min_nonneg(lm);
+    // gradient-adaptive penalties (sgm.cu:78-82), scalar
+    const bool sm = (smooth >> i) & 1ull;
+    const float P1 = sm ? a.P1s : a.P1n;
+    const float P2 = sm ? a.P2s : a.P2n;
+    const float lo = wave_shr1(L[NV - 1], INF);   // L(q, d0-1)
+    const float hi = wave_shl1(L[0], INF);        // L(q, d0+NV)
+    const float mP2 = m + P2;
+    float Ln[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const float left = j > 0 ? L[j - 1] : lo;
+      const float right = j + 1 < NV ? L[j + 1] : hi;
+      float val = umin(L[j], mP2);                 // sgm.cu:85
+      val = umin(val, umin(left, right) + P1);     // sgm.cu:86-93
+      const float cc = invalid ? kCostInf : c[i][j];
+      Ln[j] = act[j] ? (cc + val) - m : INF;       // sgm.cu:98
+      const float out = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);
+      if (act[j]) agg[p + off[j]] = out;           // sgm.cu:101-108
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) L[j] = Ln[j];
+  };
+  // a block whose B steps all exist: no exits, so the loads of the other buffer stay in flight
+  auto run_full = [&](int base, const float (&c)[B][NV], const float (&g)[B][NV], uint32_t r) {
+    smooth = smooth_mask(r);
+#pragma unroll
+    for (int i = 0; i < B; ++i) step(base + i, i, c, g);
+  };
+  auto run_checked = [&](int base, const float (&c)[B][NV], const float (&g)[B][NV],
+                         uint32_t r) {
+    smooth = smooth_mask(r);
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      if (base + i >= n) break;
+      step(base + i, i, c, g);
+    }
+  };
+
+  // step 0: L = C' (sgm.cu:21-43)
+  {
+    const long p = (long)p0 * D;
+    float c[NV], g[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      c[j] = load_cost<CT>(cost + p + off[j]);
+      g[j] = FIRST ? 0.0f : agg[p + off[j]];
+    }
+    iq = a.ref[p0];   // intensity of step 0
+    bool neg = false;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) neg = neg || c[j] < 0.0f;
     const bool invalid = __ballot(neg) != 0;
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-      L[j] = act[j] ? (invalid ? kCostInf : cbuf[0][j]) : INF;
-      if (act[j]) {
-        const float out = invalid ? kCostInf : (FIRST ? L[j] : abuf[0][j] + L[j]);
-        agg[p0 * D + d0 + j] = out;
-      }
+      L[j] = act[j] ? (invalid ? kCostInf : c[j]) : INF;
+      const float out = invalid ? kCostInf : (FIRST ? L[j] : g[j] + L[j]);
+      if (act[j]) agg[p + off[j]] = out;
     }
-    iprev = ibuf[0];
   }
-
-  for (int base = 0; base < n; base += B) {
-    float cn[B][NV], an[B][NV];
-    uint32_t in_[B];
-    load_block(base + B, cn, an, in_);
-#pragma unroll
-    for (int i = 0; i < B; ++i) {
-      const int k = base + i;
-      if (k == 0) continue;
-      if (k >= n) break;
-      const long p = p0 + (long)k * pstep;
-      // m = min_d L(q, d) (sgm.cu:55-66)
-      float lm = L[0];
-#pragma unroll
-      for (int j = 1; j < NV; ++j) lm = umin(lm, L[j]);
-      const float m = wave_min_nonneg(lm);
-      // gradient-adaptive penalties (sgm.cu:75-82)
-      const float D1 = fabsf((float)ibuf[i] - (float)iprev);
-      iprev = ibuf[i];
-      float P1 = a.pi1, P2 = a.pi2;
-      if (D1 < a.tau_so) {
-        P1 *= a.q1;
-        P2 *= a.q2;
-      }
-      const float lo = wave_shr1(L[NV - 1], INF);   // L(q, d0-1)
-      const float hi = wave_shl1(L[0], INF);        // L(q, d0+NV)
-      bool neg = false;
-#pragma unroll
-      for (int j = 0; j < NV; ++j) neg = neg || (act[j] && cbuf[i][j] < 0.0f);
-      const bool invalid = __ballot(neg) != 0;
-      const float mP2 = m + P2;
-      float Ln[NV];
-#pragma unroll
-      for (int j = 0; j < NV; ++j) {
-        const float left = j > 0 ? L[j - 1] : lo;
-        const float right = j + 1 < NV ? L[j + 1] : hi;
-        float val = umin(L[j], mP2);                 // sgm.cu:85
-        val = umin(val, umin(left, right) + P1);     // sgm.cu:86-93
-        const float c = invalid ? kCostInf : cbuf[i][j];
-        Ln[j] = act[j] ? (c + val) - m : INF;        // sgm.cu:98
-        if (act[j]) {
-          const float out = invalid ? kCostInf : (FIRST ? Ln[j] : abuf[i][j] + Ln[j]);
-          agg[p * D + d0 + j] = out;                 // sgm.cu:101-108
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < NV; ++j) L[j] = Ln[j];
-    }
-#pragma unroll
-    for (int i = 0; i < B; ++i) {
-      ibuf[i] = in_[i];
-#pragma unroll
-      for (int j = 0; j < NV; ++j) {
-        cbuf[i][j] = cn[i][j];
-        abuf[i][j] = an[i][j];
-      }
-    }
+  int base = 1;
+  load_block(base, c0, g0, r0);
+  while (base + 2 * B <= n) {
+    load_block(base + B, c1, g1, r1);
+    run_full(base, c0, g0, r0);
+    load_block(base + 2 * B, c0, g0, r0);
+    run_full(base + B, c1, g1, r1);
+    base += 2 * B;
+  }
+  run_checked(base, c0, g0, r0);
+  if (base + B < n) {
+    load_block(base + B, c1, g1, r1);
+    run_checked(base + B, c1, g1, r1);
   }
 }
 
-template <int NV, typename CT>
+template <int NV, typename CT, bool FULL>
 pm_status launch_paths(const SgmArgs& base, int paths, const CT* cost, float* agg,
                        hipStream_t st) {
-  constexpr int B = 8;
+  constexpr int B = PM_SGM_PREFETCH;
   for (int path = 0; path < paths; ++path) {
     SgmArgs a = base;
     a.dx = kPathDx[path];
     a.dy = kPathDy[path];
-    a.nlines = (a.dx != 0 && a.dy != 0) ? (long)a.W + a.H - 1 : (a.dx != 0 ? a.H : a.W);
+    a.nlines = (a.dx != 0 && a.dy != 0) ? a.W + a.H - 1 : (a.dx != 0 ? a.H : a.W);
     const unsigned blocks = (unsigned)((a.nlines + 3) / 4);
     if (path == 0)
-      sgm_path_kernel<NV, CT, true, B><<<blocks, 256, 0, st>>>(a, cost, agg);
+      sgm_path_kernel<NV, CT, true, FULL, B><<<blocks, 256, 0, st>>>(a, cost, agg);
     else
-      sgm_path_kernel<NV, CT, false, B><<<blocks, 256, 0, st>>>(a, cost, agg);
+      sgm_path_kernel<NV, CT, false, FULL, B><<<blocks, 256, 0, st>>>(a, cost, agg);
     pm_status s = check_launch("sgm_path_kernel");
     if (s != PM_OK) return s;
   }
@@ -181,9 +218,11 @@ pm_status launch_paths(const SgmArgs& base, int paths, const CT* cost, float* ag
 
 template <typename CT>
 pm_status launch_sgm(const SgmArgs& a, int paths, const CT* cost, float* agg, hipStream_t st) {
-  if (a.D <= 64) return launch_paths<1, CT>(a, paths, cost, agg, st);
-  if (a.D <= 128) return launch_paths<2, CT>(a, paths, cost, agg, st);
-  return launch_paths<4, CT>(a, paths, cost, agg, st);
+  if (a.D == 64) return launch_paths<1, CT, true>(a, paths, cost, agg, st);
+  if (a.D == 128) return launch_paths<2, CT, true>(a, paths, cost, agg, st);
+  if (a.D < 64) return launch_paths<1, CT, false>(a, paths, cost, agg, st);
+  if (a.D < 128) return launch_paths<2, CT, false>(a, paths, cost, agg, st);
+  return launch_paths<4, CT, false>(a, paths, cost, agg, st);
 }
 
 }  // namespace pm
@@ -208,8 +247,11 @@ extern "C" pm_status pm_sgm(pm_shape shape, const uint8_t* ref, const void* cost
                   "pm_sgm: pi1, pi2, tau_so, sgm_q1, sgm_q2 must be finite and >= 0");
   SgmArgs a{};
   a.W = shape.width; a.H = shape.height; a.D = shape.planes; a.ref = ref;
-  a.pi1 = params.pi1; a.pi2 = params.pi2; a.tau_so = params.tau_so;
-  a.q1 = params.sgm_q1; a.q2 = params.sgm_q2;
+  a.P1n = params.pi1;
+  a.P2n = params.pi2;
+  a.P1s = params.pi1 * params.sgm_q1;   // the same float products as sgm.cu:80-81
+  a.P2s = params.pi2 * params.sgm_q2;
+  a.tau_i = (int)std::min(256.0f, std::ceil(params.tau_so));
   hipStream_t st = (hipStream_t)stream;
   switch (cost_type) {
     case PM_COST_F32: return launch_sgm<float>(a, params.paths, (const float*)cost, agg, st);
