@@ -90,6 +90,16 @@ __device__ __forceinline__ float wave_min_nonneg(float v) {
   return __uint_as_float(min(min(a, b), min(c, d)));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block i runs on XCD i % 8).  This
+// remaps the launch index so that each XCD receives one contiguous range of logical blocks,
+// and neighbouring tiles share that XCD's L2.  A bijection on [0, nblocks).
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+  constexpr int kXcd = 8;
+  const int per = nblocks / kXcd, rem = nblocks % kXcd;
+  const int xcd = b % kXcd, idx = b / kXcd;
+  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
 __device__ __forceinline__ float texel_u8(const uint8_t* img, int W, int H, int x, int y) {
   return (x < 0 || y < 0 || x >= W || y >= H) ? 0.0f : (float)img[(long)y * W + x];
 }

@@ -7,13 +7,17 @@
 // per source).  Here one launch visits every source while the per-(pixel, plane) sum and count
 // stay in registers, and the volume is written once.
 //
-// Mapping: a wave owns a run of PX pixels of one row; lane l handles plane d = dbase + l.
-// Everything that depends only on the pixel (the 9 reference samples, the 9 homography
-// triples) is wave-uniform; each lane evaluates the 9 warps for its inverse depth.
+// Mapping: see cost_tile_kernel -- lanes are image columns, each pixel's centre warp is
+// computed once per (plane, source) and shared with the neighbours whose reference terms use
+// it, so a (pixel, plane, source) costs about 3.25 warps instead of 9.
 //
 // Bilinear sampling reads the "quad" texture: one uint32 per texel packing
 // I(x,y), I(x+1,y), I(x,y+1), I(x+1,y+1) (border 0), so a sample is a single dword gather.
 #include "common.hpp"
+
+#ifndef PM_COST_TILE_ROWS
+#define PM_COST_TILE_ROWS 4
+#endif
 
 namespace pm {
 
@@ -38,19 +42,17 @@ __global__ __launch_bounds__(256) void make_quad_kernel(const uint8_t* __restric
   quad[(long)y * W + x] = i00 | (i10 << 8) | (i01 << 16) | (i11 << 24);
 }
 
-// |ref - bilinear(meas, U, V)| with the warp's validity (calc_cost.cu:85-93).  The validity
-// test is written positively so a NaN warp (0/0) is invalid; invalid warps read texel 0 so
-// the gather stays in bounds.
-__device__ __forceinline__ float warp_term(const uint32_t* __restrict__ quad, int W, float Wm1,
-                                           float Hm1, float X, float Y, float Z, float t1,
-                                           float t2, float t3, float idep, float r,
-                                           bool& ok) {
-  const float w = Z + t3 * idep;
+// Bilinear sample of the quad texture at (U, V) with the warp's validity (calc_cost.cu:85-93);
+// returns +inf for an invalid warp.  The test is written positively so a NaN warp (0/0) is
+// invalid; invalid warps read texel 0 so the gather stays in bounds.
+__device__ __forceinline__ float warp_sample(const uint32_t* __restrict__ quad, int W, float Wm1,
+                                             float Hm1, float X, float Y, float w) {
   const float rw = __builtin_amdgcn_rcpf(w);
-  const float U = (X + t1 * idep) * rw;
-  const float V = (Y + t2 * idep) * rw;
-  const bool valid = (w >= 0.0f) && (U >= 0.0f) && (U <= Wm1) && (V >= 0.0f) && (V <= Hm1);
-  ok = ok && valid;
+  const float U = X * rw;
+  const float V = Y * rw;
+  // bitwise (not short-circuit) tests and an unconditional gather: the sample is always
+  // used, so the compiler keeps every gather of a row in flight instead of branching
+  const bool valid = (w >= 0.0f) & (U >= 0.0f) & (U <= Wm1) & (V >= 0.0f) & (V <= Hm1);
   const float Uc = valid ? U : 0.0f, Vc = valid ? V : 0.0f;
   const int x0 = (int)Uc, y0 = (int)Vc;   // non-negative, so truncation is floor
   const float ax = Uc - (float)x0, ay = Vc - (float)y0;
@@ -59,62 +61,100 @@ __device__ __forceinline__ float warp_term(const uint32_t* __restrict__ quad, in
   const float i01 = (float)((q >> 16) & 0xFFu), i11 = (float)(q >> 24);
   const float top = i00 + ax * (i10 - i00);
   const float bot = i01 + ax * (i11 - i01);
-  return fabsf(r - (top + ay * (bot - top)));
+  const float v = top + ay * (bot - top);
+  return v + (valid ? 0.0f : __uint_as_float(0x7F800000u));
 }
 
-template <typename CT, int PX>
-__global__ __launch_bounds__(256) void cost_kernel(const CostArgs a, CT* __restrict__ cost) {
+// Tile kernel.  A wave covers a strip of 64 columns u = x0-1 .. x0+62 (62 output columns and
+// a one-column halo on each side) and TY output rows, for a quarter of the planes.  For each
+// (plane, source) it walks the rows y0-1 .. y0+TY once, warping every pixel's own centre
+// H_d (u, r, 1) a single time.  The reference's nine warps of pixel (u, v) are the centre
+// warps of its 3x3 neighbours, except two garbled ones (calc_cost.cu:50-51, 70-71):
+//   term 0 (u,v), 3 (u-1,v), 5 (u-1,v-1), 6 (u+1,v+1), 7 (u-1,v+1): |I(q) - S(q)| at q
+//   term 1: warp (u,v-1) against I(u,v+1);  term 2: warp (u,v+1) against I(u,v-1)
+//   term 4: (x+h11, x+h21, x+h31) against I(u+1,v)
+//   term 8: (x+h11-h12, x+h11-h22, x+h11-h32) against I(u+1,v-1),  x = h11 u + h12 v + h13
+// so each centre sample yields |I(q)-S|, |I(q+(0,2))-S| and |I(q-(0,2))-S|, kept in a 3-row
+// register window; column neighbours come from DPP lane shifts.  An invalid warp is +inf, so
+// a source contributes iff the 9-term sum is finite (calc_cost.cu:89-91).
+template <typename CT, int TY>
+__global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, int nstrips,
+                                                        CT* __restrict__ cost) {
   const int lane = threadIdx.x & 63;
-  const int y = blockIdx.y;
-  const int xs = (blockIdx.x * 4 + (threadIdx.x >> 6)) * PX;
-  if (xs >= a.W) return;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int strip = blk % nstrips;
+  const int y0 = (blk / nstrips) * TY;
   const int W = a.W, H = a.H, D = a.D;
+  const int u = strip * 62 - 1 + lane;
+  const bool writer = lane >= 1 && lane <= 62 && u < W;
+  const int dper = (D + 3) / 4;
+  const int dbeg = wave * dper, dend = min(D, dbeg + dper);
   const float Wm1 = (float)(W - 1), Hm1 = (float)(H - 1);
-  for (int dbase = 0; dbase < D; dbase += kWave) {
-    const int d = dbase + lane;
-    const float idep = a.idep_min + (float)d * a.dep_sample;
-    for (int i = 0; i < PX; ++i) {
-      const int u = xs + i;
-      if (u >= W) break;
-      // reference samples, calc_cost.cu:93-198 (note the swapped up/down rows of terms 1, 2)
-      const float r[9] = {texel_u8(a.ref, W, H, u, y),         texel_u8(a.ref, W, H, u, y + 1),
-                          texel_u8(a.ref, W, H, u, y - 1),     texel_u8(a.ref, W, H, u - 1, y),
-                          texel_u8(a.ref, W, H, u + 1, y),     texel_u8(a.ref, W, H, u - 1, y - 1),
-                          texel_u8(a.ref, W, H, u + 1, y + 1), texel_u8(a.ref, W, H, u - 1, y + 1),
-                          texel_u8(a.ref, W, H, u + 1, y - 1)};
-      float sum = 0.0f, cnt = 0.0f;
-      for (int s = 0; s < a.S; ++s) {
-        const float* hm = a.hm[s];
-        const float h11 = hm[0], h12 = hm[1], h13 = hm[2], h21 = hm[3], h22 = hm[4],
-                    h23 = hm[5], h31 = hm[6], h32 = hm[7], h33 = hm[8];
-        const float t1 = hm[9], t2 = hm[10], t3 = hm[11];
-        const float x = h11 * u + h12 * y + h13;   // calc_cost.cu:28-30
-        const float yy = h21 * u + h22 * y + h23;
-        const float z = h31 * u + h32 * y + h33;
-        const float xr = x + h11;
-        // warp triples, calc_cost.cu:34-71, including the garbled right (:50-51) and
-        // up-right (:70-71) neighbours
-        const float X[9] = {x, x - h12, x + h12, x - h11, xr, x - h12 - h11, x + h12 + h11,
-                            x - h11 + h12, xr - h12};
-        const float Y[9] = {yy, yy - h22, yy + h22, yy - h21, x + h21, yy - h22 - h21,
-                            yy + h22 + h21, yy - h21 + h22, xr - h22};
-        const float Z[9] = {z, z - h32, z + h32, z - h31, x + h31, z - h32 - h31,
-                            z + h32 + h31, z - h31 + h32, xr - h32};
-        const uint32_t* q = a.quad[s];
-        bool ok = true;
-        float tmp = 0.0f;
+  const float INF = __uint_as_float(0x7F800000u);
+  const float fu = (float)u;
+  // reference intensities of this column for rows y0-3 .. y0+TY+2 (0 outside the image)
+  float I[TY + 6];
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
-          tmp += warp_term(q, W, Wm1, Hm1, X[k], Y[k], Z[k], t1, t2, t3, idep, r[k], ok);
-        if (ok) {   // a source counts only if all 9 warps are valid (calc_cost.cu:89-91)
-          sum += tmp;
-          cnt += 1.0f;
+  for (int k = 0; k < TY + 6; ++k) I[k] = texel_u8(a.ref, W, H, u, y0 - 3 + k);
+
+  for (int d = dbeg; d < dend; ++d) {
+    const float idep = a.idep_min + (float)d * a.dep_sample;
+    float acc[TY], cnt[TY];
+#pragma unroll
+    for (int t = 0; t < TY; ++t) { acc[t] = 0.0f; cnt[t] = 0.0f; }
+    for (int s = 0; s < a.S; ++s) {
+      const float* hm = a.hm[s];
+      const float h11 = hm[0], h12 = hm[1], h13 = hm[2], h21 = hm[3], h22 = hm[4],
+                  h23 = hm[5], h31 = hm[6], h32 = hm[7], h33 = hm[8];
+      const float tx = hm[9] * idep, ty = hm[10] * idep, tz = hm[11] * idep;
+      const uint32_t* q = a.quad[s];
+      // column parts of H (u, r, 1) + t idep
+      const float Xu = h11 * fu + h13 + tx;
+      const float Yu = h21 * fu + h23 + ty;
+      const float Zu = h31 * fu + h33 + tz;
+      // garbled warps, offsets from x = h11 u + h12 v + h13 (calc_cost.cu:49-51, 69-71)
+      const float g4x = h11 + tx, g4y = h21 + ty, g4z = h31 + tz;
+      const float g8x = h11 - h12 + tx, g8y = h11 - h22 + ty, g8z = h11 - h32 + tz;
+      const float xu = h11 * fu + h13;
+      float A0 = 0.0f, A1 = 0.0f, U0 = 0.0f, U1 = 0.0f;   // |I-S| rows r-2, r-1; up terms
+#pragma unroll
+      for (int k = 0; k < TY + 2; ++k) {   // centre row r = y0 - 1 + k
+        const float fr = (float)(y0 - 1 + k);
+        const float S = warp_sample(q, W, Wm1, Hm1, Xu + h12 * fr, Yu + h22 * fr,
+                                    Zu + h32 * fr);
+        const float A2 = fabsf(I[k + 2] - S);     // |I(u,r) - S(u,r)|
+        const float U2 = fabsf(I[k + 4] - S);     // |I(u,r+2) - S(u,r)|
+        if (k >= 2) {
+          const int t = k - 2;                    // output row v = r - 1
+          const float Dn = fabsf(I[k] - S);       // |I(u,r-2) - S(u,r)| = term 2 of row v
+          const float x = xu + h12 * (float)(y0 + t);
+          const float G4 = fabsf(wave_shl1(I[t + 3], 0.0f) -
+                                 warp_sample(q, W, Wm1, Hm1, x + g4x, x + g4y, x + g4z));
+          const float G8 = fabsf(wave_shl1(I[t + 2], 0.0f) -
+                                 warp_sample(q, W, Wm1, Hm1, x + g8x, x + g8y, x + g8z));
+          float T = A1 + wave_shr1(A0, INF);      // terms 0, 5
+          T += wave_shr1(A1, INF);                // term 3
+          T += wave_shl1(A2, INF);                // term 6
+          T += wave_shr1(A2, INF);                // term 7
+          T += U0 + Dn;                           // terms 1, 2
+          T += G4 + G8;                           // terms 4, 8
+          const bool ok = T < INF;
+          acc[t] += ok ? T : 0.0f;
+          cnt[t] += ok ? 1.0f : 0.0f;
         }
+        A0 = A1; A1 = A2;
+        U0 = U1; U1 = U2;
       }
-      if (d < D) {
-        // mean over sources of (sum of 9 terms)/9, or -1 if no source saw it
-        const float c = cnt > 0.0f ? sum / (9.0f * cnt) : -1.0f;
-        cost[((long)y * W + u) * D + d] = store_cost<CT>(c);
+    }
+    if (writer) {
+#pragma unroll
+      for (int t = 0; t < TY; ++t) {
+        const int v = y0 + t;
+        if (v < H) {
+          const float c = cnt[t] > 0.0f ? acc[t] / (9.0f * cnt[t]) : -1.0f;
+          cost[((long)v * W + u) * D + d] = store_cost<CT>(c);
+        }
       }
     }
   }
@@ -163,21 +203,23 @@ extern "C" pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t nu
       if (!std::isfinite(a.hm[s][k]))
         return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: non-finite homography");
   }
-  constexpr int PX = 8;
-  dim3 grid((a.W + 4 * PX - 1) / (4 * PX), a.H);
+  constexpr int TY = PM_COST_TILE_ROWS;
+  const int nstrips = (a.W + 61) / 62;
+  const unsigned blocks = (unsigned)(nstrips * ((a.H + TY - 1) / TY));
   hipStream_t st = (hipStream_t)stream;
   switch (cost_type) {
     case PM_COST_F32:
-      cost_kernel<float, PX><<<grid, 256, 0, st>>>(a, (float*)cost);
+      cost_tile_kernel<float, TY><<<blocks, 256, 0, st>>>(a, nstrips, (float*)cost);
       break;
     case PM_COST_F16:
-      cost_kernel<__half, PX><<<grid, 256, 0, st>>>(a, (__half*)cost);
+      cost_tile_kernel<__half, TY><<<blocks, 256, 0, st>>>(a, nstrips, (__half*)cost);
       break;
     case PM_COST_BF16:
-      cost_kernel<__hip_bfloat16, PX><<<grid, 256, 0, st>>>(a, (__hip_bfloat16*)cost);
+      cost_tile_kernel<__hip_bfloat16, TY><<<blocks, 256, 0, st>>>(a, nstrips,
+                                                                   (__hip_bfloat16*)cost);
       break;
     default:
       return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: unknown cost type");
   }
-  return check_launch("cost_kernel");
+  return check_launch("cost_tile_kernel");
 }
