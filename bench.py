@@ -97,7 +97,7 @@ def main():
 
     # timed region: K full sequences; HIP events on the launch stream bracket each stage
     stream = torch.cuda.current_stream()
-    ev = {k: [] for k in ("cost", "sgm", "filter", "fusion")}
+    ev = {k: [] for k in ("cost", "sgm", "median", "fusion")}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -109,15 +109,12 @@ def main():
             pl.calc_cost(seq.images[i], [seq.quads[j] for j in seq.sources[i]], seq.homs[i],
                          seq.idep_min, seq.dep_sample, seq.D, seq.cost_type, out=seq.cost)
             e[1].record(stream)
-            pl.sgm(seq.images[i], seq.cost, seq.knobs, seq.paths, out=seq.agg)
+            pl.sgm_filter(seq.images[i], seq.cost, seq.knobs, seq.paths, seq.idep_min,
+                          seq.dep_sample, agg=seq.agg,
+                          depth=seq.dep_raw if seq.median else seq.dep)
             e[2].record(stream)
             if seq.median:
-                pl.filter_cost(seq.agg, seq.knobs["var_scale"], seq.idep_min, seq.dep_sample,
-                               out=seq.dep_raw)
                 pl.median3x3(seq.dep_raw, out=seq.dep)
-            else:
-                pl.filter_cost(seq.agg, seq.knobs["var_scale"], seq.idep_min, seq.dep_sample,
-                               out=seq.dep)
             e[3].record(stream)
             seq.fusion_stage(i)
             e[4].record(stream)
@@ -133,17 +130,15 @@ def main():
     sgm_gbs = sgm_bytes / (ms["sgm"] * 1e-3) / 1e9
     cost_flop = V * S * 9 * 20                          # ~20 FLOP per warp term
     cost_bytes = V * bc + (S + 1) * N
-    filt_bytes = V * 4 + N * 4
     fus_bytes = 104 * N
     stages = {
         "cost": {"ms": ms["cost"], "bound": "valu", "achieved_tflops": cost_flop / (ms["cost"] * 1e-3) / 1e12,
                  "peak_tflops": FP32_VALU_PEAK_TFLOPS,
                  "hbm_gbs": cost_bytes / (ms["cost"] * 1e-3) / 1e9},
-        "sgm": {"ms": ms["sgm"], "bound": "hbm", "achieved_gbs": sgm_gbs,
-                "frac": sgm_gbs / HBM_PEAK_GBS,
-                "compulsory_gbs": V * (bc + 4) / (ms["sgm"] * 1e-3) / 1e9},
-        "filter": {"ms": ms["filter"], "bound": "hbm",
-                   "achieved_gbs": filt_bytes / (ms["filter"] * 1e-3) / 1e9},
+        "sgm_filter": {"ms": ms["sgm"], "bound": "hbm", "achieved_gbs": sgm_gbs,
+                       "frac": sgm_gbs / HBM_PEAK_GBS,
+                       "compulsory_gbs": V * (bc + 4) / (ms["sgm"] * 1e-3) / 1e9},
+        "median": {"ms": ms["median"]},
         "fusion": {"ms": ms["fusion"], "bound": "hbm",
                    "achieved_gbs": fus_bytes / (ms["fusion"] * 1e-3) / 1e9},
     }
@@ -166,7 +161,7 @@ def main():
                                "sequence per step",
                    "form": "plain", "cost_storage": args.cost_type,
                    "median": bool(args.median), "keyframes_per_step": nkf},
-        "roofline": {"bound": "hbm", "kernel": "sgm pass (8 path launches)",
+        "roofline": {"bound": "hbm", "kernel": "sgm pass (8 path launches, filter folded in)",
                      "achieved": sgm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sgm_gbs / HBM_PEAK_GBS, "traffic": None},
         "stages": stages,

@@ -116,6 +116,15 @@ pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t num_sources,
 pm_status pm_sgm(pm_shape shape, const uint8_t* ref, const void* cost, pm_cost_type cost_type,
                  pm_sgm_params params, float* agg, pm_stream stream);
 
+/* pm_sgm followed by pm_filter_cost, with the filter folded into the last path kernel: the
+ * final aggregate of each pixel is reduced across the wave that wrote it, so the aggregate is
+ * not read back.  agg (complete, as pm_sgm) and depth (identical to pm_filter_cost's) are both
+ * written.  Replaces sgm2 + filter_cost (stereo_mapper.cpp:118-126). */
+pm_status pm_sgm_filter(pm_shape shape, const uint8_t* ref, const void* cost,
+                        pm_cost_type cost_type, pm_sgm_params params, float var_scale,
+                        float idep_min, float dep_sample, float* agg, float* depth,
+                        pm_stream stream);
+
 /* Winner-take-all, rejection rules and parabola subpixel fit -> depth [H][W].
  * Replaces filter_cost (motion_stereo/filter_cost.h:7-8, filter_cost.cu:3-70); depth is a
  * dense [H][W] map instead of a pitched one.  depth = 1 / (idep_min + s * dep_sample). */

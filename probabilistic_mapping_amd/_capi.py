@@ -80,6 +80,8 @@ SIGNATURES = {
     "pm_calc_cost": (_i32, [Shape, _vp, _i32, ctypes.POINTER(_vp), ctypes.POINTER(Homography),
                             ctypes.c_float, ctypes.c_float, _vp, _i32, _vp]),
     "pm_sgm": (_i32, [Shape, _vp, _vp, _i32, SgmParams, _vp, _vp]),
+    "pm_sgm_filter": (_i32, [Shape, _vp, _vp, _i32, SgmParams, ctypes.c_float, ctypes.c_float,
+                             ctypes.c_float, _vp, _vp, _vp]),
     "pm_filter_cost": (_i32, [Shape, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp,
                               _vp]),
     "pm_wta": (_i32, [Shape, _vp, _i32, ctypes.c_float, ctypes.c_float, _vp, _vp]),

@@ -230,11 +230,9 @@ void StereoMapper::run_sgm_filter() {
   ensure_cost();
   pm_sgm_params p{pi1, pi2, tau_so, sgm_q1, sgm_q2, SGM_PATHS};
   const pm_shape sh{W_, H_, D_};
-  check(pm_sgm(sh, img_l.data, raw_cost.data, raw_cost_type_, p,
-               reinterpret_cast<float*>(sgm_cost.data), stream_));
   float* fout = reinterpret_cast<float*>(MEDIAN_FILTER ? med_tmp_.data : dep.data);
-  check(pm_filter_cost(sh, reinterpret_cast<const float*>(sgm_cost.data), var_scale, IDEP_MIN,
-                       DEP_SAMPLE, fout, stream_));
+  check(pm_sgm_filter(sh, img_l.data, raw_cost.data, raw_cost_type_, p, var_scale, IDEP_MIN,
+                      DEP_SAMPLE, reinterpret_cast<float*>(sgm_cost.data), fout, stream_));
   if (MEDIAN_FILTER)
     check(pm_median3x3(fout, reinterpret_cast<float*>(dep.data), W_, H_, stream_));
   depth_ready_ = true;

@@ -97,6 +97,23 @@ def sgm(ref: torch.Tensor, cost: torch.Tensor, knobs: dict, paths: int,
     return out
 
 
+def sgm_filter(ref: torch.Tensor, cost: torch.Tensor, knobs: dict, paths: int, idep_min: float,
+               dep_sample: float, agg: torch.Tensor | None = None,
+               depth: torch.Tensor | None = None):
+    """SGM with the WTA/subpixel filter folded into the last path -> (agg, depth)."""
+    H, W, D = cost.shape
+    if agg is None:
+        agg = torch.empty((H, W, D), dtype=torch.float32, device=cost.device)
+    if depth is None:
+        depth = torch.empty((H, W), dtype=torch.float32, device=cost.device)
+    p = C.SgmParams(knobs["pi1"], knobs["pi2"], knobs["tau_so"], knobs["sgm_q1"],
+                    knobs["sgm_q2"], paths)
+    C.check(C.load().pm_sgm_filter(C.Shape(W, H, D), _ptr(ref), _ptr(cost), _cost_code(cost), p,
+                                   float(knobs["var_scale"]), float(idep_min), float(dep_sample),
+                                   _ptr(agg), _ptr(depth), _stream()))
+    return agg, depth
+
+
 def filter_cost(agg: torch.Tensor, var_scale: float, idep_min: float, dep_sample: float,
                 out: torch.Tensor | None = None) -> torch.Tensor:
     H, W, D = agg.shape
@@ -200,18 +217,15 @@ class KeyframeSequence:
         self.prev_pose = None
 
     def depth_stage(self, i: int) -> torch.Tensor:
-        """cost -> SGM -> filter (-> median) for keyframe i; returns the filtered depth."""
+        """cost -> SGM + filter (-> median) for keyframe i; returns the filtered depth."""
         srcs = self.sources[i]
         calc_cost(self.images[i], [self.quads[j] for j in srcs], self.homs[i], self.idep_min,
                   self.dep_sample, self.D, self.cost_type, out=self.cost)
-        sgm(self.images[i], self.cost, self.knobs, self.paths, out=self.agg)
+        out = self.dep_raw if self.median else self.dep
+        sgm_filter(self.images[i], self.cost, self.knobs, self.paths, self.idep_min,
+                   self.dep_sample, agg=self.agg, depth=out)
         if self.median:
-            filter_cost(self.agg, self.knobs["var_scale"], self.idep_min, self.dep_sample,
-                        out=self.dep_raw)
             median3x3(self.dep_raw, out=self.dep)
-        else:
-            filter_cost(self.agg, self.knobs["var_scale"], self.idep_min, self.dep_sample,
-                        out=self.dep)
         return self.dep
 
     def fusion_stage(self, i: int) -> None:

@@ -199,3 +199,22 @@ def test_fusion_same_depths(dev, trajectory_scene):
         assert st["fused"]["category_mismatch"] <= 1e-3
         # both sides continue from their own state; the oracle's state is the reference
         pre ^= 1
+
+
+@pytest.mark.parametrize("paths", [4, 8])
+@pytest.mark.parametrize("planes", [32, 64, 96, 128])
+def test_sgm_filter_fused_equals_separate(dev, small_scene, paths, planes):
+    """The filter folded into the last SGM path gives the same aggregate and bit-identical
+    depth as pm_sgm followed by pm_filter_cost."""
+    s = small_scene
+    K, homs, _, _ = _small_inputs(s)
+    ds = (s["idep_max"] - s["idep_min"]) / (planes - 1)
+    ref = torch.from_numpy(s["ref"]).cuda()
+    quads = [dev.make_quad(torch.from_numpy(im).cuda()) for im in s["srcs"]]
+    cost = dev.calc_cost(ref, quads, homs, s["idep_min"], ds, planes)
+    knobs = dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+    agg_a = dev.sgm(ref, cost, knobs, paths)
+    dep_a = dev.filter_cost(agg_a, knobs["var_scale"], s["idep_min"], ds)
+    agg_b, dep_b = dev.sgm_filter(ref, cost, knobs, paths, s["idep_min"], ds)
+    assert torch.equal(agg_a, agg_b)
+    assert torch.equal(dep_a, dep_b)
