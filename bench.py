@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-baseline-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="size of the attainable-bandwidth copy (0 disables it)")
     return ap.parse_args()
 
 
@@ -68,6 +70,29 @@ def cpu_baseline(scene, seconds: float) -> dict | None:
                       f"8-path workload (float instantiation of the oracle, OpenMP)"}
 
 
+def attainable_copy(torch, mib: int) -> dict | None:
+    """Device-to-device copy of `mib` MiB: the attainable streaming bandwidth on this box, and
+    the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters of a profile."""
+    if mib <= 0:
+        return None
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda")
+    src.fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return {"bytes": 2 * n, "ms": ms, "gbs": 2 * n / (ms * 1e-3) / 1e9}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -75,6 +100,7 @@ def main():
                                    "(keyframe streams would shard one stream per GPU)"}))
         sys.exit(2)
     import torch
+    from probabilistic_mapping_amd import pipeline as pl
     from probabilistic_mapping_amd import synth
     from probabilistic_mapping_amd.pipeline import KeyframeSequence
 
@@ -91,13 +117,15 @@ def main():
     V = N * D
     bc = {"f32": 4, "f16": 2, "bf16": 2}[args.cost_type]
 
+    copy = attainable_copy(torch, args.copy_mib)
+
     for _ in range(args.warmup):
         seq.run()
     torch.cuda.synchronize()
 
     # timed region: K full sequences; HIP events on the launch stream bracket each stage
     stream = torch.cuda.current_stream()
-    ev = {k: [] for k in ("cost", "sgm", "median", "fusion")}
+    ev = {k: [] for k in ("cost", "sgm", "filter", "fusion")}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -105,14 +133,13 @@ def main():
         for i in range(nkf):
             e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
             e[0].record(stream)
-            from probabilistic_mapping_amd import pipeline as pl
             pl.calc_cost(seq.images[i], [seq.quads[j] for j in seq.sources[i]], seq.homs[i],
                          seq.idep_min, seq.dep_sample, seq.D, seq.cost_type, out=seq.cost)
             e[1].record(stream)
-            pl.sgm_filter(seq.images[i], seq.cost, seq.knobs, seq.paths, seq.idep_min,
-                          seq.dep_sample, agg=seq.agg,
-                          depth=seq.dep_raw if seq.median else seq.dep)
+            pl.sgm(seq.images[i], seq.cost, seq.knobs, seq.paths, out=seq.agg)
             e[2].record(stream)
+            pl.filter_cost(seq.agg, seq.knobs["var_scale"], seq.idep_min, seq.dep_sample,
+                           out=seq.dep_raw if seq.median else seq.dep)
             if seq.median:
                 pl.median3x3(seq.dep_raw, out=seq.dep)
             e[3].record(stream)
@@ -135,10 +162,11 @@ def main():
         "cost": {"ms": ms["cost"], "bound": "valu", "achieved_tflops": cost_flop / (ms["cost"] * 1e-3) / 1e12,
                  "peak_tflops": FP32_VALU_PEAK_TFLOPS,
                  "hbm_gbs": cost_bytes / (ms["cost"] * 1e-3) / 1e9},
-        "sgm_filter": {"ms": ms["sgm"], "bound": "hbm", "achieved_gbs": sgm_gbs,
-                       "frac": sgm_gbs / HBM_PEAK_GBS,
-                       "compulsory_gbs": V * (bc + 4) / (ms["sgm"] * 1e-3) / 1e9},
-        "median": {"ms": ms["median"]},
+        "sgm": {"ms": ms["sgm"], "bound": "hbm", "achieved_gbs": sgm_gbs,
+                "frac": sgm_gbs / HBM_PEAK_GBS,
+                "compulsory_gbs": V * (bc + 4) / (ms["sgm"] * 1e-3) / 1e9},
+        "filter": {"ms": ms["filter"], "bound": "hbm", "median": bool(args.median),
+                   "achieved_gbs": (V * 4 + N * 4) / (ms["filter"] * 1e-3) / 1e9},
         "fusion": {"ms": ms["fusion"], "bound": "hbm",
                    "achieved_gbs": fus_bytes / (ms["fusion"] * 1e-3) / 1e9},
     }
@@ -161,10 +189,11 @@ def main():
                                "sequence per step",
                    "form": "plain", "cost_storage": args.cost_type,
                    "median": bool(args.median), "keyframes_per_step": nkf},
-        "roofline": {"bound": "hbm", "kernel": "sgm pass (8 path launches, filter folded in)",
+        "roofline": {"bound": "hbm", "kernel": "sgm pass (8 path launches)",
                      "achieved": sgm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sgm_gbs / HBM_PEAK_GBS, "traffic": None},
         "stages": stages,
+        "attainable_copy": copy,
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

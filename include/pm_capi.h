@@ -116,10 +116,8 @@ pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t num_sources,
 pm_status pm_sgm(pm_shape shape, const uint8_t* ref, const void* cost, pm_cost_type cost_type,
                  pm_sgm_params params, float* agg, pm_stream stream);
 
-/* pm_sgm followed by pm_filter_cost, with the filter folded into the last path kernel: the
- * final aggregate of each pixel is reduced across the wave that wrote it, so the aggregate is
- * not read back.  agg (complete, as pm_sgm) and depth (identical to pm_filter_cost's) are both
- * written.  Replaces sgm2 + filter_cost (stereo_mapper.cpp:118-126). */
+/* pm_sgm followed by pm_filter_cost on the same stream (the sgm2 + filter_cost pair of
+ * StereoMapper::output, stereo_mapper.cpp:118-126). */
 pm_status pm_sgm_filter(pm_shape shape, const uint8_t* ref, const void* cost,
                         pm_cost_type cost_type, pm_sgm_params params, float var_scale,
                         float idep_min, float dep_sample, float* agg, float* depth,

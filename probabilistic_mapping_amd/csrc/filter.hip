@@ -2,51 +2,66 @@
 //
 // Reference: filterCostKernel + filter_cost, motion_stereo/filter_cost.cu:3-70.  The reference
 // launches one 64-thread block per pixel and reduces (value, index) pairs through shared memory
-// with log2(D) barriers.  Here one thread owns a pixel and scans its D contiguous aggregate
-// values with 16-byte loads; the strict '<' scan gives the lowest plane index on ties.
+// with log2(D) barriers.  Here a pixel's D aggregate values are read by a group of LP lanes,
+// four consecutive planes per lane (16-byte loads), so a wave reads 64/LP whole pixels as one
+// contiguous stream; the argmin is reduced across the group with lane shuffles and ties go to
+// the lowest plane index.
 #include "common.hpp"
 
 namespace pm {
 
+template <int LP>
 __global__ __launch_bounds__(256) void filter_kernel(long N, int D, bool vec4,
                                                      const float* __restrict__ agg,
                                                      float var_scale, float idep_min,
                                                      float dep_sample, float* __restrict__ depth) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  const float* c = agg + p * D;
-  float best = c[0];
-  int mi = 0;
-  if (vec4) {
-    const float4* c4 = reinterpret_cast<const float4*>(c);
-    for (int g = 0; g < D / 4; ++g) {
-      const float4 v = c4[g];
-      if (v.x < best) { best = v.x; mi = 4 * g; }
-      if (v.y < best) { best = v.y; mi = 4 * g + 1; }
-      if (v.z < best) { best = v.z; mi = 4 * g + 2; }
-      if (v.w < best) { best = v.w; mi = 4 * g + 3; }
-    }
+  const int lane = threadIdx.x & 63;
+  const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int grp = lane / LP, k = lane % LP;
+  const long p = wave * (64 / LP) + grp;
+  const long pc = p < N ? p : N - 1;   // out-of-range groups re-read the last pixel
+  const float* c = agg + pc * D;
+  const float INF = __uint_as_float(0x7F800000u);
+  const int d0 = 4 * k;
+  float v[4];
+  if (vec4 && d0 + 3 < D) {
+    const float4 t = *reinterpret_cast<const float4*>(c + d0);
+    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
   } else {
-    for (int d = 1; d < D; ++d)
-      if (c[d] < best) { best = c[d]; mi = d; }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = d0 + j < D ? c[d0 + j] : INF;
   }
+  float best = v[0];
+  int mi = d0;
+#pragma unroll
+  for (int j = 1; j < 4; ++j)
+    if (v[j] < best) { best = v[j]; mi = d0 + j; }
+#pragma unroll
+  for (int off = LP / 2; off >= 1; off >>= 1) {
+    const float ov = __shfl_xor(best, off, 64);
+    const int oi = __shfl_xor(mi, off, 64);
+    if (ov < best || (ov == best && oi < mi)) { best = ov; mi = oi; }
+  }
+  // neighbours of the minimum, fetched from the lanes that hold them
+  const int dm = mi > 0 ? mi - 1 : 0, dp = mi + 1 < D ? mi + 1 : D - 1;
+  const float sm = (dm & 3) == 0 ? v[0] : (dm & 3) == 1 ? v[1] : (dm & 3) == 2 ? v[2] : v[3];
+  const float sp = (dp & 3) == 0 ? v[0] : (dp & 3) == 1 ? v[1] : (dp & 3) == 2 ? v[2] : v[3];
+  const float cm = __shfl(sm, grp * LP + dm / 4, 64);
+  const float cp = __shfl(sp, grp * LP + dp / 4, 64);
   float out;
   if ((double)best < 1e-5 && (double)best > -1e-5) {          // rule 1, filter_cost.cu:37-38
     out = kDepInf;
   } else if (mi == 0 || mi == D - 1) {                        // rule 2, :39-40
     out = 0.0f;
-  } else {
-    const float cm = c[mi - 1], cp = c[mi + 1];
-    if (cm + cp < 2.0f * best * var_scale) {                  // rule 3, :41-42
-      out = 0.0f;
-    } else {                                                  // parabola, :45-51
-      const float A = cm - 2.0f * best + cp;
-      const float Bq = -cm + cp;
-      const float s = (float)mi - Bq / (2.0f * A);
-      out = 1.0f / (idep_min + s * dep_sample);
-    }
+  } else if (cm + cp < 2.0f * best * var_scale) {             // rule 3, :41-42
+    out = 0.0f;
+  } else {                                                    // parabola, :45-51
+    const float A = cm - 2.0f * best + cp;
+    const float Bq = -cm + cp;
+    const float s = (float)mi - Bq / (2.0f * A);
+    out = 1.0f / (idep_min + s * dep_sample);
   }
-  depth[p] = out;
+  if (k == 0 && p < N) depth[p] = out;
 }
 
 __device__ __forceinline__ bool valid_depth(float z) { return z > 0.0f && z < kDepInf; }
@@ -147,8 +162,18 @@ extern "C" pm_status pm_filter_cost(pm_shape shape, const float* agg, float var_
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_filter_cost: planes must be in 1..256");
   const long N = (long)shape.width * shape.height;
   const bool vec4 = (shape.planes % 4 == 0) && ((uintptr_t)agg % 16 == 0);
-  filter_kernel<<<(unsigned)((N + 255) / 256), 256, 0, (hipStream_t)stream>>>(
-      N, shape.planes, vec4, agg, var_scale, idep_min, dep_sample, depth);
+  const int D = shape.planes;
+  const int LP = D <= 16 ? 4 : D <= 32 ? 8 : D <= 64 ? 16 : D <= 128 ? 32 : 64;
+  const long waves = (N + 64 / LP - 1) / (64 / LP);
+  const unsigned blocks = (unsigned)((waves + 3) / 4);
+  hipStream_t st = (hipStream_t)stream;
+  switch (LP) {
+    case 4: filter_kernel<4><<<blocks, 256, 0, st>>>(N, D, vec4, agg, var_scale, idep_min, dep_sample, depth); break;
+    case 8: filter_kernel<8><<<blocks, 256, 0, st>>>(N, D, vec4, agg, var_scale, idep_min, dep_sample, depth); break;
+    case 16: filter_kernel<16><<<blocks, 256, 0, st>>>(N, D, vec4, agg, var_scale, idep_min, dep_sample, depth); break;
+    case 32: filter_kernel<32><<<blocks, 256, 0, st>>>(N, D, vec4, agg, var_scale, idep_min, dep_sample, depth); break;
+    default: filter_kernel<64><<<blocks, 256, 0, st>>>(N, D, vec4, agg, var_scale, idep_min, dep_sample, depth); break;
+  }
   return check_launch("filter_kernel");
 }
 

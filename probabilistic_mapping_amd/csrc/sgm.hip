@@ -32,53 +32,7 @@ struct SgmArgs {
   float P1n, P2n;   // penalties where |I(p) - I(q)| >= tau_so
   float P1s, P2s;   // penalties scaled by q1, q2 where |I(p) - I(q)| < tau_so
   int tau_i;        // integer form of the test: |dI| < tau_so  <=>  |dI| < ceil(tau_so)
-  // winner-take-all + subpixel filter fused into the last path (filter_cost.cu:3-55)
-  float* depth;
-  float var_scale, idep_min, dep_sample;
 };
-
-// Winner-take-all, rejection rules and parabola fit of one pixel whose final aggregate is
-// spread over the wave (lane l holds planes NV*l .. NV*l+NV-1).  Same decisions and float
-// arithmetic as filter_kernel: the lowest plane index wins ties.
-template <int NV>
-__device__ __forceinline__ void wta_filter_pixel(const SgmArgs& a, const float (&out)[NV],
-                                                 const bool (&act)[NV], int lane, long pix) {
-  const float INF = __uint_as_float(0x7F800000u);
-  float lv = act[0] ? out[0] : INF;
-  int li = 0;
-#pragma unroll
-  for (int j = 1; j < NV; ++j)
-    if (act[j] && out[j] < lv) { lv = out[j]; li = j; }
-  const float best = wave_min_nonneg(lv);
-  const uint64_t hit = __ballot(lv == best);
-  const int wl = __builtin_ctzll(hit);                         // lowest lane holding the min
-  const int mi = wl * NV + __builtin_amdgcn_readlane(li, wl);  // lowest plane holding it
-  float dep;
-  if ((double)best < 1e-5 && (double)best > -1e-5) {          // rule 1, filter_cost.cu:37-38
-    dep = kDepInf;
-  } else if (mi == 0 || mi == a.D - 1) {                      // rule 2, :39-40
-    dep = 0.0f;
-  } else {
-    const int dm = mi - 1, dp = mi + 1;
-    float vm = out[0], vp = out[0];
-#pragma unroll
-    for (int j = 1; j < NV; ++j) {
-      vm = (dm % NV == j) ? out[j] : vm;
-      vp = (dp % NV == j) ? out[j] : vp;
-    }
-    const float cm = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(vm), dm / NV));
-    const float cp = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(vp), dp / NV));
-    if (cm + cp < 2.0f * best * a.var_scale) {                 // rule 3, :41-42
-      dep = 0.0f;
-    } else {                                                   // parabola, :45-51
-      const float A = cm - 2.0f * best + cp;
-      const float Bq = -cm + cp;
-      const float sub = (float)mi - Bq / (2.0f * A);
-      dep = 1.0f / (a.idep_min + sub * a.dep_sample);
-    }
-  }
-  if (lane == 0) a.depth[pix] = dep;
-}
 
 __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, int& y) {
   if (a.dx != 0) {
@@ -97,7 +51,7 @@ __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, 
 // load address is clamped in bounds (past-the-end steps re-read the last pixel, lanes past D
 // re-read plane D-1 of the same pixel), so the prefetch has no branches and the B loads of
 // the next block stay in flight while the current block computes.
-template <int NV, typename CT, bool FIRST, bool FULL, bool FILT, int B>
+template <int NV, typename CT, bool FIRST, bool FULL, int B>
 __global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
                                                        const CT* __restrict__ cost,
                                                        float* __restrict__ agg) {
@@ -175,7 +129,7 @@ __global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
     const float lo = wave_shr1(L[NV - 1], INF);   // L(q, d0-1)
     const float hi = wave_shl1(L[0], INF);        // L(q, d0+NV)
     const float mP2 = m + P2;
-    float Ln[NV], out[NV];
+    float Ln[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
       const float left = j > 0 ? L[j - 1] : lo;
@@ -184,12 +138,11 @@ __global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
       val = umin(val, umin(left, right) + P1);     // sgm.cu:86-93
       const float cc = invalid ? kCostInf : c[i][j];
       Ln[j] = act[j] ? (cc + val) - m : INF;       // sgm.cu:98
-      out[j] = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);
-      if (act[j]) agg[p + off[j]] = out[j];        // sgm.cu:101-108
+      const float out = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);
+      if (act[j]) agg[p + off[j]] = out;           // sgm.cu:101-108
     }
 #pragma unroll
     for (int j = 0; j < NV; ++j) L[j] = Ln[j];
-    if (FILT) wta_filter_pixel<NV>(a, out, act, lane, p0 + k * pstep);
   };
   // a block whose B steps all exist: no exits, so the loads of the other buffer stay in flight
   auto run_full = [&](int base, const float (&c)[B][NV], const float (&g)[B][NV], uint32_t r) {
@@ -221,14 +174,12 @@ __global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
 #pragma unroll
     for (int j = 0; j < NV; ++j) neg = neg || c[j] < 0.0f;
     const bool invalid = __ballot(neg) != 0;
-    float out[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
       L[j] = act[j] ? (invalid ? kCostInf : c[j]) : INF;
-      out[j] = invalid ? kCostInf : (FIRST ? L[j] : g[j] + L[j]);
-      if (act[j]) agg[p + off[j]] = out[j];
+      const float out = invalid ? kCostInf : (FIRST ? L[j] : g[j] + L[j]);
+      if (act[j]) agg[p + off[j]] = out;
     }
-    if (FILT) wta_filter_pixel<NV>(a, out, act, lane, p0);
   }
   int base = 1;
   load_block(base, c0, g0, r0);
@@ -256,13 +207,10 @@ pm_status launch_paths(const SgmArgs& base, int paths, const CT* cost, float* ag
     a.dy = kPathDy[path];
     a.nlines = (a.dx != 0 && a.dy != 0) ? a.W + a.H - 1 : (a.dx != 0 ? a.H : a.W);
     const unsigned blocks = (unsigned)((a.nlines + 3) / 4);
-    const bool filt = a.depth != nullptr && path == paths - 1;
     if (path == 0)
-      sgm_path_kernel<NV, CT, true, FULL, false, B><<<blocks, 256, 0, st>>>(a, cost, agg);
-    else if (!filt)
-      sgm_path_kernel<NV, CT, false, FULL, false, B><<<blocks, 256, 0, st>>>(a, cost, agg);
+      sgm_path_kernel<NV, CT, true, FULL, B><<<blocks, 256, 0, st>>>(a, cost, agg);
     else
-      sgm_path_kernel<NV, CT, false, FULL, true, B><<<blocks, 256, 0, st>>>(a, cost, agg);
+      sgm_path_kernel<NV, CT, false, FULL, B><<<blocks, 256, 0, st>>>(a, cost, agg);
     pm_status s = check_launch("sgm_path_kernel");
     if (s != PM_OK) return s;
   }
@@ -279,8 +227,7 @@ pm_status launch_sgm(const SgmArgs& a, int paths, const CT* cost, float* agg, hi
 }
 
 pm_status sgm_entry(const char* who, pm_shape shape, const uint8_t* ref, const void* cost,
-                    pm_cost_type cost_type, pm_sgm_params params, float* agg, float* depth,
-                    float var_scale, float idep_min, float dep_sample, hipStream_t st) {
+                    pm_cost_type cost_type, pm_sgm_params params, float* agg, hipStream_t st) {
   const std::string w(who);
   if (!ref || !cost || !agg) return fail(PM_ERR_INVALID_ARGUMENT, w + ": null pointer");
   if (shape.width <= 0 || shape.height <= 0 || shape.width > 65535 || shape.height > 65535)
@@ -301,8 +248,6 @@ pm_status sgm_entry(const char* who, pm_shape shape, const uint8_t* ref, const v
   a.P1s = params.pi1 * params.sgm_q1;   // the same float products as sgm.cu:80-81
   a.P2s = params.pi2 * params.sgm_q2;
   a.tau_i = (int)std::min(256.0f, std::ceil(params.tau_so));
-  a.depth = depth;
-  a.var_scale = var_scale; a.idep_min = idep_min; a.dep_sample = dep_sample;
   switch (cost_type) {
     case PM_COST_F32: return launch_sgm<float>(a, params.paths, (const float*)cost, agg, st);
     case PM_COST_F16: return launch_sgm<__half>(a, params.paths, (const __half*)cost, agg, st);
@@ -320,8 +265,7 @@ extern "C" pm_status pm_sgm(pm_shape shape, const uint8_t* ref, const void* cost
                             pm_cost_type cost_type, pm_sgm_params params, float* agg,
                             pm_stream stream) {
   clear_error();
-  return sgm_entry("pm_sgm", shape, ref, cost, cost_type, params, agg, nullptr, 0.0f, 0.0f,
-                   0.0f, (hipStream_t)stream);
+  return sgm_entry("pm_sgm", shape, ref, cost, cost_type, params, agg, (hipStream_t)stream);
 }
 
 extern "C" pm_status pm_sgm_filter(pm_shape shape, const uint8_t* ref, const void* cost,
@@ -330,6 +274,8 @@ extern "C" pm_status pm_sgm_filter(pm_shape shape, const uint8_t* ref, const voi
                                    float* agg, float* depth, pm_stream stream) {
   clear_error();
   if (!depth) return fail(PM_ERR_INVALID_ARGUMENT, "pm_sgm_filter: null depth");
-  return sgm_entry("pm_sgm_filter", shape, ref, cost, cost_type, params, agg, depth, var_scale,
-                   idep_min, dep_sample, (hipStream_t)stream);
+  const pm_status s = sgm_entry("pm_sgm_filter", shape, ref, cost, cost_type, params, agg,
+                                (hipStream_t)stream);
+  if (s != PM_OK) return s;
+  return pm_filter_cost(shape, agg, var_scale, idep_min, dep_sample, depth, stream);
 }

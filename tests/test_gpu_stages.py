@@ -119,7 +119,7 @@ def test_sgm_zero_penalties_is_path_count_times_cost(dev, small_scene):
     assert parity.summary(parity.rel_err(g[valid], expect[valid], 1.0))["max"] <= 1e-5
 
 
-@pytest.mark.parametrize("planes", [32, 64, 96])
+@pytest.mark.parametrize("planes", [5, 16, 32, 64, 96, 128, 200])
 def test_filter_same_agg(dev, small_scene, planes):
     s = small_scene
     K, homs, _, _ = _small_inputs(s)
@@ -203,9 +203,8 @@ def test_fusion_same_depths(dev, trajectory_scene):
 
 @pytest.mark.parametrize("paths", [4, 8])
 @pytest.mark.parametrize("planes", [32, 64, 96, 128])
-def test_sgm_filter_fused_equals_separate(dev, small_scene, paths, planes):
-    """The filter folded into the last SGM path gives the same aggregate and bit-identical
-    depth as pm_sgm followed by pm_filter_cost."""
+def test_sgm_filter_equals_sgm_then_filter(dev, small_scene, paths, planes):
+    """pm_sgm_filter gives the same aggregate and depth as pm_sgm then pm_filter_cost."""
     s = small_scene
     K, homs, _, _ = _small_inputs(s)
     ds = (s["idep_max"] - s["idep_min"]) / (planes - 1)
