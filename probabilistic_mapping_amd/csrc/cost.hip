@@ -18,6 +18,9 @@
 #ifndef PM_COST_TILE_ROWS
 #define PM_COST_TILE_ROWS 4
 #endif
+#ifndef PM_COST_STAGE_PLANES
+#define PM_COST_STAGE_PLANES 8
+#endif
 
 namespace pm {
 
@@ -77,9 +80,37 @@ __device__ __forceinline__ float warp_sample(const uint32_t* __restrict__ quad, 
 // so each centre sample yields |I(q)-S|, |I(q+(0,2))-S| and |I(q-(0,2))-S|, kept in a 3-row
 // register window; column neighbours come from DPP lane shifts.  An invalid warp is +inf, so
 // a source contributes iff the 9-term sum is finite (calc_cost.cu:89-91).
-template <typename CT, int TY>
+// Output staging: a wave finishes DC planes for its TY x 62 pixels, parks them in its own LDS
+// tile [TY][DC][64] (lanes contiguous, conflict-free), then writes each pixel's DC contiguous
+// planes with consecutive lanes, so every store instruction writes whole aligned segments
+// instead of 64 isolated 4-byte words 4*D bytes apart.
+template <typename CT>
+__device__ __forceinline__ void store_quad(CT* dst, float a, float b, float c, float d);
+template <>
+__device__ __forceinline__ void store_quad<float>(float* dst, float a, float b, float c,
+                                                  float d) {
+  *reinterpret_cast<float4*>(dst) = make_float4(a, b, c, d);
+}
+template <>
+__device__ __forceinline__ void store_quad<__half>(__half* dst, float a, float b, float c,
+                                                   float d) {
+  const __half2 lo = __floats2half2_rn(a, b), hi = __floats2half2_rn(c, d);
+  uint2 u;
+  u.x = *reinterpret_cast<const uint32_t*>(&lo);
+  u.y = *reinterpret_cast<const uint32_t*>(&hi);
+  *reinterpret_cast<uint2*>(dst) = u;
+}
+template <>
+__device__ __forceinline__ void store_quad<__hip_bfloat16>(__hip_bfloat16* dst, float a,
+                                                           float b, float c, float d) {
+  dst[0] = __float2bfloat16(a); dst[1] = __float2bfloat16(b);
+  dst[2] = __float2bfloat16(c); dst[3] = __float2bfloat16(d);
+}
+
+template <typename CT, int TY, int DC>
 __global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, int nstrips,
                                                         CT* __restrict__ cost) {
+  __shared__ float stage[4][TY * DC * 64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -87,7 +118,6 @@ __global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, int ns
   const int y0 = (blk / nstrips) * TY;
   const int W = a.W, H = a.H, D = a.D;
   const int u = strip * 62 - 1 + lane;
-  const bool writer = lane >= 1 && lane <= 62 && u < W;
   const int dper = (D + 3) / 4;
   const int dbeg = wave * dper, dend = min(D, dbeg + dper);
   const float Wm1 = (float)(W - 1), Hm1 = (float)(H - 1);
@@ -98,7 +128,12 @@ __global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, int ns
 #pragma unroll
   for (int k = 0; k < TY + 6; ++k) I[k] = texel_u8(a.ref, W, H, u, y0 - 3 + k);
 
-  for (int d = dbeg; d < dend; ++d) {
+  float* tile = stage[wave];
+  // vector stores need D and the chunk start to be multiples of 4 (16-byte / 8-byte aligned)
+  const bool vec = (D % 4 == 0) && (dper % 4 == 0);
+  for (int dc0 = dbeg; dc0 < dend; dc0 += DC) {
+  const int dcn = min(DC, dend - dc0);
+  for (int d = dc0; d < dc0 + dcn; ++d) {
     const float idep = a.idep_min + (float)d * a.dep_sample;
     float acc[TY], cnt[TY];
 #pragma unroll
@@ -147,16 +182,34 @@ __global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, int ns
         U0 = U1; U1 = U2;
       }
     }
-    if (writer) {
 #pragma unroll
-      for (int t = 0; t < TY; ++t) {
-        const int v = y0 + t;
-        if (v < H) {
-          const float c = cnt[t] > 0.0f ? acc[t] / (9.0f * cnt[t]) : -1.0f;
-          cost[((long)v * W + u) * D + d] = store_cost<CT>(c);
-        }
+    for (int t = 0; t < TY; ++t)   // mean over sources of (sum of 9 terms)/9, or -1
+      tile[(t * DC + (d - dc0)) * 64 + lane] = cnt[t] > 0.0f ? acc[t] / (9.0f * cnt[t]) : -1.0f;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int x0 = strip * 62;
+  if (vec) {
+    // items: (row t, pixel 0..61, quad of 4 planes); consecutive lanes cover one pixel's planes
+    const int qp = dcn / 4, items = TY * 62 * qp;
+    for (int it = lane; it < items; it += 64) {
+      const int q = it % qp, rest = it / qp, px = rest % 62, t = rest / 62;
+      const int v = y0 + t, x = x0 + px;
+      if (v < H && x < W) {
+        const float* src = tile + (t * DC + 4 * q) * 64 + px + 1;
+        store_quad<CT>(cost + ((long)v * W + x) * D + dc0 + 4 * q, src[0], src[64], src[128],
+                       src[192]);
       }
     }
+  } else {
+    const int items = TY * 62 * dcn;
+    for (int it = lane; it < items; it += 64) {
+      const int j = it % dcn, rest = it / dcn, px = rest % 62, t = rest / 62;
+      const int v = y0 + t, x = x0 + px;
+      if (v < H && x < W)
+        cost[((long)v * W + x) * D + dc0 + j] = store_cost<CT>(tile[(t * DC + j) * 64 + px + 1]);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -204,18 +257,19 @@ extern "C" pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t nu
         return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: non-finite homography");
   }
   constexpr int TY = PM_COST_TILE_ROWS;
+  constexpr int DC = PM_COST_STAGE_PLANES;
   const int nstrips = (a.W + 61) / 62;
   const unsigned blocks = (unsigned)(nstrips * ((a.H + TY - 1) / TY));
   hipStream_t st = (hipStream_t)stream;
   switch (cost_type) {
     case PM_COST_F32:
-      cost_tile_kernel<float, TY><<<blocks, 256, 0, st>>>(a, nstrips, (float*)cost);
+      cost_tile_kernel<float, TY, DC><<<blocks, 256, 0, st>>>(a, nstrips, (float*)cost);
       break;
     case PM_COST_F16:
-      cost_tile_kernel<__half, TY><<<blocks, 256, 0, st>>>(a, nstrips, (__half*)cost);
+      cost_tile_kernel<__half, TY, DC><<<blocks, 256, 0, st>>>(a, nstrips, (__half*)cost);
       break;
     case PM_COST_BF16:
-      cost_tile_kernel<__hip_bfloat16, TY><<<blocks, 256, 0, st>>>(a, nstrips,
+      cost_tile_kernel<__hip_bfloat16, TY, DC><<<blocks, 256, 0, st>>>(a, nstrips,
                                                                    (__hip_bfloat16*)cost);
       break;
     default:
