@@ -26,6 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+# HBM bytes of one SGM pass from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) per dispatch,
+# calibrated on a 1 GiB copy (profiles/depth_pipeline_752x480x64_s5_p8_f32.md)
+SGM_PASS_TRAFFIC = {("f32", 8, 480, 752, 64): 2.15e9}
 FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed)
 
 
@@ -191,7 +194,9 @@ def main():
                    "median": bool(args.median), "keyframes_per_step": nkf},
         "roofline": {"bound": "hbm", "kernel": "sgm pass (8 path launches)",
                      "achieved": sgm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": sgm_gbs / HBM_PEAK_GBS, "traffic": None},
+                     "frac": sgm_gbs / HBM_PEAK_GBS,
+                     "algorithmic_bytes": sgm_bytes,
+                     "traffic": SGM_PASS_TRAFFIC.get((args.cost_type, P, H, W, D))},
         "stages": stages,
         "attainable_copy": copy,
         "cpu_baseline": base,
