@@ -158,6 +158,55 @@ pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam, const floa
  * stereo_mapper.cpp:143-144). */
 pm_status pm_fill_f32(float* dst, size_t n, float value, pm_stream stream);
 
+/* ---- batched form (no reference counterpart: the reference runs one keyframe at a time) --
+ * A batch covers up to 64 keyframes of any size, source count and plane count.  pm_batch_run
+ * enqueues cost -> SGM -> filter for all of them with one cost launch, one launch per SGM
+ * direction and plane class, and one filter launch per lane class, each with persistent
+ * blocks striding over a work list; results equal the single-keyframe entry points'.
+ * Fusion is sequential in time and stays with pm_depth_fuse. */
+#define PM_MAX_BATCH 64
+
+typedef struct pm_keyframe {
+  int32_t width, height, planes, num_sources;
+  float idep_min, dep_sample;
+  const uint8_t* ref;                          /* device uint8 [H][W] */
+  const uint32_t* src_quads[PM_MAX_SOURCES];   /* device quad textures */
+  pm_homography homs[PM_MAX_SOURCES];
+  void* cost;                                  /* device [H][W][D] of the batch cost type */
+  float* agg;                                  /* device [H][W][D] */
+  float* depth;                                /* device [H][W] */
+} pm_keyframe;
+
+/* Launch shapes (picked by pm_batch_autotune): cost tile rows 2 or 4, LDS staging chunk of 8
+ * or 16 planes, SGM prefetch depth 8 or 16 steps, and persistent grids of blocks_per_cu x
+ * (compute units) blocks (0 = one block per work item). */
+typedef struct pm_batch_tuning {
+  int32_t cost_tile_rows;
+  int32_t cost_stage_planes;
+  int32_t sgm_prefetch;
+  int32_t cost_blocks_per_cu;
+  int32_t sgm_blocks_per_cu;
+} pm_batch_tuning;
+
+typedef struct pm_batch pm_batch;
+
+/* Copies the keyframe table and work lists to the device (allocates; call once). */
+pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count, pm_cost_type cost_type,
+                          pm_sgm_params params, float var_scale, pm_batch** out);
+void pm_batch_destroy(pm_batch* b);
+pm_status pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t);
+pm_status pm_batch_get_tuning(const pm_batch* b, pm_batch_tuning* t);
+/* Enqueues cost, SGM and filter for every keyframe on stream. */
+pm_status pm_batch_run(pm_batch* b, pm_stream stream);
+/* One stage of pm_batch_run: 0 = cost, 1 = SGM (all paths), 2 = filter. */
+pm_status pm_batch_run_stage(pm_batch* b, int32_t stage, pm_stream stream);
+/* Times the candidate launch shapes on this batch and keeps the fastest (overwrites outputs). */
+pm_status pm_batch_autotune(pm_batch* b, pm_stream stream, int32_t reps, pm_batch_tuning* best);
+/* Records pm_batch_run into a hipGraph (stream must not be the default stream) ... */
+pm_status pm_batch_capture(pm_batch* b, pm_stream stream);
+/* ... and replays it. */
+pm_status pm_batch_launch(pm_batch* b, pm_stream stream);
+
 /* ---- StereoMapper handle (motion_stereo/stereo_mapper.h:26-63) ------------------------
  * The C++ class lives in include/pm/stereo_mapper.h; these wrappers expose it to C and
  * Python.  Images and results are HOST arrays; the mapper owns its device buffers. */

@@ -58,6 +58,24 @@ class FusionState(ctypes.Structure):
                 ("sig", ctypes.c_void_p)]
 
 
+PM_MAX_BATCH = 64
+
+
+class Keyframe(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("planes", ctypes.c_int32), ("num_sources", ctypes.c_int32),
+                ("idep_min", ctypes.c_float), ("dep_sample", ctypes.c_float),
+                ("ref", ctypes.c_void_p), ("src_quads", ctypes.c_void_p * 16),
+                ("homs", Homography * 16), ("cost", ctypes.c_void_p), ("agg", ctypes.c_void_p),
+                ("depth", ctypes.c_void_p)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("cost_tile_rows", ctypes.c_int32), ("cost_stage_planes", ctypes.c_int32),
+                ("sgm_prefetch", ctypes.c_int32), ("cost_blocks_per_cu", ctypes.c_int32),
+                ("sgm_blocks_per_cu", ctypes.c_int32)]
+
+
 class Knobs(ctypes.Structure):
     _fields_ = [("pi1", ctypes.c_float), ("pi2", ctypes.c_float), ("tau_so", ctypes.c_float),
                 ("sgm_q1", ctypes.c_float), ("sgm_q2", ctypes.c_float),
@@ -89,6 +107,16 @@ SIGNATURES = {
     "pm_depth_fuse": (_i32, [_i32, _i32, Camera, _fp, _fp, _vp, _vp, _vp, FusionState,
                              FusionState, _vp]),
     "pm_fill_f32": (_i32, [_vp, ctypes.c_size_t, ctypes.c_float, _vp]),
+    "pm_batch_create": (_i32, [ctypes.POINTER(Keyframe), _i32, _i32, SgmParams, ctypes.c_float,
+                               ctypes.POINTER(_vp)]),
+    "pm_batch_destroy": (None, [_vp]),
+    "pm_batch_set_tuning": (_i32, [_vp, ctypes.POINTER(BatchTuning)]),
+    "pm_batch_get_tuning": (_i32, [_vp, ctypes.POINTER(BatchTuning)]),
+    "pm_batch_run": (_i32, [_vp, _vp]),
+    "pm_batch_run_stage": (_i32, [_vp, _i32, _vp]),
+    "pm_batch_autotune": (_i32, [_vp, _vp, _i32, ctypes.POINTER(BatchTuning)]),
+    "pm_batch_capture": (_i32, [_vp, _vp]),
+    "pm_batch_launch": (_i32, [_vp, _vp]),
     "pm_mapper_create": (_i32, [_i32, _i32, _i32, ctypes.POINTER(_vp)]),
     "pm_mapper_destroy": (None, [_vp]),
     "pm_mapper_init_intrinsic": (_i32, [_vp, _dp, ctypes.c_float, ctypes.c_float]),

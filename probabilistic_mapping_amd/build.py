@@ -19,7 +19,7 @@ HIP_LIB = os.path.join(HERE, "libpm_hip.so")
 SYNTH_LIB = os.path.join(HERE, "libpm_synth.so")
 
 ARCH = "gfx950"
-HIP_SOURCES = ["cost.hip", "sgm.hip", "filter.hip", "fusion.hip", "capi.hip",
+HIP_SOURCES = ["cost.hip", "sgm.hip", "filter.hip", "fusion.hip", "batch.hip", "capi.hip",
                "stereo_mapper.cpp", "parameters.cpp"]
 HIP_HEADERS = ["common.hpp"]
 

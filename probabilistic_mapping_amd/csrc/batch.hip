@@ -1,0 +1,429 @@
+// Batched form: many keyframes of uneven size, source count and plane count per launch.
+//
+// A pm_batch owns a device table of keyframe descriptors and the work lists of each stage
+// (which keyframe owns which work item).  Running the batch enqueues one cost launch for all
+// keyframes, one launch per SGM direction and plane class (instead of 8 per keyframe), and one
+// filter launch per lane class; the launches use persistent blocks that stride over the work
+// list.  The enqueue sequence can be captured once into a hipGraph and replayed.  An
+// autotuner picks the cost tile shape, the SGM prefetch depth and the persistent grid sizes by
+// timing the stages on the batch itself.
+//
+// The stages are the reference's calc_cost / sgm2 / filter_cost (motion_stereo/calc_cost.cu,
+// sgm.cu, filter_cost.cu) as restated by the single-keyframe kernels; fusion stays per stream
+// (it is sequential in time) and is enqueued by the caller after the batch.
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace pm {
+
+namespace {
+
+int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+struct SgmClass {
+  int nv = 1;
+  bool full = false;
+  std::vector<int> kf;       // keyframe indices of the class
+  int kf_off = 0;            // offset of kf[] in the device int pool
+  int first_off[8] = {};     // offsets of first[] per path in the pool
+  int total[8] = {};
+};
+
+struct FilterClass {
+  int lp = 16;
+  std::vector<int> kf;
+  int kf_off = 0, first_off = 0, total = 0;
+};
+
+}  // namespace
+
+}  // namespace pm
+
+struct pm_batch {
+  int nkf = 0;
+  pm_cost_type ct = PM_COST_F32;
+  int paths = 4;
+  int cus = 256;
+  std::vector<pm::KfDesc> host;
+  pm::KfDesc* d_kfs = nullptr;
+  int* d_pool = nullptr;
+  int cost_first_off[2] = {0, 0};   // tile rows 2, 4
+  int cost_total[2] = {0, 0};
+  std::vector<pm::SgmClass> sgm;
+  std::vector<pm::FilterClass> filt;
+  pm_batch_tuning tune{4, 8, 16, 0, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+using namespace pm;
+
+namespace {
+
+int grid_for(int total, int blocks_per_cu, int cus) {
+  if (blocks_per_cu <= 0) return total;
+  return std::max(8, std::min(round_up(total, 8), round_up(blocks_per_cu * cus, 8)));
+}
+
+bool valid_tuning(const pm_batch_tuning& t) {
+  return (t.cost_tile_rows == 2 || t.cost_tile_rows == 4) &&
+         (t.cost_stage_planes == 8 || t.cost_stage_planes == 16) &&
+         (t.sgm_prefetch == 8 || t.sgm_prefetch == 16) && t.cost_blocks_per_cu >= 0 &&
+         t.sgm_blocks_per_cu >= 0 && t.cost_blocks_per_cu <= 64 && t.sgm_blocks_per_cu <= 64;
+}
+
+pm_status run_cost(pm_batch* b, hipStream_t st) {
+  const int ti = b->tune.cost_tile_rows == 2 ? 0 : 1;
+  const int total = b->cost_total[ti];
+  if (total == 0) return PM_OK;
+  if (!launch_cost_batch(b->ct, b->tune.cost_tile_rows, b->tune.cost_stage_planes, b->d_kfs,
+                         b->d_pool + b->cost_first_off[ti], b->nkf, total,
+                         grid_for(total, b->tune.cost_blocks_per_cu, b->cus), st))
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch: unsupported cost tuning");
+  return check_launch("cost_batch_kernel");
+}
+
+pm_status run_sgm(pm_batch* b, hipStream_t st) {
+  for (int p = 0; p < b->paths; ++p) {
+    for (const SgmClass& c : b->sgm) {
+      const int total = c.total[p];
+      if (total == 0) continue;
+      if (!launch_sgm_batch_path(b->ct, b->tune.sgm_prefetch, c.nv, c.full, b->d_kfs,
+                                 b->d_pool + c.kf_off, b->d_pool + c.first_off[p],
+                                 (int)c.kf.size(), total,
+                                 grid_for(total, b->tune.sgm_blocks_per_cu, b->cus), p == 0,
+                                 kPathDx[p], kPathDy[p], st))
+        return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch: unsupported sgm tuning");
+      const pm_status s = check_launch("sgm_batch_kernel");
+      if (s != PM_OK) return s;
+    }
+  }
+  return PM_OK;
+}
+
+pm_status run_filter(pm_batch* b, hipStream_t st) {
+  for (const FilterClass& c : b->filt) {
+    if (c.total == 0) continue;
+    launch_filter_batch(c.lp, b->d_kfs, b->d_pool + c.kf_off, b->d_pool + c.first_off,
+                        (int)c.kf.size(), c.total, c.total, st);
+    const pm_status s = check_launch("filter_batch_kernel");
+    if (s != PM_OK) return s;
+  }
+  return PM_OK;
+}
+
+void drop_graph(pm_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
+                                     pm_cost_type cost_type, pm_sgm_params params,
+                                     float var_scale, pm_batch** out) {
+  clear_error();
+  if (!out) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: null out");
+  *out = nullptr;
+  if (!kfs || count <= 0 || count > PM_MAX_BATCH)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: count must be in 1..64");
+  if (cost_type != PM_COST_F32 && cost_type != PM_COST_F16 && cost_type != PM_COST_BF16)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: unknown cost type");
+  if (params.paths != 4 && params.paths != 8)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: paths must be 4 or 8");
+  const float kn[6] = {params.pi1, params.pi2, params.tau_so, params.sgm_q1, params.sgm_q2,
+                       var_scale};
+  for (float v : kn)
+    if (!(std::isfinite(v) && v >= 0.0f))
+      return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: knobs must be finite and >= 0");
+  auto* b = new pm_batch();
+  b->nkf = count;
+  b->ct = cost_type;
+  b->paths = params.paths;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      cus > 0)
+    b->cus = cus;
+  b->host.resize(count);
+  for (int k = 0; k < count; ++k) {
+    const pm_keyframe& in = kfs[k];
+    const std::string who = "pm_batch_create: keyframe " + std::to_string(k) + ": ";
+    if (in.width <= 0 || in.height <= 0 || in.width > 65535 || in.height > 65535 ||
+        (long)in.width * in.height > (1L << 30)) {
+      delete b;
+      return fail(PM_ERR_INVALID_ARGUMENT, who + "bad image size");
+    }
+    if (in.planes < 1 || in.planes > PM_MAX_PLANES || in.num_sources < 0 ||
+        in.num_sources > PM_MAX_SOURCES) {
+      delete b;
+      return fail(PM_ERR_INVALID_ARGUMENT, who + "planes or num_sources out of range");
+    }
+    if (!in.ref || !in.cost || !in.agg || !in.depth ||
+        !std::isfinite(in.idep_min) || !std::isfinite(in.dep_sample)) {
+      delete b;
+      return fail(PM_ERR_INVALID_ARGUMENT, who + "null buffer or non-finite planes");
+    }
+    KfDesc& d = b->host[k];
+    d = KfDesc{};
+    d.c.W = in.width; d.c.H = in.height; d.c.D = in.planes; d.c.S = in.num_sources;
+    d.c.idep_min = in.idep_min; d.c.dep_sample = in.dep_sample; d.c.ref = in.ref;
+    for (int s = 0; s < in.num_sources; ++s) {
+      if (!in.src_quads[s]) {
+        delete b;
+        return fail(PM_ERR_INVALID_ARGUMENT, who + "null source quad");
+      }
+      d.c.quad[s] = in.src_quads[s];
+      for (int j = 0; j < 9; ++j) d.c.hm[s][j] = in.homs[s].h[j];
+      for (int j = 0; j < 3; ++j) d.c.hm[s][9 + j] = in.homs[s].t[j];
+      for (int j = 0; j < 12; ++j)
+        if (!std::isfinite(d.c.hm[s][j])) {
+          delete b;
+          return fail(PM_ERR_INVALID_ARGUMENT, who + "non-finite homography");
+        }
+    }
+    d.cost = in.cost; d.agg = in.agg; d.depth = in.depth;
+    d.P1n = params.pi1; d.P2n = params.pi2;
+    d.P1s = params.pi1 * params.sgm_q1;   // the same float products as sgm.cu:80-81
+    d.P2s = params.pi2 * params.sgm_q2;
+    d.tau_i = (int)std::min(256.0f, std::ceil(params.tau_so));
+    d.paths = params.paths;
+    d.var_scale = var_scale;
+  }
+  // work lists, packed in one int pool
+  std::vector<int> pool;
+  auto put = [&](const std::vector<int>& v) {
+    const int off = (int)pool.size();
+    pool.insert(pool.end(), v.begin(), v.end());
+    return off;
+  };
+  for (int ti = 0; ti < 2; ++ti) {
+    std::vector<int> first(count);
+    int acc = 0;
+    for (int k = 0; k < count; ++k) {
+      first[k] = acc;
+      acc += cost_items(b->host[k].c.W, b->host[k].c.H, ti == 0 ? 2 : 4);
+    }
+    b->cost_first_off[ti] = put(first);
+    b->cost_total[ti] = acc;
+  }
+  for (int k = 0; k < count; ++k) {
+    const int D = b->host[k].c.D;
+    const int nv = D <= 64 ? 1 : (D <= 128 ? 2 : 4);
+    const bool full = nv < 4 && D == 64 * nv;
+    auto it = std::find_if(b->sgm.begin(), b->sgm.end(),
+                           [&](const SgmClass& c) { return c.nv == nv && c.full == full; });
+    if (it == b->sgm.end()) {
+      b->sgm.push_back(SgmClass{});
+      it = b->sgm.end() - 1;
+      it->nv = nv;
+      it->full = full;
+    }
+    it->kf.push_back(k);
+    const int lp = filter_lanes(D);
+    auto ft = std::find_if(b->filt.begin(), b->filt.end(),
+                           [&](const FilterClass& c) { return c.lp == lp; });
+    if (ft == b->filt.end()) {
+      b->filt.push_back(FilterClass{});
+      ft = b->filt.end() - 1;
+      ft->lp = lp;
+    }
+    ft->kf.push_back(k);
+  }
+  for (SgmClass& c : b->sgm) {
+    c.kf_off = put(c.kf);
+    for (int p = 0; p < b->paths; ++p) {
+      std::vector<int> first(c.kf.size());
+      int acc = 0;
+      for (size_t j = 0; j < c.kf.size(); ++j) {
+        first[j] = acc;
+        const CostArgs& a = b->host[c.kf[j]].c;
+        acc += (sgm_lines(a.W, a.H, kPathDx[p], kPathDy[p]) + 3) / 4;
+      }
+      c.first_off[p] = put(first);
+      c.total[p] = acc;
+    }
+  }
+  for (FilterClass& c : b->filt) {
+    c.kf_off = put(c.kf);
+    std::vector<int> first(c.kf.size());
+    int acc = 0;
+    for (size_t j = 0; j < c.kf.size(); ++j) {
+      first[j] = acc;
+      const CostArgs& a = b->host[c.kf[j]].c;
+      acc += filter_items(a.W, a.H, a.D);
+    }
+    c.first_off = put(first);
+    c.total = acc;
+  }
+  hipError_t e = hipMalloc(&b->d_kfs, sizeof(KfDesc) * count);
+  if (e == hipSuccess) e = hipMalloc(&b->d_pool, sizeof(int) * std::max<size_t>(pool.size(), 1));
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_kfs, b->host.data(), sizeof(KfDesc) * count, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !pool.empty())
+    e = hipMemcpy(b->d_pool, pool.data(), sizeof(int) * pool.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    pm_batch_destroy(b);
+    return fail(e == hipErrorOutOfMemory ? PM_ERR_OUT_OF_MEMORY : PM_ERR_DEVICE,
+                std::string("pm_batch_create: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return PM_OK;
+}
+
+extern "C" void pm_batch_destroy(pm_batch* b) {
+  if (!b) return;
+  drop_graph(b);
+  if (b->d_kfs) (void)hipFree(b->d_kfs);
+  if (b->d_pool) (void)hipFree(b->d_pool);
+  delete b;
+}
+
+extern "C" pm_status pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t) {
+  clear_error();
+  if (!b || !t) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_set_tuning: null argument");
+  if (!valid_tuning(*t)) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_set_tuning: bad values");
+  b->tune = *t;
+  drop_graph(b);   // a captured graph holds the old launch shapes
+  return PM_OK;
+}
+
+extern "C" pm_status pm_batch_get_tuning(const pm_batch* b, pm_batch_tuning* t) {
+  clear_error();
+  if (!b || !t) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_get_tuning: null argument");
+  *t = b->tune;
+  return PM_OK;
+}
+
+extern "C" pm_status pm_batch_run(pm_batch* b, pm_stream stream) {
+  clear_error();
+  if (!b) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_run: null batch");
+  hipStream_t st = (hipStream_t)stream;
+  pm_status s = run_cost(b, st);
+  if (s == PM_OK) s = run_sgm(b, st);
+  if (s == PM_OK) s = run_filter(b, st);
+  return s;
+}
+
+extern "C" pm_status pm_batch_run_stage(pm_batch* b, int32_t stage, pm_stream stream) {
+  clear_error();
+  if (!b) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_run_stage: null batch");
+  hipStream_t st = (hipStream_t)stream;
+  switch (stage) {
+    case 0: return run_cost(b, st);
+    case 1: return run_sgm(b, st);
+    case 2: return run_filter(b, st);
+    default: return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_run_stage: stage must be 0, 1 or 2");
+  }
+}
+
+extern "C" pm_status pm_batch_capture(pm_batch* b, pm_stream stream) {
+  clear_error();
+  if (!b) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_capture: null batch");
+  if (!stream)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_capture: capture needs a non-default stream");
+  drop_graph(b);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess)
+    return fail(PM_ERR_DEVICE, std::string("pm_batch_capture: ") + hipGetErrorString(e));
+  const pm_status s = pm_batch_run(b, stream);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(st, &g);
+  if (s != PM_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return s;
+  }
+  if (e != hipSuccess)
+    return fail(PM_ERR_DEVICE, std::string("pm_batch_capture: ") + hipGetErrorString(e));
+  b->graph = g;
+  e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_graph(b);
+    return fail(PM_ERR_DEVICE, std::string("pm_batch_capture: ") + hipGetErrorString(e));
+  }
+  return PM_OK;
+}
+
+extern "C" pm_status pm_batch_launch(pm_batch* b, pm_stream stream) {
+  clear_error();
+  if (!b || !b->exec)
+    return fail(PM_ERR_INVALID_STATE, "pm_batch_launch: no captured graph");
+  const hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+  if (e != hipSuccess)
+    return fail(PM_ERR_LAUNCH, std::string("pm_batch_launch: ") + hipGetErrorString(e));
+  return PM_OK;
+}
+
+// Times each stage under every candidate setting (reps runs after one warm-up, mean per run)
+// and keeps the fastest per stage.  The batch's output buffers are overwritten.
+extern "C" pm_status pm_batch_autotune(pm_batch* b, pm_stream stream, int32_t reps,
+                                       pm_batch_tuning* best) {
+  clear_error();
+  if (!b) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_autotune: null batch");
+  if (reps < 1 || reps > 100) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_autotune: reps");
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(PM_ERR_DEVICE, "pm_batch_autotune: event creation failed");
+  pm_status status = PM_OK;
+  auto time_stage = [&](pm_status (*fn)(pm_batch*, hipStream_t), float* ms) {
+    pm_status s = fn(b, st);
+    if (s != PM_OK) return s;
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps && s == PM_OK; ++r) s = fn(b, st);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) return fail(PM_ERR_DEVICE, "autotune sync");
+    (void)hipEventElapsedTime(ms, e0, e1);
+    *ms /= (float)reps;
+    return s;
+  };
+  pm_batch_tuning cur = b->tune, win = b->tune;
+  float best_ms = 1e30f;
+  const int rows[2] = {2, 4}, planes[2] = {8, 16}, bpc[4] = {0, 4, 8, 16};
+  for (int r : rows)
+    for (int p : planes)
+      for (int g : bpc) {
+        if (status != PM_OK) break;
+        b->tune = cur;
+        b->tune.cost_tile_rows = r;
+        b->tune.cost_stage_planes = p;
+        b->tune.cost_blocks_per_cu = g;
+        float ms = 0;
+        status = time_stage(run_cost, &ms);
+        if (status == PM_OK && ms < best_ms) {
+          best_ms = ms;
+          win.cost_tile_rows = r;
+          win.cost_stage_planes = p;
+          win.cost_blocks_per_cu = g;
+        }
+      }
+  best_ms = 1e30f;
+  const int pref[2] = {8, 16};
+  for (int f : pref)
+    for (int g : bpc) {
+      if (status != PM_OK) break;
+      b->tune = win;
+      b->tune.sgm_prefetch = f;
+      b->tune.sgm_blocks_per_cu = g;
+      float ms = 0;
+      status = time_stage(run_sgm, &ms);
+      if (status == PM_OK && ms < best_ms) {
+        best_ms = ms;
+        win.sgm_prefetch = f;
+        win.sgm_blocks_per_cu = g;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tune = status == PM_OK ? win : cur;
+  drop_graph(b);
+  if (status == PM_OK && best) *best = win;
+  return status;
+}

@@ -22,6 +22,65 @@ constexpr int kWave = 64;
 constexpr int kPathDx[8] = {1, -1, 0, 0, 1, -1, 1, -1};
 constexpr int kPathDy[8] = {0, 0, 1, -1, 1, -1, -1, 1};
 
+// ---- per-keyframe stage arguments -------------------------------------------------------
+// Plane-sweep cost inputs of one keyframe (passed by value to the single-keyframe kernels,
+// read from a device table by the batched ones).
+struct CostArgs {
+  int W, H, D, S;
+  float idep_min, dep_sample;
+  const uint8_t* ref;
+  const uint32_t* quad[PM_MAX_SOURCES];
+  float hm[PM_MAX_SOURCES][12];   // h11..h33, t1..t3
+};
+
+// SGM inputs of one keyframe and one path direction.
+struct SgmArgs {
+  int W, H, D, dx, dy;
+  int nlines;
+  const uint8_t* ref;
+  float P1n, P2n;   // penalties where |I(p) - I(q)| >= tau_so
+  float P1s, P2s;   // penalties scaled by q1, q2 where |I(p) - I(q)| < tau_so
+  int tau_i;        // integer form of the test: |dI| < tau_so  <=>  |dI| < ceil(tau_so)
+};
+
+// One keyframe of a batch (device table entry).
+struct KfDesc {
+  CostArgs c;
+  void* cost;        // [H][W][D] of the batch's cost type
+  float* agg;        // [H][W][D]
+  float* depth;      // [H][W]
+  float P1n, P2n, P1s, P2s;
+  int tau_i, paths;
+  float var_scale;
+};
+
+__host__ __device__ inline int sgm_lines(int W, int H, int dx, int dy) {
+  return (dx != 0 && dy != 0) ? W + H - 1 : (dx != 0 ? H : W);
+}
+
+// Index of the keyframe owning work item b: the largest k with first[k] <= b
+// (first[] ascending, first[0] = 0; nkf <= 64).
+__device__ __forceinline__ int owner_of(const int* __restrict__ first, int nkf, int b) {
+  int lo = 0, hi = nkf - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---- batched launches (defined with their kernels; used by the batch host code) --------
+int cost_items(int W, int H, int tile_rows);
+bool launch_cost_batch(pm_cost_type ct, int tile_rows, int stage_planes, const KfDesc* kfs,
+                       const int* first, int nkf, int total, int grid, hipStream_t st);
+bool launch_sgm_batch_path(pm_cost_type ct, int prefetch, int nv, bool full, const KfDesc* kfs,
+                           const int* kf, const int* first, int n, int total, int grid,
+                           bool first_path, int dx, int dy, hipStream_t st);
+int filter_lanes(int D);
+int filter_items(int W, int H, int D);
+void launch_filter_batch(int LP, const KfDesc* kfs, const int* kf, const int* first, int n,
+                         int total, int grid, hipStream_t st);
+
 // ---- host error plumbing -------------------------------------------------------------
 void set_error(const std::string& msg);
 void clear_error();

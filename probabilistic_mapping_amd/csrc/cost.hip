@@ -24,13 +24,6 @@
 
 namespace pm {
 
-struct CostArgs {
-  int W, H, D, S;
-  float idep_min, dep_sample;
-  const uint8_t* ref;
-  const uint32_t* quad[PM_MAX_SOURCES];
-  float hm[PM_MAX_SOURCES][12];   // h11..h33, t1..t3
-};
 
 __global__ __launch_bounds__(256) void make_quad_kernel(const uint8_t* __restrict__ img,
                                                         uint32_t* __restrict__ quad, int W,
@@ -108,12 +101,11 @@ __device__ __forceinline__ void store_quad<__hip_bfloat16>(__hip_bfloat16* dst, 
 }
 
 template <typename CT, int TY, int DC>
-__global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, int nstrips,
-                                                        CT* __restrict__ cost) {
-  __shared__ float stage[4][TY * DC * 64];
+__device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __restrict__ cost,
+                                          float* __restrict__ tile) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int nstrips = (a.W + 61) / 62;
   const int strip = blk % nstrips;
   const int y0 = (blk / nstrips) * TY;
   const int W = a.W, H = a.H, D = a.D;
@@ -128,7 +120,6 @@ __global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, int ns
 #pragma unroll
   for (int k = 0; k < TY + 6; ++k) I[k] = texel_u8(a.ref, W, H, u, y0 - 3 + k);
 
-  float* tile = stage[wave];
   // vector stores need D and the chunk start to be multiples of 4 (16-byte / 8-byte aligned)
   const bool vec = (D % 4 == 0) && (dper % 4 == 0);
   for (int dc0 = dbeg; dc0 < dend; dc0 += DC) {
@@ -213,6 +204,72 @@ __global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, int ns
   }
 }
 
+// Tiles of a W x H keyframe: 62-column strips x TY-row blocks.
+template <int TY>
+__host__ __device__ inline int cost_tiles(int W, int H) {
+  return ((W + 61) / 62) * ((H + TY - 1) / TY);
+}
+
+template <typename CT, int TY, int DC>
+__global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, CT* __restrict__ cost) {
+  __shared__ float stage[4][TY * DC * 64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  cost_tile<CT, TY, DC>(a, xcd_block(blockIdx.x, gridDim.x), cost, stage[wave]);
+}
+
+// Batched form: the tiles of every keyframe of a batch form one work list (keyframe k owns
+// items first[k] .. first[k+1]-1).  Persistent blocks stride through it; with a grid that is a
+// multiple of 8, block i always runs on XCD i % 8, and xcd_block gives each XCD a contiguous
+// range of the list.
+template <typename CT, int TY, int DC>
+__global__ __launch_bounds__(256) void cost_batch_kernel(const KfDesc* __restrict__ kfs,
+                                                         const int* __restrict__ first, int nkf,
+                                                         int total) {
+  __shared__ float stage[4][TY * DC * 64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int i = blockIdx.x; i < total; i += gridDim.x) {
+    const int item = xcd_block(i, total);
+    const int k = owner_of(first, nkf, item);
+    cost_tile<CT, TY, DC>(kfs[k].c, item - first[k], static_cast<CT*>(kfs[k].cost),
+                          stage[wave]);
+  }
+}
+
+int cost_items(int W, int H, int tile_rows) {
+  return tile_rows == 2 ? cost_tiles<2>(W, H) : cost_tiles<4>(W, H);
+}
+
+template <typename CT>
+bool launch_cost_batch_t(int tile_rows, int stage_planes, const KfDesc* kfs, const int* first,
+                         int nkf, int total, int grid, hipStream_t st) {
+  if (tile_rows == 2 && stage_planes == 8)
+    cost_batch_kernel<CT, 2, 8><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+  else if (tile_rows == 2 && stage_planes == 16)
+    cost_batch_kernel<CT, 2, 16><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+  else if (tile_rows == 4 && stage_planes == 8)
+    cost_batch_kernel<CT, 4, 8><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+  else if (tile_rows == 4 && stage_planes == 16)
+    cost_batch_kernel<CT, 4, 16><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+  else
+    return false;
+  return true;
+}
+
+bool launch_cost_batch(pm_cost_type ct, int tile_rows, int stage_planes, const KfDesc* kfs,
+                       const int* first, int nkf, int total, int grid, hipStream_t st) {
+  switch (ct) {
+    case PM_COST_F32:
+      return launch_cost_batch_t<float>(tile_rows, stage_planes, kfs, first, nkf, total, grid, st);
+    case PM_COST_F16:
+      return launch_cost_batch_t<__half>(tile_rows, stage_planes, kfs, first, nkf, total, grid, st);
+    case PM_COST_BF16:
+      return launch_cost_batch_t<__hip_bfloat16>(tile_rows, stage_planes, kfs, first, nkf, total,
+                                                 grid, st);
+    default:
+      return false;
+  }
+}
+
 }  // namespace pm
 
 using namespace pm;
@@ -258,19 +315,17 @@ extern "C" pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t nu
   }
   constexpr int TY = PM_COST_TILE_ROWS;
   constexpr int DC = PM_COST_STAGE_PLANES;
-  const int nstrips = (a.W + 61) / 62;
-  const unsigned blocks = (unsigned)(nstrips * ((a.H + TY - 1) / TY));
+  const unsigned blocks = (unsigned)cost_tiles<TY>(a.W, a.H);
   hipStream_t st = (hipStream_t)stream;
   switch (cost_type) {
     case PM_COST_F32:
-      cost_tile_kernel<float, TY, DC><<<blocks, 256, 0, st>>>(a, nstrips, (float*)cost);
+      cost_tile_kernel<float, TY, DC><<<blocks, 256, 0, st>>>(a, (float*)cost);
       break;
     case PM_COST_F16:
-      cost_tile_kernel<__half, TY, DC><<<blocks, 256, 0, st>>>(a, nstrips, (__half*)cost);
+      cost_tile_kernel<__half, TY, DC><<<blocks, 256, 0, st>>>(a, (__half*)cost);
       break;
     case PM_COST_BF16:
-      cost_tile_kernel<__hip_bfloat16, TY, DC><<<blocks, 256, 0, st>>>(a, nstrips,
-                                                                   (__hip_bfloat16*)cost);
+      cost_tile_kernel<__hip_bfloat16, TY, DC><<<blocks, 256, 0, st>>>(a, (__hip_bfloat16*)cost);
       break;
     default:
       return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: unknown cost type");

@@ -11,12 +11,11 @@
 namespace pm {
 
 template <int LP>
-__global__ __launch_bounds__(256) void filter_kernel(long N, int D, bool vec4,
-                                                     const float* __restrict__ agg,
-                                                     float var_scale, float idep_min,
-                                                     float dep_sample, float* __restrict__ depth) {
+__device__ __forceinline__ void filter_pixels(long N, int D, bool vec4,
+                                              const float* __restrict__ agg, float var_scale,
+                                              float idep_min, float dep_sample,
+                                              float* __restrict__ depth, long wave) {
   const int lane = threadIdx.x & 63;
-  const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int grp = lane / LP, k = lane % LP;
   const long p = wave * (64 / LP) + grp;
   const long pc = p < N ? p : N - 1;   // out-of-range groups re-read the last pixel
@@ -62,6 +61,54 @@ __global__ __launch_bounds__(256) void filter_kernel(long N, int D, bool vec4,
     out = 1.0f / (idep_min + s * dep_sample);
   }
   if (k == 0 && p < N) depth[p] = out;
+}
+
+template <int LP>
+__global__ __launch_bounds__(256) void filter_kernel(long N, int D, bool vec4,
+                                                     const float* __restrict__ agg,
+                                                     float var_scale, float idep_min,
+                                                     float dep_sample, float* __restrict__ depth) {
+  const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  filter_pixels<LP>(N, D, vec4, agg, var_scale, idep_min, dep_sample, depth, wave);
+}
+
+// Batched form: keyframes kf[j] of one LP class own items first[j] .. first[j+1]-1, an item
+// being 4 waves of 64/LP pixels each.
+template <int LP>
+__global__ __launch_bounds__(256) void filter_batch_kernel(const KfDesc* __restrict__ kfs,
+                                                           const int* __restrict__ kf,
+                                                           const int* __restrict__ first, int n,
+                                                           int total) {
+  for (int i = blockIdx.x; i < total; i += gridDim.x) {
+    const int j = owner_of(first, n, i);
+    const KfDesc& k = kfs[kf[j]];
+    const long N = (long)k.c.W * k.c.H;
+    const int D = k.c.D;
+    const bool vec4 = (D % 4 == 0) && ((uintptr_t)k.agg % 16 == 0);
+    const long wave = (long)(i - first[j]) * 4 + (threadIdx.x >> 6);
+    filter_pixels<LP>(N, D, vec4, k.agg, k.var_scale, k.c.idep_min, k.c.dep_sample, k.depth,
+                      wave);
+  }
+}
+
+int filter_lanes(int D) { return D <= 16 ? 4 : D <= 32 ? 8 : D <= 64 ? 16 : D <= 128 ? 32 : 64; }
+
+int filter_items(int W, int H, int D) {
+  const long N = (long)W * H;
+  const int LP = filter_lanes(D);
+  const long waves = (N + 64 / LP - 1) / (64 / LP);
+  return (int)((waves + 3) / 4);
+}
+
+void launch_filter_batch(int LP, const KfDesc* kfs, const int* kf, const int* first, int n,
+                         int total, int grid, hipStream_t st) {
+  switch (LP) {
+    case 4: filter_batch_kernel<4><<<grid, 256, 0, st>>>(kfs, kf, first, n, total); break;
+    case 8: filter_batch_kernel<8><<<grid, 256, 0, st>>>(kfs, kf, first, n, total); break;
+    case 16: filter_batch_kernel<16><<<grid, 256, 0, st>>>(kfs, kf, first, n, total); break;
+    case 32: filter_batch_kernel<32><<<grid, 256, 0, st>>>(kfs, kf, first, n, total); break;
+    default: filter_batch_kernel<64><<<grid, 256, 0, st>>>(kfs, kf, first, n, total); break;
+  }
 }
 
 __device__ __forceinline__ bool valid_depth(float z) { return z > 0.0f && z < kDepInf; }
@@ -163,9 +210,8 @@ extern "C" pm_status pm_filter_cost(pm_shape shape, const float* agg, float var_
   const long N = (long)shape.width * shape.height;
   const bool vec4 = (shape.planes % 4 == 0) && ((uintptr_t)agg % 16 == 0);
   const int D = shape.planes;
-  const int LP = D <= 16 ? 4 : D <= 32 ? 8 : D <= 64 ? 16 : D <= 128 ? 32 : 64;
-  const long waves = (N + 64 / LP - 1) / (64 / LP);
-  const unsigned blocks = (unsigned)((waves + 3) / 4);
+  const int LP = filter_lanes(D);
+  const unsigned blocks = (unsigned)filter_items(shape.width, shape.height, D);
   hipStream_t st = (hipStream_t)stream;
   switch (LP) {
     case 4: filter_kernel<4><<<blocks, 256, 0, st>>>(N, D, vec4, agg, var_scale, idep_min, dep_sample, depth); break;

@@ -25,14 +25,6 @@
 
 namespace pm {
 
-struct SgmArgs {
-  int W, H, D, dx, dy;
-  int nlines;
-  const uint8_t* ref;
-  float P1n, P2n;   // penalties where |I(p) - I(q)| >= tau_so
-  float P1s, P2s;   // penalties scaled by q1, q2 where |I(p) - I(q)| < tau_so
-  int tau_i;        // integer form of the test: |dI| < tau_so  <=>  |dI| < ceil(tau_so)
-};
 
 __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, int& y) {
   if (a.dx != 0) {
@@ -52,13 +44,10 @@ __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, 
 // re-read plane D-1 of the same pixel), so the prefetch has no branches and the B loads of
 // the next block stay in flight while the current block computes.
 template <int NV, typename CT, bool FIRST, bool FULL, int B>
-__global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
-                                                       const CT* __restrict__ cost,
-                                                       float* __restrict__ agg) {
+__device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
+                                             const CT* __restrict__ cost,
+                                             float* __restrict__ agg) {
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int line = blockIdx.x * 4 + wave;
-  if (line >= a.nlines) return;
   int x, y;
   scanline_start(a, line, x, y);
   int n = 1 << 30;
@@ -197,6 +186,87 @@ __global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
   }
 }
 
+template <int NV, typename CT, bool FIRST, bool FULL, int B>
+__global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
+                                                       const CT* __restrict__ cost,
+                                                       float* __restrict__ agg) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int line = blockIdx.x * 4 + wave;
+  if (line >= a.nlines) return;
+  sgm_scanline<NV, CT, FIRST, FULL, B>(a, line, cost, agg);
+}
+
+// Batched form of one path direction: the scanlines of every keyframe of a class (same NV,
+// FULL) form one work list of 4-scanline items; keyframe kf[j] owns items first[j] ..
+// first[j+1]-1.  Persistent blocks stride through the list.
+template <int NV, typename CT, bool FIRST, bool FULL, int B>
+__global__ __launch_bounds__(256) void sgm_batch_kernel(const KfDesc* __restrict__ kfs,
+                                                        const int* __restrict__ kf,
+                                                        const int* __restrict__ first, int n,
+                                                        int total, int dx, int dy) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int i = blockIdx.x; i < total; i += gridDim.x) {
+    const int j = owner_of(first, n, i);
+    const KfDesc& k = kfs[kf[j]];
+    SgmArgs a;
+    a.W = k.c.W; a.H = k.c.H; a.D = k.c.D; a.dx = dx; a.dy = dy;
+    a.nlines = sgm_lines(a.W, a.H, dx, dy);
+    a.ref = k.c.ref;
+    a.P1n = k.P1n; a.P2n = k.P2n; a.P1s = k.P1s; a.P2s = k.P2s; a.tau_i = k.tau_i;
+    const int line = (i - first[j]) * 4 + wave;
+    if (line < a.nlines)
+      sgm_scanline<NV, CT, FIRST, FULL, B>(a, line, static_cast<const CT*>(k.cost), k.agg);
+  }
+}
+
+template <int NV, typename CT, bool FULL, int B>
+void sgm_batch_path_t(const KfDesc* kfs, const int* kf, const int* first, int n, int total,
+                      int grid, bool first_path, int dx, int dy, hipStream_t st) {
+  if (first_path)
+    sgm_batch_kernel<NV, CT, true, FULL, B><<<grid, 256, 0, st>>>(kfs, kf, first, n, total, dx, dy);
+  else
+    sgm_batch_kernel<NV, CT, false, FULL, B><<<grid, 256, 0, st>>>(kfs, kf, first, n, total, dx, dy);
+}
+
+template <typename CT, int B>
+bool sgm_batch_path_b(int nv, bool full, const KfDesc* kfs, const int* kf, const int* first,
+                      int n, int total, int grid, bool first_path, int dx, int dy,
+                      hipStream_t st) {
+  if (nv == 1 && full) sgm_batch_path_t<1, CT, true, B>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+  else if (nv == 1) sgm_batch_path_t<1, CT, false, B>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+  else if (nv == 2 && full) sgm_batch_path_t<2, CT, true, B>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+  else if (nv == 2) sgm_batch_path_t<2, CT, false, B>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+  else if (nv == 4) sgm_batch_path_t<4, CT, false, B>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+  else return false;
+  return true;
+}
+
+template <typename CT>
+bool sgm_batch_path_c(int prefetch, int nv, bool full, const KfDesc* kfs, const int* kf,
+                      const int* first, int n, int total, int grid, bool first_path, int dx,
+                      int dy, hipStream_t st) {
+  if (prefetch == 8)
+    return sgm_batch_path_b<CT, 8>(nv, full, kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+  if (prefetch == 16)
+    return sgm_batch_path_b<CT, 16>(nv, full, kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+  return false;
+}
+
+bool launch_sgm_batch_path(pm_cost_type ct, int prefetch, int nv, bool full, const KfDesc* kfs,
+                           const int* kf, const int* first, int n, int total, int grid,
+                           bool first_path, int dx, int dy, hipStream_t st) {
+  switch (ct) {
+    case PM_COST_F32:
+      return sgm_batch_path_c<float>(prefetch, nv, full, kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+    case PM_COST_F16:
+      return sgm_batch_path_c<__half>(prefetch, nv, full, kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+    case PM_COST_BF16:
+      return sgm_batch_path_c<__hip_bfloat16>(prefetch, nv, full, kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+    default:
+      return false;
+  }
+}
+
 template <int NV, typename CT, bool FULL>
 pm_status launch_paths(const SgmArgs& base, int paths, const CT* cost, float* agg,
                        hipStream_t st) {
@@ -205,7 +275,7 @@ pm_status launch_paths(const SgmArgs& base, int paths, const CT* cost, float* ag
     SgmArgs a = base;
     a.dx = kPathDx[path];
     a.dy = kPathDy[path];
-    a.nlines = (a.dx != 0 && a.dy != 0) ? a.W + a.H - 1 : (a.dx != 0 ? a.H : a.W);
+    a.nlines = sgm_lines(a.W, a.H, a.dx, a.dy);
     const unsigned blocks = (unsigned)((a.nlines + 3) / 4);
     if (path == 0)
       sgm_path_kernel<NV, CT, true, FULL, B><<<blocks, 256, 0, st>>>(a, cost, agg);

@@ -229,6 +229,7 @@ class KeyframeSequence:
         return self.dep
 
     def fusion_stage(self, i: int) -> None:
+        """One outputFusion step on ``self.dep`` (stereo_mapper.cpp:128-243)."""
         R, t = self.poses[i]
         if self.first:
             self.first = False
@@ -258,3 +259,111 @@ class KeyframeSequence:
     def state(self) -> dict:
         """The state written by the last fusion step."""
         return self.states[self.pre]
+
+
+class Batch:
+    """A ``pm_batch``: cost -> SGM -> filter for many keyframes of uneven size per launch.
+
+    ``keyframes`` is a list of dicts with ``ref`` (uint8 device tensor), ``quads`` (device quad
+    textures of the measurement frames), ``homs`` ([(h, t)]), ``planes``, ``idep_min``,
+    ``dep_sample``.  Output tensors ``cost``, ``agg`` and ``depth`` are allocated per keyframe.
+    """
+
+    def __init__(self, keyframes, cost_type, knobs, paths):
+        self.lib = C.load()
+        self.cost_type = cost_type
+        code, dt = COST_TYPES[cost_type]
+        n = len(keyframes)
+        self.kfs = keyframes
+        arr = (C.Keyframe * n)()
+        self.outputs = []
+        for i, kf in enumerate(keyframes):
+            H, W = kf["ref"].shape
+            D = kf["planes"]
+            dev = kf["ref"].device
+            out = {"cost": torch.empty((H, W, D), dtype=dt, device=dev),
+                   "agg": torch.empty((H, W, D), dtype=torch.float32, device=dev),
+                   "depth": torch.empty((H, W), dtype=torch.float32, device=dev)}
+            self.outputs.append(out)
+            k = arr[i]
+            k.width, k.height, k.planes = W, H, D
+            k.num_sources = len(kf["quads"])
+            k.idep_min, k.dep_sample = float(kf["idep_min"]), float(kf["dep_sample"])
+            k.ref = kf["ref"].data_ptr()
+            for s, (q, (h, t)) in enumerate(zip(kf["quads"], kf["homs"])):
+                k.src_quads[s] = q.data_ptr()
+                k.homs[s].h[:] = [float(v) for v in np.asarray(h, np.float32).reshape(9)]
+                k.homs[s].t[:] = [float(v) for v in np.asarray(t, np.float32).reshape(3)]
+            k.cost, k.agg, k.depth = (out["cost"].data_ptr(), out["agg"].data_ptr(),
+                                      out["depth"].data_ptr())
+        p = C.SgmParams(knobs["pi1"], knobs["pi2"], knobs["tau_so"], knobs["sgm_q1"],
+                        knobs["sgm_q2"], paths)
+        h = ctypes.c_void_p()
+        C.check(self.lib.pm_batch_create(arr, n, code, p, float(knobs["var_scale"]),
+                                         ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.pm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def tuning(self) -> dict:
+        t = C.BatchTuning()
+        C.check(self.lib.pm_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in C.BatchTuning._fields_}
+
+    @tuning.setter
+    def tuning(self, values: dict):
+        t = C.BatchTuning(**{**self.tuning, **values})
+        C.check(self.lib.pm_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def run(self):
+        C.check(self.lib.pm_batch_run(self._h, _stream()))
+
+    def run_stage(self, stage: int):
+        """0 = cost, 1 = SGM (all paths), 2 = filter."""
+        C.check(self.lib.pm_batch_run_stage(self._h, int(stage), _stream()))
+
+    def autotune(self, reps: int = 3) -> dict:
+        t = C.BatchTuning()
+        C.check(self.lib.pm_batch_autotune(self._h, _stream(), reps, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in C.BatchTuning._fields_}
+
+    def capture(self):
+        C.check(self.lib.pm_batch_capture(self._h, _stream()))
+
+    def launch(self):
+        C.check(self.lib.pm_batch_launch(self._h, _stream()))
+
+
+class BatchedSequence(KeyframeSequence):
+    """The keyframe sequence with every keyframe's cost, SGM and filter in one batch, then the
+    fusion steps in order (fusion is sequential in time, stereo_mapper.cpp:128-243)."""
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        H, W, D = self.H, self.W, self.D
+        self.cost = self.agg = None   # per keyframe buffers live in the batch
+        torch.cuda.empty_cache()
+        specs = [{"ref": self.images[i], "quads": [self.quads[j] for j in self.sources[i]],
+                  "homs": self.homs[i], "planes": D, "idep_min": self.idep_min,
+                  "dep_sample": self.dep_sample} for i in range(len(self.images))]
+        self.batch = Batch(specs, self.cost_type, self.knobs, self.paths)
+
+    def depth_of(self, i: int) -> torch.Tensor:
+        return self.batch.outputs[i]["depth"]
+
+    def fusion_step(self, i: int) -> None:
+        self.dep = self.depth_of(i)
+        self.fusion_stage(i)
+
+    def run(self) -> None:
+        self.reset()
+        self.batch.run()
+        for i in range(len(self.images)):
+            self.fusion_step(i)

@@ -1,0 +1,151 @@
+"""Batched form: uneven keyframes per launch (needs a GPU).
+
+The batch runs the same stage bodies as the single-keyframe entry points, so its outputs must
+be bit-identical to them for every launch shape the autotuner can pick and through hipGraph
+replay.  The FP16 / BF16 storage forms are then measured against the FP64 oracle.
+"""
+import json
+
+import numpy as np
+import pytest
+
+from oracle.sequence import OracleSequence, homography as np_homography
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+KNOBS = dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+
+
+def _report(name, stats):
+    print(f"[parity] {name}: {json.dumps(stats)}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def uneven():
+    """Four keyframes of different size, plane count and source count."""
+    from probabilistic_mapping_amd import pipeline, synth
+    specs = []
+    for seed, (W, H, D, S, fx) in enumerate([(320, 240, 32, 2, 250.0), (160, 120, 96, 3, 110.0),
+                                              (200, 100, 128, 4, 130.0), (256, 128, 64, 5, 170.0)]):
+        sc = synth.trajectory_scene(seed + 7, width=W, height=H, n_frames=S + 1, n_sources=S,
+                                    planes=D, fx=fx)
+        K = sc["scene"].K
+        imgs = [torch.from_numpy(im).cuda() for im in sc["images"]]
+        quads = [pipeline.make_quad(im) for im in imgs]
+        homs = [np_homography(K, *sc["poses"][0], *sc["poses"][j]) for j in sc["sources"][0]]
+        specs.append({"ref": imgs[0], "quads": [quads[j] for j in sc["sources"][0]],
+                      "homs": homs, "planes": D, "idep_min": sc["idep_min"],
+                      "dep_sample": (sc["idep_max"] - sc["idep_min"]) / (D - 1)})
+    return specs
+
+
+def _single(pipeline, kf, cost_type, paths):
+    cost = pipeline.calc_cost(kf["ref"], kf["quads"], kf["homs"], kf["idep_min"],
+                              kf["dep_sample"], kf["planes"], cost_type)
+    agg = pipeline.sgm(kf["ref"], cost, KNOBS, paths)
+    dep = pipeline.filter_cost(agg, KNOBS["var_scale"], kf["idep_min"], kf["dep_sample"])
+    return {"cost": cost, "agg": agg, "depth": dep}
+
+
+def _assert_equal(batch, expect):
+    for i, (got, want) in enumerate(zip(batch.outputs, expect)):
+        for k in ("cost", "agg", "depth"):
+            assert torch.equal(got[k], want[k]), (i, k)
+
+
+@pytest.mark.parametrize("cost_type,paths", [("f32", 8), ("f16", 8), ("bf16", 4)])
+def test_batch_equals_single_keyframe(uneven, cost_type, paths):
+    from probabilistic_mapping_amd import pipeline
+    expect = [_single(pipeline, kf, cost_type, paths) for kf in uneven]
+    batch = pipeline.Batch(uneven, cost_type, KNOBS, paths)
+    batch.run()
+    torch.cuda.synchronize()
+    _assert_equal(batch, expect)
+    # every launch shape gives the same bits
+    for rows in (2, 4):
+        for planes in (8, 16):
+            for pref in (8, 16):
+                for bpc in (0, 4):
+                    for o in batch.outputs:
+                        for v in o.values():
+                            v.fill_(7)
+                    batch.tuning = dict(cost_tile_rows=rows, cost_stage_planes=planes,
+                                        sgm_prefetch=pref, cost_blocks_per_cu=bpc,
+                                        sgm_blocks_per_cu=bpc)
+                    batch.run()
+                    torch.cuda.synchronize()
+                    _assert_equal(batch, expect)
+
+
+def test_batch_autotune_and_graph_replay(uneven):
+    from probabilistic_mapping_amd import pipeline
+    expect = [_single(pipeline, kf, "f16", 8) for kf in uneven]
+    batch = pipeline.Batch(uneven, "f16", KNOBS, 8)
+    best = batch.autotune(reps=2)
+    print(f"[autotune] {json.dumps(best)}", flush=True)
+    assert best == batch.tuning
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        batch.capture()
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for o in batch.outputs:          # eager work between replays
+            for v in o.values():
+                v.fill_(rep + 1)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            batch.launch()
+        side.synchronize()
+        _assert_equal(batch, expect)
+
+
+def test_batch_rejects_bad_input(uneven):
+    from probabilistic_mapping_amd import _capi as C, pipeline
+    bad = dict(uneven[0], planes=0)
+    with pytest.raises(C.PmError):
+        pipeline.Batch([bad], "f32", KNOBS, 8)
+    with pytest.raises(C.PmError):
+        pipeline.Batch(uneven, "f32", KNOBS, 6)
+    b = pipeline.Batch(uneven[:1], "f32", KNOBS, 4)
+    with pytest.raises(C.PmError):
+        b.tuning = dict(cost_tile_rows=3)
+    with pytest.raises(C.PmError):
+        b.launch()   # nothing captured
+
+
+@pytest.mark.parametrize("cost_type", ["f16", "bf16"])
+def test_storage_precision_error_report(trajectory_scene, cost_type):
+    """Cost volume stored in FP16 / BF16, SGM and fusion accumulated in FP32, FP64 homography
+    setup: errors of the whole batched pipeline against the FP64 oracle on the first 5
+    keyframes of the 752x480x64, 5-source, 8-path sequence."""
+    from probabilistic_mapping_amd.pipeline import BatchedSequence
+    s = trajectory_scene
+    n = 5
+    args = (s["images"][:n], s["poses"][:n], [[j for j in src if j < n] or [1]
+                                               for src in s["sources"][:n]],
+            s["scene"].K, s["planes"], s["idep_min"], s["idep_max"], s["knobs"], s["paths"])
+    g = BatchedSequence(*args, cost_type=cost_type)
+    o = OracleSequence(*args, precision="f64")
+    g.reset()
+    g.batch.run()
+    report = []
+    for i in range(n):
+        g.fusion_step(i)
+        o.run_keyframe(i)
+        agg = g.batch.outputs[i]["agg"].cpu().numpy()
+        st = {"wta_agree": float((parity.wta_index(agg) == parity.wta_index(o.agg)).mean()),
+              "depth": parity.depth_parity(g.depth_of(i).cpu().numpy(), o.depth)}
+        if i > 0:
+            st["state"] = parity.state_parity({k: v.cpu().numpy() for k, v in g.state.items()},
+                                              o.state)
+        report.append(st)
+        _report(f"{cost_type} kf{i}", st)
+    depth_p999 = max(r["depth"]["rel"]["p999"] for r in report)
+    if cost_type == "f16":
+        # the accuracy bar of the FP16-storage form: <= 1e-3 relative depth error
+        assert depth_p999 <= 1e-3
+        assert min(r["wta_agree"] for r in report) >= 0.995
+    else:
+        assert min(r["wta_agree"] for r in report) >= 0.98
