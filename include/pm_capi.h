@@ -13,8 +13,10 @@
  *
  * Data layouts (device memory):
  *   image   : uint8 [H][W]                  (row-major, pitch = W)
- *   quad    : uint32 [H][W]  texel (x,y) packs I(x,y), I(x+1,y), I(x,y+1), I(x+1,y+1) in bytes
- *             0..3, with 0 outside the image -- one load per bilinear sample
+ *   quad    : uint64 [H][W]  texel (x,y) holds four IEEE FP16 numbers, low half first:
+ *             a = I(x,y), I(x+1,y) - a, I(x,y+1) - a, I(x+1,y+1) - I(x+1,y) - I(x,y+1) + a,
+ *             with I = 0 outside the image (exact: all are integers in [-510, 510]) --
+ *             one 8-byte load and three FMAs per bilinear sample
  *   cost    : [H][W][D] pixel-major, D contiguous (the reference's INDEX(y,x,d),
  *             motion_stereo/cuda_utils.h:34), element type per pm_cost_type
  *   agg     : float [H][W][D], same layout
@@ -96,7 +98,7 @@ pm_status pm_homography_setup(const double K[9], const double R_l[9], const doub
 /* Builds the quad texture of an image (border 0).  Replaces the texture binding of
  * ad_calc_cost (motion_stereo/calc_cost.cu:221-244) and the float upload of
  * StereoMapper::update (stereo_mapper.cpp:74-77). */
-pm_status pm_make_quad(const uint8_t* image, uint32_t* quad, int32_t width, int32_t height,
+pm_status pm_make_quad(const uint8_t* image, uint64_t* quad, int32_t width, int32_t height,
                        pm_stream stream);
 
 /* Plane-sweep matching cost fused over all measurement frames of one keyframe.
@@ -104,9 +106,10 @@ pm_status pm_make_quad(const uint8_t* image, uint32_t* quad, int32_t width, int3
  * calc_cost.cu:11-253) on a volume reset to -1 (stereo_mapper.cpp:69-70).
  * idep of plane d = idep_min + d * dep_sample (the reference has idep_min = 0).
  * src_quads and homs are HOST arrays of num_sources entries (the quads are device buffers).
- * cost: device [H][W][D] of cost_type; -1 where no source saw the (pixel, plane). */
+ * cost: device [H][W][D] of cost_type; -1 where no source saw the (pixel, plane).
+ * width * height <= 2^29 (texel byte offsets are 32-bit). */
 pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t num_sources,
-                       const uint32_t* const* src_quads, const pm_homography* homs,
+                       const uint64_t* const* src_quads, const pm_homography* homs,
                        float idep_min, float dep_sample, void* cost, pm_cost_type cost_type,
                        pm_stream stream);
 
@@ -170,7 +173,7 @@ typedef struct pm_keyframe {
   int32_t width, height, planes, num_sources;
   float idep_min, dep_sample;
   const uint8_t* ref;                          /* device uint8 [H][W] */
-  const uint32_t* src_quads[PM_MAX_SOURCES];   /* device quad textures */
+  const uint64_t* src_quads[PM_MAX_SOURCES];   /* device quad textures */
   pm_homography homs[PM_MAX_SOURCES];
   void* cost;                                  /* device [H][W][D] of the batch cost type */
   float* agg;                                  /* device [H][W][D] */

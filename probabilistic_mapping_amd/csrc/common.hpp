@@ -29,7 +29,7 @@ struct CostArgs {
   int W, H, D, S;
   float idep_min, dep_sample;
   const uint8_t* ref;
-  const uint32_t* quad[PM_MAX_SOURCES];
+  const uint64_t* quad[PM_MAX_SOURCES];   // difference texels, see pm_make_quad
   float hm[PM_MAX_SOURCES][12];   // h11..h33, t1..t3
 };
 
@@ -135,6 +135,17 @@ __device__ __forceinline__ float wave_shr1(float v, float fill) {
 // lane i receives lane i+1's value; lane 63 receives `fill`
 __device__ __forceinline__ float wave_shl1(float v, float fill) {
   return __uint_as_float(dpp_u32<0x130>(__float_as_uint(fill), __float_as_uint(v)));
+}
+
+// Lane shifts whose edge lane receives 0; the compiler folds these into the consuming VALU
+// instruction as a DPP source operand.
+__device__ __forceinline__ float wave_shr1z(float v) {
+  return __uint_as_float((uint32_t)__builtin_amdgcn_update_dpp(0, (int)__float_as_uint(v), 0x138,
+                                                               0xF, 0xF, true));
+}
+__device__ __forceinline__ float wave_shl1z(float v) {
+  return __uint_as_float((uint32_t)__builtin_amdgcn_update_dpp(0, (int)__float_as_uint(v), 0x130,
+                                                               0xF, 0xF, true));
 }
 
 // Minimum over the 64 lanes of a non-negative float, returned wave-uniform.

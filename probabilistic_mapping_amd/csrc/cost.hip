@@ -11,8 +11,12 @@
 // computed once per (plane, source) and shared with the neighbours whose reference terms use
 // it, so a (pixel, plane, source) costs about 3.25 warps instead of 9.
 //
-// Bilinear sampling reads the "quad" texture: one uint32 per texel packing
-// I(x,y), I(x+1,y), I(x,y+1), I(x+1,y+1) (border 0), so a sample is a single dword gather.
+// Bilinear sampling reads the "quad" texture of a measurement frame: 8 bytes per texel holding
+// the bilinear patch from (x, y) to (x+1, y+1) as four FP16 numbers
+//   a = I(x,y),  bx = I(x+1,y) - a,  by = I(x,y+1) - a,  bxy = I(x+1,y+1) - I(x+1,y) - by
+// (border 0).  They are integers in [-510, 510], exact in FP16, and the sample
+//   a + ax bx + ay (by + ax bxy)
+// is three FMAs with FP16 operands widened in the instruction (v_fma_mix_f32).
 #include "common.hpp"
 
 #ifndef PM_COST_TILE_ROWS
@@ -26,39 +30,46 @@ namespace pm {
 
 
 __global__ __launch_bounds__(256) void make_quad_kernel(const uint8_t* __restrict__ img,
-                                                        uint32_t* __restrict__ quad, int W,
+                                                        uint64_t* __restrict__ quad, int W,
                                                         int H) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= W) return;
-  const uint32_t i00 = img[(long)y * W + x];
-  const uint32_t i10 = x + 1 < W ? img[(long)y * W + x + 1] : 0u;
-  const uint32_t i01 = y + 1 < H ? img[(long)(y + 1) * W + x] : 0u;
-  const uint32_t i11 = (x + 1 < W && y + 1 < H) ? img[(long)(y + 1) * W + x + 1] : 0u;
-  quad[(long)y * W + x] = i00 | (i10 << 8) | (i01 << 16) | (i11 << 24);
+  const int i00 = img[(long)y * W + x];
+  const int i10 = x + 1 < W ? img[(long)y * W + x + 1] : 0;
+  const int i01 = y + 1 < H ? img[(long)(y + 1) * W + x] : 0;
+  const int i11 = (x + 1 < W && y + 1 < H) ? img[(long)(y + 1) * W + x + 1] : 0;
+  const __half2 lo = __floats2half2_rn((float)i00, (float)(i10 - i00));
+  const __half2 hi = __floats2half2_rn((float)(i01 - i00), (float)(i11 - i10 - i01 + i00));
+  uint2 t;
+  t.x = *reinterpret_cast<const uint32_t*>(&lo);
+  t.y = *reinterpret_cast<const uint32_t*>(&hi);
+  reinterpret_cast<uint2*>(quad)[(long)y * W + x] = t;
 }
 
 // Bilinear sample of the quad texture at (U, V) with the warp's validity (calc_cost.cu:85-93);
-// returns +inf for an invalid warp.  The test is written positively so a NaN warp (0/0) is
-// invalid; invalid warps read texel 0 so the gather stays in bounds.
-__device__ __forceinline__ float warp_sample(const uint32_t* __restrict__ quad, int W, float Wm1,
-                                             float Hm1, float X, float Y, float w) {
+// returns +inf for an invalid warp.  Validity is "the clamp to the image changes nothing" and
+// w >= 0, so a NaN warp (0/0) is invalid; an invalid warp still reads an in-bounds texel (the
+// clamped position), so every gather of a row can be in flight at once.  The byte offset fits
+// 32 bits (W H <= 2^29 is checked on the host), which lets the gather use the scalar base
+// address + 32-bit vector offset form.
+__device__ __forceinline__ float warp_sample(const uint64_t* __restrict__ quad, uint32_t W,
+                                             float Wm1, float Hm1, float X, float Y, float w) {
   const float rw = __builtin_amdgcn_rcpf(w);
   const float U = X * rw;
   const float V = Y * rw;
-  // bitwise (not short-circuit) tests and an unconditional gather: the sample is always
-  // used, so the compiler keeps every gather of a row in flight instead of branching
-  const bool valid = (w >= 0.0f) & (U >= 0.0f) & (U <= Wm1) & (V >= 0.0f) & (V <= Hm1);
-  const float Uc = valid ? U : 0.0f, Vc = valid ? V : 0.0f;
-  const int x0 = (int)Uc, y0 = (int)Vc;   // non-negative, so truncation is floor
-  const float ax = Uc - (float)x0, ay = Vc - (float)y0;
-  const uint32_t q = quad[y0 * W + x0];
-  const float i00 = (float)(q & 0xFFu), i10 = (float)((q >> 8) & 0xFFu);
-  const float i01 = (float)((q >> 16) & 0xFFu), i11 = (float)(q >> 24);
-  const float top = i00 + ax * (i10 - i00);
-  const float bot = i01 + ax * (i11 - i01);
-  const float v = top + ay * (bot - top);
-  return v + (valid ? 0.0f : __uint_as_float(0x7F800000u));
+  const float Uc = __builtin_amdgcn_fmed3f(U, 0.0f, Wm1);
+  const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
+  const bool valid = (w >= 0.0f) & (Uc == U) & (Vc == V);
+  // non-negative, so truncation is floor; W, y <= 65535 fit the 24-bit multiply
+  const uint32_t off = __umul24((uint32_t)Vc, W) * 8u + (uint32_t)Uc * 8u;
+  const uint2 t = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(quad) + off);
+  const float ax = __builtin_amdgcn_fractf(Uc), ay = __builtin_amdgcn_fractf(Vc);
+  const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
+  const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
+  const float top = fmaf(ax, __high2float(lo), __low2float(lo));
+  const float dv = fmaf(ax, __high2float(hi), __low2float(hi));
+  return valid ? fmaf(ay, dv, top) : __uint_as_float(0x7F800000u);
 }
 
 // Tile kernel.  A wave covers a strip of 64 columns u = x0-1 .. x0+62 (62 output columns and
@@ -115,10 +126,13 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
   const float Wm1 = (float)(W - 1), Hm1 = (float)(H - 1);
   const float INF = __uint_as_float(0x7F800000u);
   const float fu = (float)u;
-  // reference intensities of this column for rows y0-3 .. y0+TY+2 (0 outside the image)
-  float I[TY + 6];
+  // reference intensities of this column for rows y0-3 .. y0+TY+2 (0 outside the image), and
+  // of the right-hand neighbour column for the garbled terms' rows y0-1 .. y0+TY-1
+  float I[TY + 6], IR[TY + 1];
 #pragma unroll
   for (int k = 0; k < TY + 6; ++k) I[k] = texel_u8(a.ref, W, H, u, y0 - 3 + k);
+#pragma unroll
+  for (int k = 0; k <= TY; ++k) IR[k] = wave_shl1z(I[k + 2]);
 
   // vector stores need D and the chunk start to be multiples of 4 (16-byte / 8-byte aligned)
   const bool vec = (D % 4 == 0) && (dper % 4 == 0);
@@ -134,43 +148,50 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
       const float h11 = hm[0], h12 = hm[1], h13 = hm[2], h21 = hm[3], h22 = hm[4],
                   h23 = hm[5], h31 = hm[6], h32 = hm[7], h33 = hm[8];
       const float tx = hm[9] * idep, ty = hm[10] * idep, tz = hm[11] * idep;
-      const uint32_t* q = a.quad[s];
+      const uint64_t* q = a.quad[s];
       // column parts of H (u, r, 1) + t idep
       const float Xu = h11 * fu + h13 + tx;
       const float Yu = h21 * fu + h23 + ty;
       const float Zu = h31 * fu + h33 + tz;
       // garbled warps, offsets from x = h11 u + h12 v + h13 (calc_cost.cu:49-51, 69-71)
-      const float g4x = h11 + tx, g4y = h21 + ty, g4z = h31 + tz;
-      const float g8x = h11 - h12 + tx, g8y = h11 - h22 + ty, g8z = h11 - h32 + tz;
       const float xu = h11 * fu + h13;
-      float A0 = 0.0f, A1 = 0.0f, U0 = 0.0f, U1 = 0.0f;   // |I-S| rows r-2, r-1; up terms
+      const float g4x = xu + (h11 + tx), g4y = xu + (h21 + ty), g4z = xu + (h31 + tz);
+      const float g8x = xu + (h11 - h12 + tx), g8y = xu + (h11 - h22 + ty),
+                  g8z = xu + (h11 - h32 + tz);
+      // window over centre rows r-2, r-1: |I-S| (A, row r-1 only), its left-neighbour copy (B),
+      // up terms (Up)
+      float A1 = 0.0f, B0 = 0.0f, B1 = 0.0f, Up0 = 0.0f, Up1 = 0.0f;
 #pragma unroll
       for (int k = 0; k < TY + 2; ++k) {   // centre row r = y0 - 1 + k
         const float fr = (float)(y0 - 1 + k);
-        const float S = warp_sample(q, W, Wm1, Hm1, Xu + h12 * fr, Yu + h22 * fr,
-                                    Zu + h32 * fr);
+        const float S = warp_sample(q, (uint32_t)W, Wm1, Hm1, fmaf(h12, fr, Xu),
+                                    fmaf(h22, fr, Yu), fmaf(h32, fr, Zu));
         const float A2 = fabsf(I[k + 2] - S);     // |I(u,r) - S(u,r)|
-        const float U2 = fabsf(I[k + 4] - S);     // |I(u,r+2) - S(u,r)|
+        const float Up2 = fabsf(I[k + 4] - S);    // |I(u,r+2) - S(u,r)|
+        const float B2 = wave_shr1z(A2);          // the same at (u-1, r)
         if (k >= 2) {
           const int t = k - 2;                    // output row v = r - 1
+          const float fv = (float)(y0 + t);
           const float Dn = fabsf(I[k] - S);       // |I(u,r-2) - S(u,r)| = term 2 of row v
-          const float x = xu + h12 * (float)(y0 + t);
-          const float G4 = fabsf(wave_shl1(I[t + 3], 0.0f) -
-                                 warp_sample(q, W, Wm1, Hm1, x + g4x, x + g4y, x + g4z));
-          const float G8 = fabsf(wave_shl1(I[t + 2], 0.0f) -
-                                 warp_sample(q, W, Wm1, Hm1, x + g8x, x + g8y, x + g8z));
-          float T = A1 + wave_shr1(A0, INF);      // terms 0, 5
-          T += wave_shr1(A1, INF);                // term 3
-          T += wave_shl1(A2, INF);                // term 6
-          T += wave_shr1(A2, INF);                // term 7
-          T += U0 + Dn;                           // terms 1, 2
+          const float G4 = fabsf(IR[t + 1] - warp_sample(q, (uint32_t)W, Wm1, Hm1,
+                                                         fmaf(h12, fv, g4x), fmaf(h12, fv, g4y),
+                                                         fmaf(h12, fv, g4z)));
+          const float G8 = fabsf(IR[t] - warp_sample(q, (uint32_t)W, Wm1, Hm1,
+                                                     fmaf(h12, fv, g8x), fmaf(h12, fv, g8y),
+                                                     fmaf(h12, fv, g8z)));
+          float T = A1 + B0;                      // terms 0, 5
+          T += B1;                                // term 3
+          T += wave_shl1z(A2);                    // term 6
+          T += B2;                                // term 7
+          T += Up0 + Dn;                          // terms 1, 2
           T += G4 + G8;                           // terms 4, 8
           const bool ok = T < INF;
           acc[t] += ok ? T : 0.0f;
           cnt[t] += ok ? 1.0f : 0.0f;
         }
-        A0 = A1; A1 = A2;
-        U0 = U1; U1 = U2;
+        A1 = A2;
+        B0 = B1; B1 = B2;
+        Up0 = Up1; Up1 = Up2;
       }
     }
 #pragma unroll
@@ -274,7 +295,7 @@ bool launch_cost_batch(pm_cost_type ct, int tile_rows, int stage_planes, const K
 
 using namespace pm;
 
-extern "C" pm_status pm_make_quad(const uint8_t* image, uint32_t* quad, int32_t width,
+extern "C" pm_status pm_make_quad(const uint8_t* image, uint64_t* quad, int32_t width,
                                   int32_t height, pm_stream stream) {
   clear_error();
   if (!image || !quad) return fail(PM_ERR_INVALID_ARGUMENT, "pm_make_quad: null pointer");
@@ -286,12 +307,13 @@ extern "C" pm_status pm_make_quad(const uint8_t* image, uint32_t* quad, int32_t 
 }
 
 extern "C" pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t num_sources,
-                                  const uint32_t* const* src_quads, const pm_homography* homs,
+                                  const uint64_t* const* src_quads, const pm_homography* homs,
                                   float idep_min, float dep_sample, void* cost,
                                   pm_cost_type cost_type, pm_stream stream) {
   clear_error();
   if (!ref || !cost) return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: null pointer");
-  if (shape.width <= 0 || shape.height <= 0 || shape.width > 65535 || shape.height > 65535)
+  if (shape.width <= 0 || shape.height <= 0 || shape.width > 65535 || shape.height > 65535 ||
+      (long)shape.width * shape.height > (1L << 29))
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: bad image size");
   if (shape.planes <= 0 || shape.planes > PM_MAX_PLANES)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: planes must be in 1..256");

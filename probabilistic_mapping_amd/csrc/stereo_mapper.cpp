@@ -190,14 +190,14 @@ void StereoMapper::update(const Mat& _img_r, const Mat& R_l, const Mat& T_l, con
   if (!homography_fp64(K, rl, tl, rr, tr, Hm, t))
     throw Error(PM_ERR_INVALID_ARGUMENT, "update: non-finite pose or singular K");
   const size_t slot = homs_.size();
-  if (quads_.size() <= slot) quads_.push_back(dalloc(H_, W_, CV_32S));
+  if (quads_.size() <= slot) quads_.push_back(dalloc(H_, 2 * W_, CV_32S));   // 8 B texels
   hipStream_t st = (hipStream_t)stream_;
   // the u8 frame goes through the filtered-depth buffer's memory, which is free until output
   DeviceMat staging = wta_tmp_;
   staging.type = CV_8U;
   staging.step = (size_t)W_;
   upload_image(_img_r, staging, st, "update");
-  check(pm_make_quad(staging.data, reinterpret_cast<uint32_t*>(quads_[slot].data), W_, H_,
+  check(pm_make_quad(staging.data, reinterpret_cast<uint64_t*>(quads_[slot].data), W_, H_,
                      stream_));
   pm_homography h;
   for (int k = 0; k < 9; ++k) h.h[k] = (float)Hm[k];
@@ -216,8 +216,8 @@ void StereoMapper::ensure_cost() {
   if (cost_ready_) return;
   if (!have_reference_)
     throw Error(PM_ERR_INVALID_STATE, "output: initReference has not been called");
-  std::vector<const uint32_t*> q(homs_.size());
-  for (size_t s = 0; s < homs_.size(); ++s) q[s] = reinterpret_cast<const uint32_t*>(quads_[s].data);
+  std::vector<const uint64_t*> q(homs_.size());
+  for (size_t s = 0; s < homs_.size(); ++s) q[s] = reinterpret_cast<const uint64_t*>(quads_[s].data);
   check(pm_calc_cost(pm_shape{W_, H_, D_}, img_l.data, (int32_t)homs_.size(), q.data(),
                      homs_.data(), IDEP_MIN, DEP_SAMPLE, raw_cost.data, raw_cost_type_,
                      stream_));

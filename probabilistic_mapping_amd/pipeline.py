@@ -53,11 +53,11 @@ def _hom_array(homs):
 
 
 def make_quad(img: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-    """uint8 [H][W] image -> packed bilinear quad texture (int32 [H][W])."""
+    """uint8 [H][W] image -> bilinear quad texture (int64 [H][W], four FP16 per texel)."""
     assert img.dtype == torch.uint8 and img.dim() == 2
     H, W = img.shape
     if out is None:
-        out = torch.empty((H, W), dtype=torch.int32, device=img.device)
+        out = torch.empty((H, W), dtype=torch.int64, device=img.device)
     C.check(C.load().pm_make_quad(_ptr(img), _ptr(out), W, H, _stream()))
     return out
 
