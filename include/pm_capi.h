@@ -253,6 +253,54 @@ typedef struct pm_knobs {
 void pm_get_knobs(pm_knobs* out);
 void pm_set_knobs(const pm_knobs* in);
 
+/* ---- differentiable form: soft-argmin depth readout and its adjoint ------------------------
+ * No counterpart in the reference.  Forward: the reference's cost (calc_cost.cu:11-210) on float
+ * reference intensities, the multi-path SGM (sgm.cu:5-113), then
+ *   depth_hat(p) = sum_d softmax_d(-agg(p,d) / (cost_scale * temperature)) / idep_d
+ *   loss = mean over M of |depth_hat - gt|,  M = pixels with a valid cost and gt > 0.
+ * Backward: dL/dI_ref (float [H][W]) and dL/d(H, t) per source (double [S][12]: h11..h33,
+ * t1..t3 of pm_homography), from a hand-written adjoint; pm_se3_pose_grad maps the latter to
+ * the se3 pose.  The handle owns its device buffers (cost, source counts, aggregate, and
+ * either all per-path L volumes or, with recompute = 1, one). */
+typedef struct pm_diff_params {
+  float temperature;   /* softmax temperature T */
+  float cost_scale;    /* aggregate cost units per logit unit at T = 1 (255: full-scale intensity) */
+  int32_t recompute;   /* 1: the backward recomputes the aggregate and the per-path L volumes */
+} pm_diff_params;
+
+typedef struct pm_diff pm_diff;
+
+pm_status pm_diff_create(pm_shape shape, int32_t num_sources, pm_sgm_params sgm,
+                         pm_diff_params params, pm_diff** out);
+void pm_diff_destroy(pm_diff* h);
+/* Device bytes held by the handle. */
+pm_status pm_diff_memory(const pm_diff* h, uint64_t* bytes);
+/* ref: device float [H][W]; src_quads: host array of num_sources device quad textures;
+ * homs: host array (FP32 H, t per source); gt: device float [H][W] (<= 0 or non-finite =
+ * unknown); depth: device float [H][W] (depth_hat); loss: device double.  ref, the quads, gt
+ * and depth must stay valid until pm_diff_backward has run. */
+pm_status pm_diff_forward(pm_diff* h, const float* ref, const uint64_t* const* src_quads,
+                          const pm_homography* homs, float idep_min, float dep_sample,
+                          const float* gt, float* depth, double* loss, pm_stream stream);
+/* grad_ref: device float [H][W]; grad_homs: device double [S][12]; both overwritten. */
+pm_status pm_diff_backward(pm_diff* h, float grad_loss, float* grad_ref, double* grad_homs,
+                           pm_stream stream);
+
+/* Copies a handle buffer to host memory (synchronous): "cost" (float [H][W][D]), "count"
+ * (uint8 sources per (pixel, plane)), "agg" (float: the aggregate after the forward, dL/dagg
+ * after the backward), "grad_cost" (float dL/dC after the backward), "valid" (uint8 [H][W]).
+ * bytes must equal the buffer size. */
+pm_status pm_diff_download(const pm_diff* h, const char* name, void* host, size_t bytes);
+
+/* se3 pose of a measurement frame (host, FP64).  xi = (rho, phi), translation first;
+ * (R, t) = Exp(xi) (R0, t0) takes reference-camera coordinates to the measurement camera.
+ * pm_se3_homography: H = K R K^-1, t_h = K t (the relative form of stereo_mapper.cpp:79-80).
+ * pm_se3_pose_grad: dL/dxi from dL/d(H, t_h) (grad_hom[12] as in pm_diff_backward). */
+pm_status pm_se3_homography(const double K[9], const double R0[9], const double t0[3],
+                            const double xi[6], double H[9], double t[3]);
+pm_status pm_se3_pose_grad(const double K[9], const double R0[9], const double t0[3],
+                           const double xi[6], const double grad_hom[12], double grad_xi[6]);
+
 #ifdef __cplusplus
 }
 #endif

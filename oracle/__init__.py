@@ -118,6 +118,26 @@ def sgm(ref, cost, knobs, paths, precision="f64"):
     return agg
 
 
+def sgm_adjoint(ref, cost, knobs, paths, gagg, precision="f64"):
+    """dL/dC of the multi-path aggregation given dL/dagg (both ``[H][W][D]``, float64 out).
+    ``precision`` sets the arithmetic of the recomputed forward and of its routing decisions
+    ("f32" reproduces an FP32 forward); gradients accumulate in double."""
+    ref = np.ascontiguousarray(ref, dtype=np.uint8)
+    H, W = ref.shape
+    dt, ct = _dtype(precision)
+    cost = np.ascontiguousarray(cost, dtype=dt)
+    D = cost.shape[2]
+    g = np.ascontiguousarray(gagg, dtype=np.float64)
+    assert g.shape == cost.shape
+    out = np.empty(cost.shape, np.float64)
+    kn = _knob_array(dict(knobs, var_scale=knobs.get("var_scale", 1.0)))
+    fn = lib().oracle_sgm_adjoint_f64 if precision == "f64" else lib().oracle_sgm_adjoint_f32
+    fn(ctypes.c_int(W), ctypes.c_int(H), ctypes.c_int(D), _ptr(ref, ctypes.c_uint8),
+       _ptr(cost, ct), _ptr(kn, ctypes.c_double), ctypes.c_int(paths),
+       _ptr(g, ctypes.c_double), _ptr(out, ctypes.c_double))
+    return out
+
+
 def filter_cost(agg, var_scale, idep_min, dep_sample, precision="f64"):
     """WTA + rejection rules + parabola subpixel fit -> depth map ``[H][W]``."""
     dt, ct = _dtype(precision)

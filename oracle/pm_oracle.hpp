@@ -16,7 +16,7 @@
 //   cost / agg    : pixel-major [H][W][D] (D contiguous)
 //   depth, state  : row-major [H][W] float
 //
-// Deliberate deviations from the CUDA reference (all documented in DESIGN.md):
+// Deliberate deviations from the CUDA reference:
 //   * bilinear sampling is exact (the reference's tex2D uses hardware fixed-point weights);
 //   * a non-finite warp (0/0) counts as invalid (the reference's comparisons let NaN through);
 //   * min reductions are exact for every D (the reference's tree skips elements for
@@ -226,6 +226,97 @@ void sgm(int W, int H, int D, const uint8_t* ref, const T* cost, const Knobs& kn
           agg[p * D + d] = invalid[p] ? T(kCostInf) : agg[p * D + d] + cur[d];
         }
         std::swap(prev, cur);
+      }
+    }
+  }
+}
+
+// Adjoint of sgm<T> (no reference counterpart: the checker of the differentiable readout's
+// hand-written backward).  Given dL/dagg, returns dL/dC.  Each line's L values are recomputed
+// in T exactly as sgm<T> computes them, so T = float reproduces the decisions of an FP32
+// forward; gradients accumulate in double.  Walking each line backwards,
+//   dL/dL(p,d) = [p valid] dL/dagg(p,d) + (routing from the successor),  dL/dC(p,d) += it
+// where L(q,d) = C'(q,d) + min(L(p,d), L(p,d-1)+P1, L(p,d+1)+P1, m+P2) - m sends its gradient
+// to the argument that won (ties: L(p,d), then L(p,d-1), then L(p,d+1), then m), and the
+// gradients of m (from the -m term and the m+P2 branch) go to argmin_d L(p,d), lowest index.
+template <typename T>
+void sgm_adjoint(int W, int H, int D, const uint8_t* ref, const T* cost, const Knobs& kn,
+                 const double* gagg, double* gcost) {
+  const long N = (long)W * H;
+  std::vector<uint8_t> invalid(N);
+  for (long p = 0; p < N; ++p) {
+    T m = cost[p * D];
+    for (int d = 1; d < D; ++d) m = std::min(m, cost[p * D + d]);
+    invalid[p] = m < T(0);
+  }
+  std::fill(gcost, gcost + N * D, 0.0);
+  const T inf = std::numeric_limits<T>::infinity();
+  for (int path = 0; path < kn.paths; ++path) {
+    const int dx = kPathDx[path], dy = kPathDy[path];
+    const long nlines = scanline_count(W, H, dx, dy);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long li = 0; li < nlines; ++li) {
+      int x, y;
+      scanline_start(W, H, dx, dy, li, &x, &y);
+      std::vector<long> pix;
+      for (int k = 0;; ++k) {
+        const int px = x + k * dx, py = y + k * dy;
+        if (px < 0 || py < 0 || px >= W || py >= H) break;
+        pix.push_back((long)py * W + px);
+      }
+      const int n = (int)pix.size();
+      std::vector<T> L((size_t)n * D), P1v(n), P2v(n);
+      for (int k = 0; k < n; ++k) {
+        const long p = pix[k];
+        T* cur = &L[(size_t)k * D];
+        if (k == 0) {
+          for (int d = 0; d < D; ++d) cur[d] = invalid[p] ? T(kCostInf) : cost[p * D + d];
+          continue;
+        }
+        const T* prev = &L[(size_t)(k - 1) * D];
+        T m = inf;
+        for (int d = 0; d < D; ++d) m = std::min(m, prev[d]);
+        T P1 = T(kn.pi1), P2 = T(kn.pi2);
+        if (std::fabs(T(ref[p]) - T(ref[pix[k - 1]])) < T(kn.tau_so)) {
+          P1 *= T(kn.sgm_q1);
+          P2 *= T(kn.sgm_q2);
+        }
+        P1v[k] = P1;
+        P2v[k] = P2;
+        for (int d = 0; d < D; ++d) {
+          T val = std::min(prev[d], m + P2);
+          if (d > 0) val = std::min(val, prev[d - 1] + P1);
+          if (d + 1 < D) val = std::min(val, prev[d + 1] + P1);
+          const T c = invalid[p] ? T(kCostInf) : cost[p * D + d];
+          cur[d] = c + val - m;
+        }
+      }
+      std::vector<double> carry(D, 0.0), gL(D), next(D);
+      for (int k = n - 1; k >= 0; --k) {
+        const long p = pix[k];
+        for (int d = 0; d < D; ++d) {
+          gL[d] = carry[d] + (invalid[p] ? 0.0 : gagg[p * D + d]);
+          if (!invalid[p]) gcost[p * D + d] += gL[d];
+        }
+        if (k == 0) break;
+        const T* Lp = &L[(size_t)(k - 1) * D];
+        T m = inf;
+        int dstar = 0;
+        for (int d = 0; d < D; ++d)
+          if (Lp[d] < m) { m = Lp[d]; dstar = d; }
+        const T c3 = m + P2v[k];
+        std::fill(next.begin(), next.end(), 0.0);
+        double gm = 0.0;
+        for (int d = 0; d < D; ++d) {
+          const T left = d > 0 ? Lp[d - 1] : inf, right = d + 1 < D ? Lp[d + 1] : inf;
+          const T nb = std::min(left, right) + P1v[k];
+          if (Lp[d] <= nb && Lp[d] <= c3) next[d] += gL[d];
+          else if (!(c3 < nb)) next[left <= right ? d - 1 : d + 1] += gL[d];
+          else gm += gL[d];
+          gm -= gL[d];
+        }
+        next[dstar] += gm;
+        carry.swap(next);
       }
     }
   }

@@ -70,6 +70,16 @@ void oracle_sgm_f32(int W, int H, int D, const uint8_t* ref, const float* cost,
   pmo::sgm<float>(W, H, D, ref, cost, knobs_from(knobs, paths), agg);
 }
 
+// dL/dC from dL/dagg; decisions in the cost's precision, gradients in double
+void oracle_sgm_adjoint_f64(int W, int H, int D, const uint8_t* ref, const double* cost,
+                            const double* knobs, int paths, const double* gagg, double* gcost) {
+  pmo::sgm_adjoint<double>(W, H, D, ref, cost, knobs_from(knobs, paths), gagg, gcost);
+}
+void oracle_sgm_adjoint_f32(int W, int H, int D, const uint8_t* ref, const float* cost,
+                            const double* knobs, int paths, const double* gagg, double* gcost) {
+  pmo::sgm_adjoint<float>(W, H, D, ref, cost, knobs_from(knobs, paths), gagg, gcost);
+}
+
 void oracle_filter_f64(int W, int H, int D, const double* agg, double var_scale,
                        double idep_min, double dep_sample, double* depth) {
   pmo::filter_cost<double>(W, H, D, agg, var_scale, idep_min, dep_sample, depth);

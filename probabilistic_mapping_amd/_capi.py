@@ -84,6 +84,11 @@ class Knobs(ctypes.Structure):
                 ("median", ctypes.c_int32)]
 
 
+class DiffParams(ctypes.Structure):
+    _fields_ = [("temperature", ctypes.c_float), ("cost_scale", ctypes.c_float),
+                ("recompute", ctypes.c_int32)]
+
+
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -130,6 +135,15 @@ SIGNATURES = {
     "pm_mapper_set_cost_type": (_i32, [_vp, _i32]),
     "pm_get_knobs": (None, [ctypes.POINTER(Knobs)]),
     "pm_set_knobs": (None, [ctypes.POINTER(Knobs)]),
+    "pm_diff_create": (_i32, [Shape, _i32, SgmParams, DiffParams, ctypes.POINTER(_vp)]),
+    "pm_diff_destroy": (None, [_vp]),
+    "pm_diff_memory": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "pm_diff_forward": (_i32, [_vp, _vp, ctypes.POINTER(_vp), ctypes.POINTER(Homography),
+                               ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, _vp]),
+    "pm_diff_backward": (_i32, [_vp, ctypes.c_float, _vp, _vp, _vp]),
+    "pm_diff_download": (_i32, [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]),
+    "pm_se3_homography": (_i32, [_dp, _dp, _dp, _dp, _dp, _dp]),
+    "pm_se3_pose_grad": (_i32, [_dp, _dp, _dp, _dp, _dp, _dp]),
 }
 
 _lib = None

@@ -19,9 +19,9 @@ HIP_LIB = os.path.join(HERE, "libpm_hip.so")
 SYNTH_LIB = os.path.join(HERE, "libpm_synth.so")
 
 ARCH = "gfx950"
-HIP_SOURCES = ["cost.hip", "sgm.hip", "filter.hip", "fusion.hip", "batch.hip", "capi.hip",
+HIP_SOURCES = ["cost.hip", "sgm.hip", "filter.hip", "fusion.hip", "batch.hip", "diff.hip", "capi.hip", "se3.cpp",
                "stereo_mapper.cpp", "parameters.cpp"]
-HIP_HEADERS = ["common.hpp"]
+HIP_HEADERS = ["common.hpp", "cost_tile.hpp", "sgm_scan.hpp"]
 
 # Packed-FP32 VALU forms are disabled for every device compile.
 HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -28,7 +28,7 @@ constexpr int kPathDy[8] = {0, 0, 1, -1, 1, -1, -1, 1};
 struct CostArgs {
   int W, H, D, S;
   float idep_min, dep_sample;
-  const uint8_t* ref;
+  const void* ref;                        // uint8 [H][W], or float [H][W] (differentiable form)
   const uint64_t* quad[PM_MAX_SOURCES];   // difference texels, see pm_make_quad
   float hm[PM_MAX_SOURCES][12];   // h11..h33, t1..t3
 };
@@ -37,10 +37,12 @@ struct CostArgs {
 struct SgmArgs {
   int W, H, D, dx, dy;
   int nlines;
-  const uint8_t* ref;
+  const void* ref;  // uint8 [H][W], or float [H][W] (differentiable form)
   float P1n, P2n;   // penalties where |I(p) - I(q)| >= tau_so
   float P1s, P2s;   // penalties scaled by q1, q2 where |I(p) - I(q)| < tau_so
   int tau_i;        // integer form of the test: |dI| < tau_so  <=>  |dI| < ceil(tau_so)
+  float tau_f;      // tau_so itself, for float reference intensities
+  float* lout;      // per-path L volume [H][W][D] (differentiable form), else unused
 };
 
 // One keyframe of a batch (device table entry).
@@ -170,7 +172,9 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
   return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
-__device__ __forceinline__ float texel_u8(const uint8_t* img, int W, int H, int x, int y) {
+// Reference intensity at an integer position, 0 outside the image (border addressing).
+template <typename RT>
+__device__ __forceinline__ float texel(const RT* img, int W, int H, int x, int y) {
   return (x < 0 || y < 0 || x >= W || y >= H) ? 0.0f : (float)img[(long)y * W + x];
 }
 

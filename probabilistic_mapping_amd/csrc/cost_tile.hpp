@@ -1,0 +1,242 @@
+// Plane-sweep cost tile body shared by the cost kernels (plain, batched, differentiable) and
+// the cost adjoint.  Reference: ADCalcCostKernel, motion_stereo/calc_cost.cu:11-210.
+#pragma once
+
+#include "common.hpp"
+
+namespace pm {
+
+// Bilinear sample of the quad texture at (U, V) with the warp's validity (calc_cost.cu:85-93);
+// returns +inf for an invalid warp.  Validity is "the clamp to the image changes nothing" and
+// w >= 0, so a NaN warp (0/0) is invalid; an invalid warp still reads an in-bounds texel (the
+// clamped position), so every gather of a row can be in flight at once.  The byte offset fits
+// 32 bits (W H <= 2^29 is checked on the host), which lets the gather use the scalar base
+// address + 32-bit vector offset form.
+__device__ __forceinline__ float warp_sample(const uint64_t* __restrict__ quad, uint32_t W,
+                                             float Wm1, float Hm1, float X, float Y, float w) {
+  const float rw = __builtin_amdgcn_rcpf(w);
+  const float U = X * rw;
+  const float V = Y * rw;
+  const float Uc = __builtin_amdgcn_fmed3f(U, 0.0f, Wm1);
+  const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
+  const bool valid = (w >= 0.0f) & (Uc == U) & (Vc == V);
+  // non-negative, so truncation is floor; W, y <= 65535 fit the 24-bit multiply
+  const uint32_t off = __umul24((uint32_t)Vc, W) * 8u + (uint32_t)Uc * 8u;
+  const uint2 t = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(quad) + off);
+  const float ax = __builtin_amdgcn_fractf(Uc), ay = __builtin_amdgcn_fractf(Vc);
+  const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
+  const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
+  const float top = fmaf(ax, __high2float(lo), __low2float(lo));
+  const float dv = fmaf(ax, __high2float(hi), __low2float(hi));
+  return valid ? fmaf(ay, dv, top) : __uint_as_float(0x7F800000u);
+}
+
+// warp_sample with the sample's derivatives with respect to the warped point (X, Y, w):
+// S = B(X/w, Y/w), dS/dX = a, dS/dY = b, dS/dw = c.  The value is computed exactly as
+// warp_sample computes it.  Inside a texel cell the bilinear slopes are
+// dB/dU = bx + ay bxy and dB/dV = by + ax bxy (the cell of floor(U), floor(V)); an invalid warp
+// returns +inf and zero derivatives.
+struct SampleGrad {
+  float s, a, b, c;
+};
+__device__ __forceinline__ SampleGrad warp_sample_grad(const uint64_t* __restrict__ quad,
+                                                       uint32_t W, float Wm1, float Hm1,
+                                                       float X, float Y, float w) {
+  const float rw = __builtin_amdgcn_rcpf(w);
+  const float U = X * rw;
+  const float V = Y * rw;
+  const float Uc = __builtin_amdgcn_fmed3f(U, 0.0f, Wm1);
+  const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
+  const bool valid = (w >= 0.0f) & (Uc == U) & (Vc == V);
+  const uint32_t off = __umul24((uint32_t)Vc, W) * 8u + (uint32_t)Uc * 8u;
+  const uint2 t = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(quad) + off);
+  const float ax = __builtin_amdgcn_fractf(Uc), ay = __builtin_amdgcn_fractf(Vc);
+  const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
+  const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
+  const float top = fmaf(ax, __high2float(lo), __low2float(lo));
+  const float dv = fmaf(ax, __high2float(hi), __low2float(hi));
+  const float dU = fmaf(ay, __high2float(hi), __high2float(lo));
+  SampleGrad g;
+  g.s = valid ? fmaf(ay, dv, top) : __uint_as_float(0x7F800000u);
+  const float a = dU * rw, b = dv * rw;
+  g.a = valid ? a : 0.0f;
+  g.b = valid ? b : 0.0f;
+  g.c = valid ? -fmaf(a, Uc, b * Vc) : 0.0f;
+  return g;
+}
+
+// Tile kernel.  A wave covers a strip of 64 columns u = x0-1 .. x0+62 (62 output columns and
+// a one-column halo on each side) and TY output rows, for a quarter of the planes.  For each
+// (plane, source) it walks the rows y0-1 .. y0+TY once, warping every pixel's own centre
+// H_d (u, r, 1) a single time.  The reference's nine warps of pixel (u, v) are the centre
+// warps of its 3x3 neighbours, except two garbled ones (calc_cost.cu:50-51, 70-71):
+//   term 0 (u,v), 3 (u-1,v), 5 (u-1,v-1), 6 (u+1,v+1), 7 (u-1,v+1): |I(q) - S(q)| at q
+//   term 1: warp (u,v-1) against I(u,v+1);  term 2: warp (u,v+1) against I(u,v-1)
+//   term 4: (x+h11, x+h21, x+h31) against I(u+1,v)
+//   term 8: (x+h11-h12, x+h11-h22, x+h11-h32) against I(u+1,v-1),  x = h11 u + h12 v + h13
+// so each centre sample yields |I(q)-S|, |I(q+(0,2))-S| and |I(q-(0,2))-S|, kept in a 3-row
+// register window; column neighbours come from DPP lane shifts.  An invalid warp is +inf, so
+// a source contributes iff the 9-term sum is finite (calc_cost.cu:89-91).
+// Output staging: a wave finishes DC planes for its TY x 62 pixels, parks them in its own LDS
+// tile [TY][DC][64] (lanes contiguous, conflict-free), then writes each pixel's DC contiguous
+// planes with consecutive lanes, so every store instruction writes whole aligned segments
+// instead of 64 isolated 4-byte words 4*D bytes apart.
+template <typename CT>
+__device__ __forceinline__ void store_quad(CT* dst, float a, float b, float c, float d);
+template <>
+__device__ __forceinline__ void store_quad<float>(float* dst, float a, float b, float c,
+                                                  float d) {
+  *reinterpret_cast<float4*>(dst) = make_float4(a, b, c, d);
+}
+template <>
+__device__ __forceinline__ void store_quad<__half>(__half* dst, float a, float b, float c,
+                                                   float d) {
+  const __half2 lo = __floats2half2_rn(a, b), hi = __floats2half2_rn(c, d);
+  uint2 u;
+  u.x = *reinterpret_cast<const uint32_t*>(&lo);
+  u.y = *reinterpret_cast<const uint32_t*>(&hi);
+  *reinterpret_cast<uint2*>(dst) = u;
+}
+template <>
+__device__ __forceinline__ void store_quad<__hip_bfloat16>(__hip_bfloat16* dst, float a,
+                                                           float b, float c, float d) {
+  dst[0] = __float2bfloat16(a); dst[1] = __float2bfloat16(b);
+  dst[2] = __float2bfloat16(c); dst[3] = __float2bfloat16(d);
+}
+
+// RT: reference element type (uint8_t, or float for the differentiable form).  CNT: also write
+// the number of sources that saw each (pixel, plane) to cnt (uint8 [H][W][D]), staged in ctile.
+template <typename CT, int TY, int DC, typename RT = uint8_t, bool CNT = false>
+__device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __restrict__ cost,
+                                          float* __restrict__ tile,
+                                          uint8_t* __restrict__ cnt_out = nullptr,
+                                          uint8_t* __restrict__ ctile = nullptr) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nstrips = (a.W + 61) / 62;
+  const int strip = blk % nstrips;
+  const int y0 = (blk / nstrips) * TY;
+  const int W = a.W, H = a.H, D = a.D;
+  const int u = strip * 62 - 1 + lane;
+  const int dper = (D + 3) / 4;
+  const int dbeg = wave * dper, dend = min(D, dbeg + dper);
+  const float Wm1 = (float)(W - 1), Hm1 = (float)(H - 1);
+  const float INF = __uint_as_float(0x7F800000u);
+  const float fu = (float)u;
+  // reference intensities of this column for rows y0-3 .. y0+TY+2 (0 outside the image), and
+  // of the right-hand neighbour column for the garbled terms' rows y0-1 .. y0+TY-1
+  float I[TY + 6], IR[TY + 1];
+#pragma unroll
+  for (int k = 0; k < TY + 6; ++k)
+    I[k] = texel(static_cast<const RT*>(a.ref), W, H, u, y0 - 3 + k);
+#pragma unroll
+  for (int k = 0; k <= TY; ++k) IR[k] = wave_shl1z(I[k + 2]);
+
+  // vector stores need D and the chunk start to be multiples of 4 (16-byte / 8-byte aligned)
+  const bool vec = (D % 4 == 0) && (dper % 4 == 0);
+  for (int dc0 = dbeg; dc0 < dend; dc0 += DC) {
+  const int dcn = min(DC, dend - dc0);
+  for (int d = dc0; d < dc0 + dcn; ++d) {
+    const float idep = a.idep_min + (float)d * a.dep_sample;
+    float acc[TY], cnt[TY];
+#pragma unroll
+    for (int t = 0; t < TY; ++t) { acc[t] = 0.0f; cnt[t] = 0.0f; }
+    for (int s = 0; s < a.S; ++s) {
+      const float* hm = a.hm[s];
+      const float h11 = hm[0], h12 = hm[1], h13 = hm[2], h21 = hm[3], h22 = hm[4],
+                  h23 = hm[5], h31 = hm[6], h32 = hm[7], h33 = hm[8];
+      const float tx = hm[9] * idep, ty = hm[10] * idep, tz = hm[11] * idep;
+      const uint64_t* q = a.quad[s];
+      // column parts of H (u, r, 1) + t idep
+      const float Xu = h11 * fu + h13 + tx;
+      const float Yu = h21 * fu + h23 + ty;
+      const float Zu = h31 * fu + h33 + tz;
+      // garbled warps, offsets from x = h11 u + h12 v + h13 (calc_cost.cu:49-51, 69-71)
+      const float xu = h11 * fu + h13;
+      const float g4x = xu + (h11 + tx), g4y = xu + (h21 + ty), g4z = xu + (h31 + tz);
+      const float g8x = xu + (h11 - h12 + tx), g8y = xu + (h11 - h22 + ty),
+                  g8z = xu + (h11 - h32 + tz);
+      // window over centre rows r-2, r-1: |I-S| (A, row r-1 only), its left-neighbour copy (B),
+      // up terms (Up)
+      float A1 = 0.0f, B0 = 0.0f, B1 = 0.0f, Up0 = 0.0f, Up1 = 0.0f;
+#pragma unroll
+      for (int k = 0; k < TY + 2; ++k) {   // centre row r = y0 - 1 + k
+        const float fr = (float)(y0 - 1 + k);
+        const float S = warp_sample(q, (uint32_t)W, Wm1, Hm1, fmaf(h12, fr, Xu),
+                                    fmaf(h22, fr, Yu), fmaf(h32, fr, Zu));
+        const float A2 = fabsf(I[k + 2] - S);     // |I(u,r) - S(u,r)|
+        const float Up2 = fabsf(I[k + 4] - S);    // |I(u,r+2) - S(u,r)|
+        const float B2 = wave_shr1z(A2);          // the same at (u-1, r)
+        if (k >= 2) {
+          const int t = k - 2;                    // output row v = r - 1
+          const float fv = (float)(y0 + t);
+          const float Dn = fabsf(I[k] - S);       // |I(u,r-2) - S(u,r)| = term 2 of row v
+          const float G4 = fabsf(IR[t + 1] - warp_sample(q, (uint32_t)W, Wm1, Hm1,
+                                                         fmaf(h12, fv, g4x), fmaf(h12, fv, g4y),
+                                                         fmaf(h12, fv, g4z)));
+          const float G8 = fabsf(IR[t] - warp_sample(q, (uint32_t)W, Wm1, Hm1,
+                                                     fmaf(h12, fv, g8x), fmaf(h12, fv, g8y),
+                                                     fmaf(h12, fv, g8z)));
+          float T = A1 + B0;                      // terms 0, 5
+          T += B1;                                // term 3
+          T += wave_shl1z(A2);                    // term 6
+          T += B2;                                // term 7
+          T += Up0 + Dn;                          // terms 1, 2
+          T += G4 + G8;                           // terms 4, 8
+          const bool ok = T < INF;
+          acc[t] += ok ? T : 0.0f;
+          cnt[t] += ok ? 1.0f : 0.0f;
+        }
+        A1 = A2;
+        B0 = B1; B1 = B2;
+        Up0 = Up1; Up1 = Up2;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TY; ++t) {   // mean over sources of (sum of 9 terms)/9, or -1
+      tile[(t * DC + (d - dc0)) * 64 + lane] = cnt[t] > 0.0f ? acc[t] / (9.0f * cnt[t]) : -1.0f;
+      if constexpr (CNT) ctile[(t * DC + (d - dc0)) * 64 + lane] = (uint8_t)cnt[t];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int x0 = strip * 62;
+  if (vec) {
+    // items: (row t, pixel 0..61, quad of 4 planes); consecutive lanes cover one pixel's planes
+    const int qp = dcn / 4, items = TY * 62 * qp;
+    for (int it = lane; it < items; it += 64) {
+      const int q = it % qp, rest = it / qp, px = rest % 62, t = rest / 62;
+      const int v = y0 + t, x = x0 + px;
+      if (v < H && x < W) {
+        const float* src = tile + (t * DC + 4 * q) * 64 + px + 1;
+        store_quad<CT>(cost + ((long)v * W + x) * D + dc0 + 4 * q, src[0], src[64], src[128],
+                       src[192]);
+      }
+    }
+  } else {
+    const int items = TY * 62 * dcn;
+    for (int it = lane; it < items; it += 64) {
+      const int j = it % dcn, rest = it / dcn, px = rest % 62, t = rest / 62;
+      const int v = y0 + t, x = x0 + px;
+      if (v < H && x < W)
+        cost[((long)v * W + x) * D + dc0 + j] = store_cost<CT>(tile[(t * DC + j) * 64 + px + 1]);
+    }
+  }
+  if constexpr (CNT) {
+    const int items = TY * 62 * dcn;
+    for (int it = lane; it < items; it += 64) {
+      const int j = it % dcn, rest = it / dcn, px = rest % 62, t = rest / 62;
+      const int v = y0 + t, x = x0 + px;
+      if (v < H && x < W) cnt_out[((long)v * W + x) * D + dc0 + j] = ctile[(t * DC + j) * 64 + px + 1];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Tiles of a W x H keyframe: 62-column strips x TY-row blocks.
+template <int TY>
+__host__ __device__ inline int cost_tiles(int W, int H) {
+  return ((W + 61) / 62) * ((H + TY - 1) / TY);
+}
+
+}  // namespace pm

@@ -213,3 +213,48 @@ def trajectory_scene(seed: int = 0, width: int = 752, height: int = 480, n_frame
         "planes": planes, "idep_min": 0.125, "idep_max": 2.0, "paths": 8,
         "knobs": dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02),
     }
+
+
+def diff_scene(seed: int = 0, width: int = 640, height: int = 480, n_sources: int = 4,
+               planes: int = 64):
+    """Reference frame + ``n_sources`` measurement frames of 4 textured planes at 1-5 m, with
+    ground-truth depth, for the differentiable readout (focal length 250 px per 320 px of width).
+
+    Poses are also given relative to the reference camera: (R, t) with
+    X_src = R X_ref + t, i.e. R = R_r^T R_l, t = R_r^T (T_l - T_r) (stereo_mapper.cpp:79-80).
+    """
+    rng = np.random.default_rng(1000 + seed)
+    f = 250.0 * width / 320.0
+    K = np.array([[f, 0, width / 2.0], [0, f, height / 2.0], [0, 0, 1.0]])
+    patches = []
+    n = rot_xyz(rng.uniform(-0.1, 0.1), rng.uniform(-0.15, 0.15), 0.0)
+    patches.append(_patch([0, 0, rng.uniform(4.3, 4.8)], n[:, 0], n[:, 1], 0, 0,
+                          seed * 101 + 51, freq=2.5, amp=160))
+    for k in range(3):
+        z = rng.uniform(1.2, 3.8)
+        cx_ = rng.uniform(-0.35, 0.35) * z
+        cy_ = rng.uniform(-0.25, 0.25) * z
+        Rk = rot_xyz(rng.uniform(-0.5, 0.5), rng.uniform(-0.6, 0.6), rng.uniform(-0.3, 0.3))
+        patches.append(_patch([cx_, cy_, z], Rk[:, 0], Rk[:, 1],
+                              rng.uniform(0.15, 0.35) * z, rng.uniform(0.12, 0.3) * z,
+                              seed * 101 + 52 + k, freq=float(rng.uniform(4, 8)),
+                              mean=float(rng.uniform(100, 160)), amp=160))
+    scene = Scene(width, height, K, np.array(patches))
+    poses = [(np.eye(3), np.zeros(3))]
+    for _ in range(n_sources):
+        d = rng.normal(size=3); d[2] *= 0.3; d /= np.linalg.norm(d)
+        T = d * rng.uniform(0.05, 0.10)
+        R = axis_angle(rng.normal(size=3), math.radians(rng.uniform(0.3, 2.0)))
+        poses.append((R, T))
+    scene.poses = poses
+    frames = [scene.render(R, T) for R, T in poses]
+    R_l, T_l = poses[0]
+    rel = [(R_r.T @ R_l, R_r.T @ (T_l - T_r)) for R_r, T_r in poses[1:]]
+    return {
+        "scene": scene, "K": K,
+        "ref": frames[0][0], "gt_depth": frames[0][1],
+        "srcs": [fr[0] for fr in frames[1:]], "rel_poses": rel,
+        "planes": planes, "idep_min": 0.2, "idep_max": 1.0, "paths": 8,
+        "knobs": dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0),
+        "temperature": 0.05, "cost_scale": 255.0,
+    }
