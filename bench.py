@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cost-type", default="f32", choices=["f32", "f16", "bf16"])
-    ap.add_argument("--form", default="plain", choices=["plain", "batched"],
+    ap.add_argument("--form", default="plain", choices=["plain", "batched", "diff"],
                     help="plain: one keyframe at a time; batched: all keyframes' cost, SGM and "
                          "filter in one autotuned, hipGraph-captured batch, then the fusion steps")
     ap.add_argument("--median", type=int, default=0)
@@ -185,6 +185,76 @@ def bench_batched(args, torch, seq, s, copy):
     print(json.dumps(out), flush=True)
 
 
+def bench_diff(args, torch):
+    """Differentiable form: forward (cost, 8-path SGM, soft-argmin readout, L1 loss) + hand-written
+    backward (dL/dI_ref, dL/d(H, t)) of one 640x480 keyframe with 4 sources and 64 planes, in
+    the store and recompute modes."""
+    from probabilistic_mapping_amd import diff, pipeline, synth
+    s = synth.diff_scene(args.seed)
+    H, W = s["ref"].shape
+    D, S = s["planes"], len(s["srcs"])
+    ds = (s["idep_max"] - s["idep_min"]) / (D - 1)
+    ref = torch.from_numpy(s["ref"]).cuda().float()
+    quads = [pipeline.make_quad(torch.from_numpy(im).cuda()) for im in s["srcs"]]
+    homs = []
+    for R, t in s["rel_poses"]:
+        Hm, th = diff.se3_homography(s["K"], R, t, np.zeros(6))
+        homs.append((Hm.astype(np.float32), th.astype(np.float32)))
+    gt = torch.from_numpy(s["gt_depth"].astype(np.float32)).cuda()
+    stream = torch.cuda.current_stream()
+    modes = {}
+    for mode in ("store", "recompute"):
+        op = diff.SoftArgminDepth(W, H, D, S, s["knobs"], paths=8, temperature=s["temperature"],
+                                  cost_scale=s["cost_scale"], recompute=(mode == "recompute"))
+        for _ in range(args.warmup):
+            op.forward(ref, quads, homs, s["idep_min"], ds, gt)
+            op.backward(1.0)
+        torch.cuda.synchronize()
+        evs = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record(stream)
+            op.forward(ref, quads, homs, s["idep_min"], ds, gt)
+            e[1].record(stream)
+            op.backward(1.0)
+            e[2].record(stream)
+            evs.append(e)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        modes[mode] = {"keyframes_per_s": args.steps / elapsed,
+                       "ms_per_step": elapsed / args.steps * 1e3,
+                       "forward_ms": float(np.mean([a.elapsed_time(b) for a, b, _ in evs])),
+                       "backward_ms": float(np.mean([b.elapsed_time(c) for _, b, c in evs])),
+                       "device_bytes": op.memory_bytes}
+        op.close()
+    V = H * W * D
+    # backward algorithmic HBM bytes, store mode: readout adjoint V*4 read + V*4 write; 8 SGM
+    # adjoint sweeps of (L read, dL/dagg read, dL/dC read+write) = 16 B; cost adjoint reads
+    # dL/dC and the counts (5 B)
+    bwd_bytes = V * 8 + 8 * V * 16 + V * 5
+    st = modes["store"]
+    out = {
+        "metric": "differentiable soft-argmin depth: forward + backward keyframes/s "
+                  "@640x480x64 planes, 4 sources",
+        "value": st["keyframes_per_s"],
+        "unit": "keyframes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": st["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded textured-plane scene with ground-truth depth)",
+        "config": {"workload": "640x480 reference, 4 measurement frames, 64 planes "
+                               "(idep 0.2-1.0), 8-path SGM, soft-argmin T=0.05 over agg/255, "
+                               "mean L1 loss, backward to I_ref and (H, t) per source",
+                   "form": "diff"},
+        "modes": modes,
+        "backward_algorithmic_bytes_store": bwd_bytes,
+        "backward_gbs_store": bwd_bytes / (st["backward_ms"] * 1e-3) / 1e9,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -198,6 +268,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.form == "diff":
+        return bench_diff(args, torch)
     s = synth.trajectory_scene(args.seed)
     nkf = len(s["images"])
     Seq = pl.BatchedSequence if args.form == "batched" else KeyframeSequence

@@ -44,11 +44,23 @@ __device__ __forceinline__ float sgnf(float x) {
   return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
 }
 
-// Sum over the 64 lanes in a fixed order; the result is wave-uniform.
+// Sum over the 64 lanes in a fixed order, wave-uniform: DPP butterflies inside each row of 16
+// lanes, then the four row sums (read from lanes 0, 16, 32, 48) added pairwise.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __uint_as_float(dpp_u32<CTRL>(__float_as_uint(v), __float_as_uint(v)));
+}
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+  v += dpp_f32<0xB1>(v);    // quad_perm [1,0,3,2]
+  v += dpp_f32<0x4E>(v);    // quad_perm [2,3,0,1]
+  v += dpp_f32<0x124>(v);   // row_ror:4
+  v += dpp_f32<0x128>(v);   // row_ror:8
+  const uint32_t u = __float_as_uint(v);
+  const float a = __uint_as_float(__builtin_amdgcn_readlane(u, 0));
+  const float b = __uint_as_float(__builtin_amdgcn_readlane(u, 16));
+  const float c = __uint_as_float(__builtin_amdgcn_readlane(u, 32));
+  const float d = __uint_as_float(__builtin_amdgcn_readlane(u, 48));
+  return (a + b) + (c + d);
 }
 
 // ---- forward ------------------------------------------------------------------------------
@@ -170,25 +182,23 @@ __global__ __launch_bounds__(256) void readout_kernel(const ReadoutArgs r) {
   }
 }
 
-// Sums n rows of `width` values (row stride `stride`) column-wise into out[width] in FP64
-// (one block, fixed order).
+// Sums n rows of values (row stride `stride`) column-wise into out[] in FP64, one block per
+// column (launched with `width` blocks), in a fixed order.
 template <typename T>
 __global__ __launch_bounds__(256) void reduce_columns_kernel(const T* __restrict__ part, int n,
-                                                            int width, double* __restrict__ out,
+                                                            double* __restrict__ out,
                                                             int stride) {
   __shared__ double red[256];
-  for (int c = 0; c < width; ++c) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) s += (double)part[(long)i * stride + c];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int h = 128; h >= 1; h >>= 1) {
-      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[c] = red[0];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += (double)part[(long)i * stride + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[c] = red[0];
 }
 
 __global__ void finish_loss_kernel(double* sums, double* loss) {
@@ -857,7 +867,7 @@ extern "C" pm_status pm_diff_forward(pm_diff* h, const float* ref,
   if (s != PM_OK) return s;
   const ReadoutArgs r = diff_readout_args(h);
   dispatch_ppl(h->shape.planes, [&](auto ppl) { launch_readout<decltype(ppl)::value>(r, st); });
-  reduce_columns_kernel<double><<<1, 256, 0, st>>>(h->part, kReadoutBlocks, 2, h->sums, 2);
+  reduce_columns_kernel<double><<<2, 256, 0, st>>>(h->part, kReadoutBlocks, h->sums, 2);
   finish_loss_kernel<<<1, 64, 0, st>>>(h->sums, loss);
   s = check_launch("readout_kernel");
   if (s != PM_OK) return s;
@@ -913,7 +923,7 @@ extern "C" pm_status pm_diff_backward(pm_diff* h, float grad_loss, float* grad_r
   s = check_launch("ref_grad_gather_kernel");
   if (s != PM_OK) return s;
   // dL/d(H, t): the S * 12 columns of the per-(tile, wave) partials, summed in FP64
-  reduce_columns_kernel<float><<<1, 256, 0, st>>>(h->posepart, h->tiles_adj * 4, h->S * 12,
-                                                  grad_homs, h->S * 12);
+  reduce_columns_kernel<float><<<h->S * 12, 256, 0, st>>>(h->posepart, h->tiles_adj * 4,
+                                                          grad_homs, h->S * 12);
   return check_launch("reduce_columns_kernel");
 }
