@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # HBM bytes of one SGM pass from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) per dispatch,
 # calibrated on a 1 GiB copy (profiles/depth_pipeline_752x480x64_s5_p8_f32.md)
-SGM_PASS_TRAFFIC = {("f32", 8, 480, 752, 64): 2.15e9}
+SGM_PASS_TRAFFIC = {}
 FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed)
 
 
@@ -293,7 +293,7 @@ def main():
 
     # timed region: K full sequences; HIP events on the launch stream bracket each stage
     stream = torch.cuda.current_stream()
-    ev = {k: [] for k in ("cost", "sgm", "filter", "fusion")}
+    ev = {k: [] for k in ("cost", "sgm", "median", "fusion")}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -304,10 +304,10 @@ def main():
             pl.calc_cost(seq.images[i], [seq.quads[j] for j in seq.sources[i]], seq.homs[i],
                          seq.idep_min, seq.dep_sample, seq.D, seq.cost_type, out=seq.cost)
             e[1].record(stream)
-            pl.sgm(seq.images[i], seq.cost, seq.knobs, seq.paths, out=seq.agg)
+            pl.sgm_filter_paths(seq.images[i], seq.cost, seq.knobs, seq.paths, seq.idep_min,
+                                seq.dep_sample, workspace=seq.sgm_ws, agg=seq.agg,
+                                depth=seq.dep_raw if seq.median else seq.dep)
             e[2].record(stream)
-            pl.filter_cost(seq.agg, seq.knobs["var_scale"], seq.idep_min, seq.dep_sample,
-                           out=seq.dep_raw if seq.median else seq.dep)
             if seq.median:
                 pl.median3x3(seq.dep_raw, out=seq.dep)
             e[3].record(stream)
@@ -321,7 +321,11 @@ def main():
 
     frames = args.steps * nkf
     value = frames / elapsed
-    sgm_bytes = P * V * (bc + 8)                       # reference dataflow per pass
+    # the SGM pass: all paths at once (read cost, write the path's L volume), then one pass
+    # that reads the P volumes, writes the aggregate and the filtered depth.  The same bytes
+    # as the reference's path-after-path read-modify-write (P V (b_c + 8)) plus its filter
+    # pass (V 4 read, N 4 write), minus the filter's re-read of the aggregate.
+    sgm_bytes = P * V * (bc + 8) + V * 4 + N * 4
     sgm_gbs = sgm_bytes / (ms["sgm"] * 1e-3) / 1e9
     cost_flop = V * S * 9 * 20                          # ~20 FLOP per warp term
     cost_bytes = V * bc + (S + 1) * N
@@ -331,10 +335,9 @@ def main():
                  "peak_tflops": FP32_VALU_PEAK_TFLOPS,
                  "hbm_gbs": cost_bytes / (ms["cost"] * 1e-3) / 1e9},
         "sgm": {"ms": ms["sgm"], "bound": "hbm", "achieved_gbs": sgm_gbs,
-                "frac": sgm_gbs / HBM_PEAK_GBS,
-                "compulsory_gbs": V * (bc + 4) / (ms["sgm"] * 1e-3) / 1e9},
-        "filter": {"ms": ms["filter"], "bound": "hbm", "median": bool(args.median),
-                   "achieved_gbs": (V * 4 + N * 4) / (ms["filter"] * 1e-3) / 1e9},
+                "frac": sgm_gbs / HBM_PEAK_GBS, "includes": "filter (WTA + subpixel)",
+                "compulsory_gbs": (V * (bc + 4) + N * 4) / (ms["sgm"] * 1e-3) / 1e9},
+        "median": {"ms": ms["median"], "enabled": bool(args.median)},
         "fusion": {"ms": ms["fusion"], "bound": "hbm",
                    "achieved_gbs": fus_bytes / (ms["fusion"] * 1e-3) / 1e9},
     }
@@ -357,7 +360,8 @@ def main():
                                "sequence per step",
                    "form": "plain", "cost_storage": args.cost_type,
                    "median": bool(args.median), "keyframes_per_step": nkf},
-        "roofline": {"bound": "hbm", "kernel": "sgm pass (8 path launches)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "sgm pass (all paths in one launch + combine/filter)",
                      "achieved": sgm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sgm_gbs / HBM_PEAK_GBS,
                      "algorithmic_bytes": sgm_bytes,

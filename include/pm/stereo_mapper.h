@@ -88,9 +88,10 @@ class StereoMapper {
   pm_cost_type cost_type_ = PM_COST_F32;
   pm_cost_type raw_cost_type_ = PM_COST_F32;
   void* stream_ = nullptr;
-  std::vector<DeviceMat> quads_;           // per staged source, uint32 [H][W]
+  std::vector<DeviceMat> quads_;           // per staged source, 8-byte texels [H][W]
   std::vector<pm_homography> homs_;
   DeviceMat zbuf_, med_tmp_, wta_tmp_;
+  DeviceMat sgm_ws_;                       // per-path L volumes (pm_sgm_filter_paths)
   bool cost_ready_ = false;
   bool depth_ready_ = false;
   bool have_reference_ = false;

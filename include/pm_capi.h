@@ -126,6 +126,18 @@ pm_status pm_sgm_filter(pm_shape shape, const uint8_t* ref, const void* cost,
                         float idep_min, float dep_sample, float* agg, float* depth,
                         pm_stream stream);
 
+/* pm_sgm_filter with every path in flight at once: the paths write their own L volumes to a
+ * caller-provided workspace (pm_sgm_workspace_size bytes: paths * W * H * D floats, 16-byte
+ * aligned) instead of updating agg one after another, and one pass sums them in path order
+ * into agg and filters.  agg and depth are identical, bit for bit, to pm_sgm_filter's (the
+ * sum runs in the same order); agg may be NULL when only depth is wanted. */
+pm_status pm_sgm_workspace_size(pm_shape shape, int32_t paths, uint64_t* bytes);
+pm_status pm_sgm_filter_paths(pm_shape shape, const uint8_t* ref, const void* cost,
+                              pm_cost_type cost_type, pm_sgm_params params, float var_scale,
+                              float idep_min, float dep_sample, void* workspace,
+                              uint64_t workspace_bytes, float* agg, float* depth,
+                              pm_stream stream);
+
 /* Winner-take-all, rejection rules and parabola subpixel fit -> depth [H][W].
  * Replaces filter_cost (motion_stereo/filter_cost.h:7-8, filter_cost.cu:3-70); depth is a
  * dense [H][W] map instead of a pitched one.  depth = 1 / (idep_min + s * dep_sample). */

@@ -105,6 +105,10 @@ SIGNATURES = {
     "pm_sgm": (_i32, [Shape, _vp, _vp, _i32, SgmParams, _vp, _vp]),
     "pm_sgm_filter": (_i32, [Shape, _vp, _vp, _i32, SgmParams, ctypes.c_float, ctypes.c_float,
                              ctypes.c_float, _vp, _vp, _vp]),
+    "pm_sgm_workspace_size": (_i32, [Shape, _i32, ctypes.POINTER(ctypes.c_uint64)]),
+    "pm_sgm_filter_paths": (_i32, [Shape, _vp, _vp, _i32, SgmParams, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_float, _vp, ctypes.c_uint64, _vp,
+                                   _vp, _vp]),
     "pm_filter_cost": (_i32, [Shape, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp,
                               _vp]),
     "pm_wta": (_i32, [Shape, _vp, _i32, ctypes.c_float, ctypes.c_float, _vp, _vp]),

@@ -82,6 +82,10 @@ int filter_lanes(int D);
 int filter_items(int W, int H, int D);
 void launch_filter_batch(int LP, const KfDesc* kfs, const int* kf, const int* first, int n,
                          int total, int grid, hipStream_t st);
+// Sum of the per-path L volumes (path order) -> agg (skipped if NULL) -> filter -> depth.
+void launch_combine_filter(int W, int H, int D, int paths, const float* lbuf, float* agg,
+                           float var_scale, float idep_min, float dep_sample, float* depth,
+                           hipStream_t st);
 
 // ---- host error plumbing -------------------------------------------------------------
 void set_error(const std::string& msg);

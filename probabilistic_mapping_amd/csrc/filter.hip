@@ -10,26 +10,19 @@
 
 namespace pm {
 
-template <int LP>
-__device__ __forceinline__ void filter_pixels(long N, int D, bool vec4,
-                                              const float* __restrict__ agg, float var_scale,
-                                              float idep_min, float dep_sample,
-                                              float* __restrict__ depth, long wave) {
+// Filter of the pixels of one wave; load(pc, d0, v) fills v[0..3] with planes d0 .. d0+3 of
+// pixel pc (+inf past D).
+template <int LP, typename Load>
+__device__ __forceinline__ void filter_pixels_impl(long N, int D, Load&& load, float var_scale,
+                                                   float idep_min, float dep_sample,
+                                                   float* __restrict__ depth, long wave) {
   const int lane = threadIdx.x & 63;
   const int grp = lane / LP, k = lane % LP;
   const long p = wave * (64 / LP) + grp;
   const long pc = p < N ? p : N - 1;   // out-of-range groups re-read the last pixel
-  const float* c = agg + pc * D;
-  const float INF = __uint_as_float(0x7F800000u);
   const int d0 = 4 * k;
   float v[4];
-  if (vec4 && d0 + 3 < D) {
-    const float4 t = *reinterpret_cast<const float4*>(c + d0);
-    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-  } else {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = d0 + j < D ? c[d0 + j] : INF;
-  }
+  load(pc, p < N, d0, v);
   float best = v[0];
   int mi = d0;
 #pragma unroll
@@ -61,6 +54,81 @@ __device__ __forceinline__ void filter_pixels(long N, int D, bool vec4,
     out = 1.0f / (idep_min + s * dep_sample);
   }
   if (k == 0 && p < N) depth[p] = out;
+}
+
+template <int LP>
+__device__ __forceinline__ void filter_pixels(long N, int D, bool vec4,
+                                              const float* __restrict__ agg, float var_scale,
+                                              float idep_min, float dep_sample,
+                                              float* __restrict__ depth, long wave) {
+  const float INF = __uint_as_float(0x7F800000u);
+  auto load = [&](long pc, bool, int d0, float (&v)[4]) {
+    const float* c = agg + pc * D;
+    if (vec4 && d0 + 3 < D) {
+      const float4 t = *reinterpret_cast<const float4*>(c + d0);
+      v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = d0 + j < D ? c[d0 + j] : INF;
+    }
+  };
+  filter_pixels_impl<LP>(N, D, load, var_scale, idep_min, dep_sample, depth, wave);
+}
+
+// Aggregate = sum of the per-path volumes lbuf[r] (r = 0 .. paths-1, each [H][W][D], 0 at
+// invalid pixels), added in path order exactly as the sequential read-modify-write passes add
+// them (sgm.cu:101-108), stored to agg, then filtered as above.
+template <int LP>
+__global__ __launch_bounds__(256) void combine_filter_kernel(long N, int D, int paths, bool vec4,
+                                                             const float* __restrict__ lbuf,
+                                                             long V, float* __restrict__ agg,
+                                                             float var_scale, float idep_min,
+                                                             float dep_sample,
+                                                             float* __restrict__ depth) {
+  const float INF = __uint_as_float(0x7F800000u);
+  const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  auto load = [&](long pc, bool own, int d0, float (&v)[4]) {
+    const long e = pc * D + d0;
+    if (vec4 && d0 + 3 < D) {
+      float4 t = *reinterpret_cast<const float4*>(lbuf + e);
+      for (int r = 1; r < paths; ++r) {
+        const float4 u = *reinterpret_cast<const float4*>(lbuf + (long)r * V + e);
+        t.x += u.x; t.y += u.y; t.z += u.z; t.w += u.w;
+      }
+      if (own && agg) *reinterpret_cast<float4*>(agg + e) = t;
+      v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (d0 + j < D) {
+          float a = lbuf[e + j];
+          for (int r = 1; r < paths; ++r) a += lbuf[(long)r * V + e + j];
+          if (own && agg) agg[e + j] = a;
+          v[j] = a;
+        } else {
+          v[j] = INF;
+        }
+      }
+    }
+  };
+  filter_pixels_impl<LP>(N, D, load, var_scale, idep_min, dep_sample, depth, wave);
+}
+
+void launch_combine_filter(int W, int H, int D, int paths, const float* lbuf, float* agg,
+                           float var_scale, float idep_min, float dep_sample, float* depth,
+                           hipStream_t st) {
+  const long N = (long)W * H;
+  const long V = N * D;
+  const bool vec4 = (D % 4 == 0) && (agg == nullptr || (uintptr_t)agg % 16 == 0);
+  const int LP = filter_lanes(D);
+  const unsigned blocks = (unsigned)filter_items(W, H, D);
+  switch (LP) {
+    case 4: combine_filter_kernel<4><<<blocks, 256, 0, st>>>(N, D, paths, vec4, lbuf, V, agg, var_scale, idep_min, dep_sample, depth); break;
+    case 8: combine_filter_kernel<8><<<blocks, 256, 0, st>>>(N, D, paths, vec4, lbuf, V, agg, var_scale, idep_min, dep_sample, depth); break;
+    case 16: combine_filter_kernel<16><<<blocks, 256, 0, st>>>(N, D, paths, vec4, lbuf, V, agg, var_scale, idep_min, dep_sample, depth); break;
+    case 32: combine_filter_kernel<32><<<blocks, 256, 0, st>>>(N, D, paths, vec4, lbuf, V, agg, var_scale, idep_min, dep_sample, depth); break;
+    default: combine_filter_kernel<64><<<blocks, 256, 0, st>>>(N, D, paths, vec4, lbuf, V, agg, var_scale, idep_min, dep_sample, depth); break;
+  }
 }
 
 template <int LP>

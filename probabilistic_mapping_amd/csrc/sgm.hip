@@ -36,6 +36,35 @@ __global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
   sgm_scanline<NV, CT, FIRST, FULL, B>(a, line, cost, agg);
 }
 
+// Every scanline direction of one keyframe in a single launch.  Global line g counts the lines
+// of path 0, then path 1, ...; each path writes its own L volume (0 at invalid pixels) to
+// lbuf + path * V, so no two waves touch the same output and all paths run at once.  The
+// aggregate is the sum of those volumes in path order (combine_filter_kernel), which is the
+// same sum, in the same order, as the path-after-path read-modify-write of sgm_path_kernel.
+template <int NV, typename CT, bool FULL, int B>
+__global__ __launch_bounds__(256) void sgm_paths_kernel(const SgmArgs base, int paths,
+                                                        const CT* __restrict__ cost,
+                                                        float* __restrict__ lbuf) {
+  const int dxs[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+  const int dys[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int g = blockIdx.x * 4 + wave;
+  const size_t V = (size_t)base.W * base.H * base.D;
+  for (int path = 0; path < paths; ++path) {
+    const int n = sgm_lines(base.W, base.H, dxs[path], dys[path]);
+    if (g < n) {
+      SgmArgs a = base;
+      a.dx = dxs[path];
+      a.dy = dys[path];
+      a.nlines = n;
+      a.lout = lbuf + (size_t)path * V;
+      sgm_scanline<NV, CT, true, FULL, B, uint8_t, false, true, true>(a, g, cost, nullptr);
+      return;
+    }
+    g -= n;
+  }
+}
+
 // Batched form of one path direction: the scanlines of every keyframe of a class (same NV,
 // FULL) form one work list of 4-scanline items; keyframe kf[j] owns items first[j] ..
 // first[j+1]-1.  Persistent blocks stride through the list.
@@ -136,10 +165,31 @@ pm_status launch_sgm(const SgmArgs& a, int paths, const CT* cost, float* agg, hi
   return launch_paths<4, CT, false>(a, paths, cost, agg, st);
 }
 
-pm_status sgm_entry(const char* who, pm_shape shape, const uint8_t* ref, const void* cost,
-                    pm_cost_type cost_type, pm_sgm_params params, float* agg, hipStream_t st) {
-  const std::string w(who);
-  if (!ref || !cost || !agg) return fail(PM_ERR_INVALID_ARGUMENT, w + ": null pointer");
+template <int NV, typename CT, bool FULL>
+void launch_paths_concurrent(const SgmArgs& a, int paths, const CT* cost, float* lbuf,
+                             hipStream_t st) {
+  constexpr int B = PM_SGM_PREFETCH;
+  long lines = 0;
+  for (int path = 0; path < paths; ++path)
+    lines += sgm_lines(a.W, a.H, kPathDx[path], kPathDy[path]);
+  const unsigned blocks = (unsigned)((lines + 3) / 4);
+  sgm_paths_kernel<NV, CT, FULL, B><<<blocks, 256, 0, st>>>(a, paths, cost, lbuf);
+}
+
+template <typename CT>
+pm_status launch_sgm_concurrent(const SgmArgs& a, int paths, const CT* cost, float* lbuf,
+                                hipStream_t st) {
+  if (a.D == 64) launch_paths_concurrent<1, CT, true>(a, paths, cost, lbuf, st);
+  else if (a.D == 128) launch_paths_concurrent<2, CT, true>(a, paths, cost, lbuf, st);
+  else if (a.D < 64) launch_paths_concurrent<1, CT, false>(a, paths, cost, lbuf, st);
+  else if (a.D < 128) launch_paths_concurrent<2, CT, false>(a, paths, cost, lbuf, st);
+  else launch_paths_concurrent<4, CT, false>(a, paths, cost, lbuf, st);
+  return check_launch("sgm_paths_kernel");
+}
+
+// Validates the common SGM arguments and fills the kernel arguments.
+pm_status sgm_args(const std::string& w, pm_shape shape, const uint8_t* ref,
+                   pm_sgm_params params, SgmArgs* out) {
   if (shape.width <= 0 || shape.height <= 0 || shape.width > 65535 || shape.height > 65535)
     return fail(PM_ERR_INVALID_ARGUMENT, w + ": bad image size");
   if (shape.planes <= 0 || shape.planes > PM_MAX_PLANES)
@@ -158,6 +208,17 @@ pm_status sgm_entry(const char* who, pm_shape shape, const uint8_t* ref, const v
   a.P1s = params.pi1 * params.sgm_q1;   // the same float products as sgm.cu:80-81
   a.P2s = params.pi2 * params.sgm_q2;
   a.tau_i = (int)std::min(256.0f, std::ceil(params.tau_so));
+  *out = a;
+  return PM_OK;
+}
+
+pm_status sgm_entry(const char* who, pm_shape shape, const uint8_t* ref, const void* cost,
+                    pm_cost_type cost_type, pm_sgm_params params, float* agg, hipStream_t st) {
+  const std::string w(who);
+  if (!ref || !cost || !agg) return fail(PM_ERR_INVALID_ARGUMENT, w + ": null pointer");
+  SgmArgs a;
+  const pm_status s = sgm_args(w, shape, ref, params, &a);
+  if (s != PM_OK) return s;
   switch (cost_type) {
     case PM_COST_F32: return launch_sgm<float>(a, params.paths, (const float*)cost, agg, st);
     case PM_COST_F16: return launch_sgm<__half>(a, params.paths, (const __half*)cost, agg, st);
@@ -176,6 +237,50 @@ extern "C" pm_status pm_sgm(pm_shape shape, const uint8_t* ref, const void* cost
                             pm_stream stream) {
   clear_error();
   return sgm_entry("pm_sgm", shape, ref, cost, cost_type, params, agg, (hipStream_t)stream);
+}
+
+extern "C" pm_status pm_sgm_workspace_size(pm_shape shape, int32_t paths, uint64_t* bytes) {
+  clear_error();
+  if (!bytes) return fail(PM_ERR_INVALID_ARGUMENT, "pm_sgm_workspace_size: null pointer");
+  if (shape.width <= 0 || shape.height <= 0 || shape.planes <= 0 || (paths != 4 && paths != 8))
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_sgm_workspace_size: bad shape or path count");
+  *bytes = (uint64_t)paths * shape.width * shape.height * shape.planes * sizeof(float);
+  return PM_OK;
+}
+
+extern "C" pm_status pm_sgm_filter_paths(pm_shape shape, const uint8_t* ref, const void* cost,
+                                         pm_cost_type cost_type, pm_sgm_params params,
+                                         float var_scale, float idep_min, float dep_sample,
+                                         void* workspace, uint64_t workspace_bytes, float* agg,
+                                         float* depth, pm_stream stream) {
+  clear_error();
+  const std::string w("pm_sgm_filter_paths");
+  if (!ref || !cost || !workspace || !depth)
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": null pointer");
+  SgmArgs a;
+  pm_status s = sgm_args(w, shape, ref, params, &a);
+  if (s != PM_OK) return s;
+  uint64_t need = 0;
+  s = pm_sgm_workspace_size(shape, params.paths, &need);
+  if (s != PM_OK) return s;
+  if (workspace_bytes < need)
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": workspace smaller than pm_sgm_workspace_size");
+  if ((uintptr_t)workspace % 16 != 0)
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": workspace must be 16-byte aligned");
+  hipStream_t st = (hipStream_t)stream;
+  float* lbuf = static_cast<float*>(workspace);
+  switch (cost_type) {
+    case PM_COST_F32: s = launch_sgm_concurrent<float>(a, params.paths, (const float*)cost, lbuf, st); break;
+    case PM_COST_F16: s = launch_sgm_concurrent<__half>(a, params.paths, (const __half*)cost, lbuf, st); break;
+    case PM_COST_BF16:
+      s = launch_sgm_concurrent<__hip_bfloat16>(a, params.paths, (const __hip_bfloat16*)cost, lbuf, st);
+      break;
+    default: return fail(PM_ERR_INVALID_ARGUMENT, w + ": unknown cost type");
+  }
+  if (s != PM_OK) return s;
+  launch_combine_filter(a.W, a.H, a.D, params.paths, lbuf, agg, var_scale, idep_min, dep_sample,
+                        depth, st);
+  return check_launch("combine_filter_kernel");
 }
 
 extern "C" pm_status pm_sgm_filter(pm_shape shape, const uint8_t* ref, const void* cost,

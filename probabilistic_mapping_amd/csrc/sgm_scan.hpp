@@ -29,9 +29,10 @@ __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, 
 // re-read plane D-1 of the same pixel), so the prefetch has no branches and the B loads of
 // the next block stay in flight while the current block computes.
 // RT: reference element type (uint8_t, or float for the differentiable form).  AGG: accumulate
-// into agg (else agg is untouched).  LOUT: also write this path's L to a.lout.
+// into agg (else agg is untouched).  LOUT: also write this path's L to a.lout; with LMASK the
+// stored value is 0 at invalid pixels (the path's contribution to the aggregate).
 template <int NV, typename CT, bool FIRST, bool FULL, int B, typename RT = uint8_t,
-          bool AGG = true, bool LOUT = false>
+          bool AGG = true, bool LOUT = false, bool LMASK = false>
 __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
                                              const CT* __restrict__ cost,
                                              float* __restrict__ agg) {
@@ -121,7 +122,7 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       Ln[j] = act[j] ? (cc + val) - m : INF;       // sgm.cu:98
       const float out = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);
       if (AGG && act[j]) agg[p + off[j]] = out;    // sgm.cu:101-108
-      if (LOUT && act[j]) lout[p + off[j]] = Ln[j];
+      if (LOUT && act[j]) lout[p + off[j]] = (LMASK && invalid) ? kCostInf : Ln[j];
     }
 #pragma unroll
     for (int j = 0; j < NV; ++j) L[j] = Ln[j];
@@ -161,7 +162,7 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       L[j] = act[j] ? (invalid ? kCostInf : c[j]) : INF;
       const float out = invalid ? kCostInf : (FIRST ? L[j] : g[j] + L[j]);
       if (AGG && act[j]) agg[p + off[j]] = out;
-      if (LOUT && act[j]) lout[p + off[j]] = L[j];
+      if (LOUT && act[j]) lout[p + off[j]] = (LMASK && invalid) ? kCostInf : L[j];
     }
   }
   int base = 1;

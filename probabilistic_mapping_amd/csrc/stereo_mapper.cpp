@@ -36,7 +36,7 @@ DeviceMat dalloc(int rows, int cols, int type) {
 
 void dfree(DeviceMat& m) {
   if (m.data) (void)hipFree(m.data);
-  m.data = nullptr;
+  m = DeviceMat();
 }
 
 int storage_type(pm_cost_type t) {
@@ -132,7 +132,7 @@ void StereoMapper::alloc() {
 void StereoMapper::release() {
   DeviceMat* all[] = {&img_l, &raw_cost, &sgm_cost, &dep, &fuseDepth, &propogate_table,
                       &alpha0, &beta0, &mu0, &sigma0, &alpha1, &beta1, &mu1, &sigma1,
-                      &zbuf_, &med_tmp_, &wta_tmp_};
+                      &zbuf_, &med_tmp_, &wta_tmp_, &sgm_ws_};
   for (DeviceMat* m : all) dfree(*m);
   for (DeviceMat& q : quads_) dfree(q);
   quads_.clear();
@@ -231,8 +231,17 @@ void StereoMapper::run_sgm_filter() {
   pm_sgm_params p{pi1, pi2, tau_so, sgm_q1, sgm_q2, SGM_PATHS};
   const pm_shape sh{W_, H_, D_};
   float* fout = reinterpret_cast<float*>(MEDIAN_FILTER ? med_tmp_.data : dep.data);
-  check(pm_sgm_filter(sh, img_l.data, raw_cost.data, raw_cost_type_, p, var_scale, IDEP_MIN,
-                      DEP_SAMPLE, reinterpret_cast<float*>(sgm_cost.data), fout, stream_));
+  // all paths in flight at once through the per-path workspace (same bits as pm_sgm_filter)
+  uint64_t need = 0;
+  check(pm_sgm_workspace_size(sh, SGM_PATHS, &need));
+  if (sgm_ws_.bytes() < need) {
+    hip_check(hipStreamSynchronize((hipStream_t)stream_), "hipStreamSynchronize");
+    dfree(sgm_ws_);
+    sgm_ws_ = dalloc(SGM_PATHS, (int)((long)W_ * H_ * D_), CV_32F);
+  }
+  check(pm_sgm_filter_paths(sh, img_l.data, raw_cost.data, raw_cost_type_, p, var_scale,
+                            IDEP_MIN, DEP_SAMPLE, sgm_ws_.data, sgm_ws_.bytes(),
+                            reinterpret_cast<float*>(sgm_cost.data), fout, stream_));
   if (MEDIAN_FILTER)
     check(pm_median3x3(fout, reinterpret_cast<float*>(dep.data), W_, H_, stream_));
   depth_ready_ = true;

@@ -100,7 +100,7 @@ def sgm(ref: torch.Tensor, cost: torch.Tensor, knobs: dict, paths: int,
 def sgm_filter(ref: torch.Tensor, cost: torch.Tensor, knobs: dict, paths: int, idep_min: float,
                dep_sample: float, agg: torch.Tensor | None = None,
                depth: torch.Tensor | None = None):
-    """SGM with the WTA/subpixel filter folded into the last path -> (agg, depth)."""
+    """SGM (paths one after another) then the WTA/subpixel filter -> (agg, depth)."""
     H, W, D = cost.shape
     if agg is None:
         agg = torch.empty((H, W, D), dtype=torch.float32, device=cost.device)
@@ -111,6 +111,35 @@ def sgm_filter(ref: torch.Tensor, cost: torch.Tensor, knobs: dict, paths: int, i
     C.check(C.load().pm_sgm_filter(C.Shape(W, H, D), _ptr(ref), _ptr(cost), _cost_code(cost), p,
                                    float(knobs["var_scale"]), float(idep_min), float(dep_sample),
                                    _ptr(agg), _ptr(depth), _stream()))
+    return agg, depth
+
+
+def sgm_workspace(H: int, W: int, D: int, paths: int, device="cuda") -> torch.Tensor:
+    """Workspace of ``sgm_filter_paths``: one float32 L volume per path."""
+    b = ctypes.c_uint64()
+    C.check(C.load().pm_sgm_workspace_size(C.Shape(W, H, D), paths, ctypes.byref(b)))
+    return torch.empty((b.value // 4,), dtype=torch.float32, device=device)
+
+
+def sgm_filter_paths(ref: torch.Tensor, cost: torch.Tensor, knobs: dict, paths: int,
+                     idep_min: float, dep_sample: float, workspace: torch.Tensor | None = None,
+                     agg: torch.Tensor | None = None, depth: torch.Tensor | None = None,
+                     write_agg: bool = True):
+    """SGM with every path in flight at once (per-path volumes in ``workspace``), summed in
+    path order and filtered -> (agg, depth); bit-identical to ``sgm_filter``."""
+    H, W, D = cost.shape
+    if workspace is None:
+        workspace = sgm_workspace(H, W, D, paths, cost.device)
+    if agg is None and write_agg:
+        agg = torch.empty((H, W, D), dtype=torch.float32, device=cost.device)
+    if depth is None:
+        depth = torch.empty((H, W), dtype=torch.float32, device=cost.device)
+    p = C.SgmParams(knobs["pi1"], knobs["pi2"], knobs["tau_so"], knobs["sgm_q1"],
+                    knobs["sgm_q2"], paths)
+    C.check(C.load().pm_sgm_filter_paths(
+        C.Shape(W, H, D), _ptr(ref), _ptr(cost), _cost_code(cost), p, float(knobs["var_scale"]),
+        float(idep_min), float(dep_sample), _ptr(workspace), workspace.numel() * 4,
+        _ptr(agg) if agg is not None else None, _ptr(depth), _stream()))
     return agg, depth
 
 
@@ -203,6 +232,7 @@ class KeyframeSequence:
         dt = COST_TYPES[cost_type][1]
         self.cost = torch.empty((H, W, D), dtype=dt, device=self.device)
         self.agg = torch.empty((H, W, D), dtype=torch.float32, device=self.device)
+        self.sgm_ws = sgm_workspace(H, W, D, paths, self.device)
         self.dep = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.dep_raw = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.fused = torch.empty((H, W), dtype=torch.float32, device=self.device)
@@ -222,8 +252,8 @@ class KeyframeSequence:
         calc_cost(self.images[i], [self.quads[j] for j in srcs], self.homs[i], self.idep_min,
                   self.dep_sample, self.D, self.cost_type, out=self.cost)
         out = self.dep_raw if self.median else self.dep
-        sgm_filter(self.images[i], self.cost, self.knobs, self.paths, self.idep_min,
-                   self.dep_sample, agg=self.agg, depth=out)
+        sgm_filter_paths(self.images[i], self.cost, self.knobs, self.paths, self.idep_min,
+                         self.dep_sample, workspace=self.sgm_ws, agg=self.agg, depth=out)
         if self.median:
             median3x3(self.dep_raw, out=self.dep)
         return self.dep

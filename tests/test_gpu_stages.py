@@ -217,3 +217,27 @@ def test_sgm_filter_equals_sgm_then_filter(dev, small_scene, paths, planes):
     agg_b, dep_b = dev.sgm_filter(ref, cost, knobs, paths, s["idep_min"], ds)
     assert torch.equal(agg_a, agg_b)
     assert torch.equal(dep_a, dep_b)
+    # every path in flight at once, summed in path order: the same bits
+    agg_c, dep_c = dev.sgm_filter_paths(ref, cost, knobs, paths, s["idep_min"], ds)
+    assert torch.equal(agg_a, agg_c)
+    assert torch.equal(dep_a, dep_c)
+    _, dep_d = dev.sgm_filter_paths(ref, cost, knobs, paths, s["idep_min"], ds, write_agg=False)
+    assert torch.equal(dep_a, dep_d)
+
+
+@pytest.mark.parametrize("cost_type,planes", [("f16", 64), ("bf16", 200), ("f32", 200)])
+def test_concurrent_paths_equal_sequential(dev, small_scene, cost_type, planes):
+    """pm_sgm_filter_paths against pm_sgm + pm_filter_cost for half-precision storage and a
+    plane count spread over four planes per lane."""
+    s = small_scene
+    K, homs, _, _ = _small_inputs(s)
+    ds = (s["idep_max"] - s["idep_min"]) / (planes - 1)
+    ref = torch.from_numpy(s["ref"]).cuda()
+    quads = [dev.make_quad(torch.from_numpy(im).cuda()) for im in s["srcs"]]
+    cost = dev.calc_cost(ref, quads, homs, s["idep_min"], ds, planes, cost_type)
+    knobs = dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+    agg_a = dev.sgm(ref, cost, knobs, 8)
+    dep_a = dev.filter_cost(agg_a, knobs["var_scale"], s["idep_min"], ds)
+    agg_c, dep_c = dev.sgm_filter_paths(ref, cost, knobs, 8, s["idep_min"], ds)
+    assert torch.equal(agg_a, agg_c)
+    assert torch.equal(dep_a, dep_c)
