@@ -8,9 +8,14 @@ keyframe uses its 5 nearest frames as measurement frames; 64 planes over inverse
 One step = one pass of the whole 20-keyframe sequence (cost over 5 sources -> SGM -> filter ->
 fusion per keyframe) with every frame already resident in HBM.  value = keyframes / second.
 
-Prints one JSON line (rank 0).  Extra objects: "roofline" (the SGM pass, HBM-bound),
-"stages" (per-stage ms and roofline fractions) and "cpu_baseline" (the float/OpenMP oracle on
-a bounded sample of the same workload, on this host's cores).
+Forms: "plain" runs one keyframe at a time (FP32 cost, exact SGM storage by default); "batched"
+runs every keyframe's cost, SGM and filter as one autotuned, hipGraph-captured batch, then the
+fusion steps (FP16 cost, compact SGM storage by default).  The default, "both", reports the
+plain FP32 form as ``value`` and the FP16 batched form beside it, with their ratio.
+
+Prints one JSON line.  Extra objects: "roofline" (the SGM pass, HBM-bound), "stages" (per-stage
+ms and roofline fractions) and "cpu_baseline" (the float/OpenMP oracle on a bounded sample of
+the same workload, on this host's cores).
 """
 from __future__ import annotations
 
@@ -26,10 +31,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-# HBM bytes of one SGM pass from rocprofv3 FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) per dispatch,
-# calibrated on a 1 GiB copy (profiles/depth_pipeline_752x480x64_s5_p8_f32.md)
-SGM_PASS_TRAFFIC = {}
 FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed)
+METRIC = "fused depth frames/s @752x480x64 planes, plain vs FP16-batched; % HBM peak"
+WORKLOAD = ("752x480 frames, 64 planes (idep 0.125-2.0), 5 sources, 8-path SGM, filter, "
+            "Gaussian x Beta fusion, 20-keyframe sequence per step")
+COST_BYTES = {"f32": 4, "f16": 2, "bf16": 2}
 
 
 def parse():
@@ -37,10 +43,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cost-type", default="f32", choices=["f32", "f16", "bf16"])
-    ap.add_argument("--form", default="plain", choices=["plain", "batched", "diff"],
-                    help="plain: one keyframe at a time; batched: all keyframes' cost, SGM and "
-                         "filter in one autotuned, hipGraph-captured batch, then the fusion steps")
+    ap.add_argument("--cost-type", default="f32", choices=["f32", "f16", "bf16"],
+                    help="cost storage of the plain / batched form (both: of the plain form)")
+    ap.add_argument("--form", default="both", choices=["both", "plain", "batched", "diff"])
+    ap.add_argument("--sgm-storage", default=None, choices=["exact", "compact"],
+                    help="SGM per-path storage (default: exact for f32 cost, compact otherwise)")
     ap.add_argument("--median", type=int, default=0)
     ap.add_argument("--cpu-baseline-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample (0 disables it)")
@@ -48,6 +55,10 @@ def parse():
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the attainable-bandwidth copy (0 disables it)")
     return ap.parse_args()
+
+
+def default_storage(cost_type: str, storage: str | None) -> str:
+    return storage or ("exact" if cost_type == "f32" else "compact")
 
 
 def cpu_baseline(scene, seconds: float) -> dict | None:
@@ -99,13 +110,93 @@ def attainable_copy(torch, mib: int) -> dict | None:
     return {"bytes": 2 * n, "ms": ms, "gbs": 2 * n / (ms * 1e-3) / 1e9}
 
 
-def bench_batched(args, torch, seq, s, copy):
-    """Batched form: autotune the batch, capture it into a hipGraph on a side stream, then
-    time K steps of (graph replay for all keyframes' cost/SGM/filter + the fusion steps)."""
+def sgm_pass_bytes(seq, cost_type: str) -> int:
+    """Algorithmic bytes of the SGM pass with its filter: the reference dataflow (each path
+    streams the cost and read-modify-writes the FP32 aggregate, P V (b_c + 8)) plus the filter
+    (V 4 read, N 4 written)."""
+    N = seq.H * seq.W
+    V = N * seq.D
+    return seq.paths * V * (COST_BYTES[cost_type] + 8) + V * 4 + N * 4
+
+
+def stage_report(seq, s, cost_type, ms_cost, ms_sgm, ms_fusion, storage):
+    H, W, D, P = seq.H, seq.W, seq.D, seq.paths
+    S = len(s["sources"][0])
+    N, V = H * W, H * W * D
+    bc = COST_BYTES[cost_type]
+    sgm_bytes = sgm_pass_bytes(seq, cost_type)
+    # compulsory bytes of the pass: the cost read once, the aggregate written once, the depth
+    compulsory = V * (bc + 4) + N * 4
+    cost_flop = V * S * 9 * 20                          # ~20 FLOP per warp term
+    return {
+        "cost": {"ms": ms_cost, "bound": "valu",
+                 "achieved_tflops": cost_flop / (ms_cost * 1e-3) / 1e12,
+                 "peak_tflops": FP32_VALU_PEAK_TFLOPS,
+                 "hbm_gbs": (V * bc + (S + 1) * N) / (ms_cost * 1e-3) / 1e9},
+        "sgm": {"ms": ms_sgm, "bound": "hbm", "storage": storage,
+                "achieved_gbs": sgm_bytes / (ms_sgm * 1e-3) / 1e9,
+                "frac": sgm_bytes / (ms_sgm * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "includes": "filter (WTA + subpixel)",
+                "compulsory_gbs": compulsory / (ms_sgm * 1e-3) / 1e9},
+        "fusion": {"ms": ms_fusion, "bound": "hbm",
+                   "achieved_gbs": 104 * N / (ms_fusion * 1e-3) / 1e9},
+    }
+
+
+def measure_plain(args, torch, s, cost_type, storage):
+    """One keyframe at a time; HIP events on the launch stream bracket each stage."""
+    from probabilistic_mapping_amd import pipeline as pl
+    seq = pl.KeyframeSequence(s["images"], s["poses"], s["sources"], s["scene"].K, s["planes"],
+                              s["idep_min"], s["idep_max"], s["knobs"], s["paths"],
+                              cost_type=cost_type, median=bool(args.median),
+                              sgm_storage=storage)
     nkf = len(s["images"])
-    tuning = seq.batch.autotune(3)
+    for _ in range(args.warmup):
+        seq.run()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = {k: [] for k in ("cost", "sgm", "median", "fusion")}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        seq.reset()
+        for i in range(nkf):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+            e[0].record(stream)
+            pl.calc_cost(seq.images[i], [seq.quads[j] for j in seq.sources[i]], seq.homs[i],
+                         seq.idep_min, seq.dep_sample, seq.D, seq.cost_type, out=seq.cost)
+            e[1].record(stream)
+            seq.sgm_stage(i, seq.dep_raw if seq.median else seq.dep)
+            e[2].record(stream)
+            if seq.median:
+                pl.median3x3(seq.dep_raw, out=seq.dep)
+            e[3].record(stream)
+            seq.fusion_stage(i)
+            e[4].record(stream)
+            for k, (a, b) in zip(ev, zip(e[:-1], e[1:])):
+                ev[k].append((a, b))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
+    stages = stage_report(seq, s, cost_type, ms["cost"], ms["sgm"], ms["fusion"], storage)
+    stages["median"] = {"ms": ms["median"], "enabled": bool(args.median)}
+    return {"value": args.steps * nkf / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
+            "stages": stages, "sgm_bytes": sgm_pass_bytes(seq, cost_type),
+            "form": "plain", "cost_storage": cost_type, "sgm_storage": storage,
+            "keyframes_per_step": nkf}
+
+
+def measure_batched(args, torch, s, cost_type, storage):
+    """Autotune the batch, capture it into a hipGraph on a side stream, then time K steps of
+    (graph replay for all keyframes' cost / SGM / filter + the fusion steps)."""
+    from probabilistic_mapping_amd import pipeline as pl
+    seq = pl.BatchedSequence(s["images"], s["poses"], s["sources"], s["scene"].K, s["planes"],
+                             s["idep_min"], s["idep_max"], s["knobs"], s["paths"],
+                             cost_type=cost_type, median=bool(args.median),
+                             sgm_storage=storage)
+    nkf = len(s["images"])
     side = torch.cuda.Stream()
     with torch.cuda.stream(side):
+        tuning = seq.batch.autotune(3)
         seq.batch.capture()
         torch.cuda.synchronize()
 
@@ -118,71 +209,73 @@ def bench_batched(args, torch, seq, s, copy):
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
-        e0.record(side)
         for _ in range(args.steps):
             step()
-        e1.record(side)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        # per-stage times of the whole batch, from an instrumented pass after the timed steps
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
-        ev[0].record(side)
-        for st in range(3):
-            seq.batch.run_stage(st)
-            ev[st + 1].record(side)
-        seq.reset()
-        for i in range(nkf):
-            seq.fusion_step(i)
-        ev[4].record(side)
-        torch.cuda.synchronize()
-    ms = [ev[i].elapsed_time(ev[i + 1]) / nkf for i in range(4)]
-    H, W, D, P = seq.H, seq.W, seq.D, seq.paths
-    S = len(s["sources"][0])
-    N, V = H * W, H * W * D
-    bc = {"f32": 4, "f16": 2, "bf16": 2}[args.cost_type]
-    sgm_bytes = P * V * (bc + 8)
-    sgm_gbs = sgm_bytes / (ms[1] * 1e-3) / 1e9
-    value = args.steps * nkf / elapsed
-    stages = {
-        "cost": {"ms": ms[0], "bound": "valu",
-                 "achieved_tflops": V * S * 9 * 20 / (ms[0] * 1e-3) / 1e12,
-                 "peak_tflops": FP32_VALU_PEAK_TFLOPS},
-        "sgm": {"ms": ms[1], "bound": "hbm", "achieved_gbs": sgm_gbs,
-                "frac": sgm_gbs / HBM_PEAK_GBS},
-        "filter": {"ms": ms[2], "bound": "hbm"},
-        "fusion": {"ms": ms[3], "bound": "hbm"},
-    }
-    base = cpu_baseline(s, args.cpu_baseline_seconds)
+        # per-stage times of the whole batch, from instrumented passes after the timed steps
+        reps = max(1, min(args.steps, 5))
+        acc = np.zeros(4)
+        for _ in range(reps):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+            ev[0].record(side)
+            for st in range(3):
+                seq.batch.run_stage(st)
+                ev[st + 1].record(side)
+            seq.reset()
+            for i in range(nkf):
+                seq.fusion_step(i)
+            ev[4].record(side)
+            torch.cuda.synchronize()
+            acc += [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
+    ms = acc / reps / nkf
+    # the SGM pass is stage 1 (paths) + stage 2 (filter; compact: combine + filter)
+    stages = stage_report(seq, s, cost_type, ms[0], ms[1] + ms[2], ms[3], storage)
+    stages["sgm"]["paths_ms"] = float(ms[1])
+    stages["sgm"]["filter_ms"] = float(ms[2])
+    return {"value": args.steps * nkf / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
+            "stages": stages, "sgm_bytes": sgm_pass_bytes(seq, cost_type),
+            "form": "batched (20 keyframes per hipGraph replay)", "cost_storage": cost_type,
+            "sgm_storage": storage, "keyframes_per_step": nkf, "tuning": tuning}
+
+
+def report(args, main: dict, extra: dict | None, copy, base) -> dict:
+    sg = main["stages"]["sgm"]
     out = {
-        "metric": "fused depth frames/s @752x480x64 planes, plain vs FP16-batched; % HBM peak",
-        "value": value,
+        "metric": METRIC,
+        "value": main["value"],
         "unit": "fused depth frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": main["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.cost_type == "f32" else f"{args.cost_type}-storage/f32",
+        "dtype": "f32" if main["cost_storage"] == "f32" else f"{main['cost_storage']}-storage/f32",
         "data": "synthetic (seeded textured-plane scene, 20-frame trajectory)",
-        "config": {"workload": "752x480 frames, 64 planes (idep 0.125-2.0), 5 sources, "
-                               "8-path SGM, filter, Gaussian x Beta fusion, 20-keyframe "
-                               "sequence per step",
-                   "form": "batched (20 keyframes per hipGraph replay)",
-                   "cost_storage": args.cost_type, "keyframes_per_step": nkf,
-                   "tuning": tuning},
-        "roofline": {"bound": "hbm", "kernel": "sgm pass (8 path launches over the batch)",
-                     "achieved": sgm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": sgm_gbs / HBM_PEAK_GBS, "algorithmic_bytes": sgm_bytes,
+        "config": {"workload": WORKLOAD, "form": main["form"],
+                   "cost_storage": main["cost_storage"], "sgm_storage": main["sgm_storage"],
+                   "median": bool(args.median), "keyframes_per_step": main["keyframes_per_step"]},
+        "roofline": {"bound": "hbm",
+                     "kernel": "sgm pass (all paths in one launch + combine/filter)",
+                     "achieved": sg["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": sg["frac"], "algorithmic_bytes": main["sgm_bytes"],
                      "traffic": None},
-        "stages": stages,
+        "stages": main["stages"],
         "attainable_copy": copy,
         "cpu_baseline": base,
     }
-    print(json.dumps(out), flush=True)
+    if "tuning" in main:
+        out["config"]["tuning"] = main["tuning"]
+    if extra is not None:
+        out["fp16_batched"] = {k: extra[k] for k in ("value", "ms_per_step", "form",
+                                                     "cost_storage", "sgm_storage", "stages",
+                                                     "tuning")}
+        out["fp16_batched"]["roofline_frac"] = extra["stages"]["sgm"]["frac"]
+        out["fp16_batched_vs_plain"] = extra["value"] / main["value"]
+    return out
 
 
 def bench_diff(args, torch):
@@ -262,115 +355,25 @@ def main():
                                    "(keyframe streams would shard one stream per GPU)"}))
         sys.exit(2)
     import torch
-    from probabilistic_mapping_amd import pipeline as pl
     from probabilistic_mapping_amd import synth
-    from probabilistic_mapping_amd.pipeline import KeyframeSequence
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     if args.form == "diff":
         return bench_diff(args, torch)
     s = synth.trajectory_scene(args.seed)
-    nkf = len(s["images"])
-    Seq = pl.BatchedSequence if args.form == "batched" else KeyframeSequence
-    seq = Seq(s["images"], s["poses"], s["sources"], s["scene"].K, s["planes"],
-              s["idep_min"], s["idep_max"], s["knobs"], s["paths"],
-              cost_type=args.cost_type, median=bool(args.median))
-    H, W, D, P = seq.H, seq.W, seq.D, seq.paths
-    S = len(s["sources"][0])
-    N = H * W
-    V = N * D
-    bc = {"f32": 4, "f16": 2, "bf16": 2}[args.cost_type]
-
     copy = attainable_copy(torch, args.copy_mib)
-
+    storage = default_storage(args.cost_type, args.sgm_storage)
+    extra = None
     if args.form == "batched":
-        return bench_batched(args, torch, seq, s, copy)
-
-    for _ in range(args.warmup):
-        seq.run()
-    torch.cuda.synchronize()
-
-    # timed region: K full sequences; HIP events on the launch stream bracket each stage
-    stream = torch.cuda.current_stream()
-    ev = {k: [] for k in ("cost", "sgm", "median", "fusion")}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        seq.reset()
-        for i in range(nkf):
-            e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
-            e[0].record(stream)
-            pl.calc_cost(seq.images[i], [seq.quads[j] for j in seq.sources[i]], seq.homs[i],
-                         seq.idep_min, seq.dep_sample, seq.D, seq.cost_type, out=seq.cost)
-            e[1].record(stream)
-            pl.sgm_filter_paths(seq.images[i], seq.cost, seq.knobs, seq.paths, seq.idep_min,
-                                seq.dep_sample, workspace=seq.sgm_ws, agg=seq.agg,
-                                depth=seq.dep_raw if seq.median else seq.dep)
-            e[2].record(stream)
-            if seq.median:
-                pl.median3x3(seq.dep_raw, out=seq.dep)
-            e[3].record(stream)
-            seq.fusion_stage(i)
-            e[4].record(stream)
-            for k, (a, b) in zip(ev, zip(e[:-1], e[1:])):
-                ev[k].append((a, b))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
-
-    frames = args.steps * nkf
-    value = frames / elapsed
-    # the SGM pass: all paths at once (read cost, write the path's L volume), then one pass
-    # that reads the P volumes, writes the aggregate and the filtered depth.  The same bytes
-    # as the reference's path-after-path read-modify-write (P V (b_c + 8)) plus its filter
-    # pass (V 4 read, N 4 write), minus the filter's re-read of the aggregate.
-    sgm_bytes = P * V * (bc + 8) + V * 4 + N * 4
-    sgm_gbs = sgm_bytes / (ms["sgm"] * 1e-3) / 1e9
-    cost_flop = V * S * 9 * 20                          # ~20 FLOP per warp term
-    cost_bytes = V * bc + (S + 1) * N
-    fus_bytes = 104 * N
-    stages = {
-        "cost": {"ms": ms["cost"], "bound": "valu", "achieved_tflops": cost_flop / (ms["cost"] * 1e-3) / 1e12,
-                 "peak_tflops": FP32_VALU_PEAK_TFLOPS,
-                 "hbm_gbs": cost_bytes / (ms["cost"] * 1e-3) / 1e9},
-        "sgm": {"ms": ms["sgm"], "bound": "hbm", "achieved_gbs": sgm_gbs,
-                "frac": sgm_gbs / HBM_PEAK_GBS, "includes": "filter (WTA + subpixel)",
-                "compulsory_gbs": (V * (bc + 4) + N * 4) / (ms["sgm"] * 1e-3) / 1e9},
-        "median": {"ms": ms["median"], "enabled": bool(args.median)},
-        "fusion": {"ms": ms["fusion"], "bound": "hbm",
-                   "achieved_gbs": fus_bytes / (ms["fusion"] * 1e-3) / 1e9},
-    }
+        main_r = measure_batched(args, torch, s, args.cost_type, storage)
+    else:
+        main_r = measure_plain(args, torch, s, args.cost_type, storage)
+        if args.form == "both":
+            torch.cuda.empty_cache()
+            extra = measure_batched(args, torch, s, "f16", "compact")
     base = cpu_baseline(s, args.cpu_baseline_seconds)
-    out = {
-        "metric": "fused depth frames/s @752x480x64 planes, plain vs FP16-batched; % HBM peak",
-        "value": value,
-        "unit": "fused depth frames/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if args.cost_type == "f32" else f"{args.cost_type}-storage/f32",
-        "data": "synthetic (seeded textured-plane scene, 20-frame trajectory)",
-        "config": {"workload": "752x480 frames, 64 planes (idep 0.125-2.0), 5 sources, "
-                               "8-path SGM, filter, Gaussian x Beta fusion, 20-keyframe "
-                               "sequence per step",
-                   "form": "plain", "cost_storage": args.cost_type,
-                   "median": bool(args.median), "keyframes_per_step": nkf},
-        "roofline": {"bound": "hbm",
-                     "kernel": "sgm pass (all paths in one launch + combine/filter)",
-                     "achieved": sgm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": sgm_gbs / HBM_PEAK_GBS,
-                     "algorithmic_bytes": sgm_bytes,
-                     "traffic": SGM_PASS_TRAFFIC.get((args.cost_type, P, H, W, D))},
-        "stages": stages,
-        "attainable_copy": copy,
-        "cpu_baseline": base,
-    }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(report(args, main_r, extra, copy, base)), flush=True)
 
 
 if __name__ == "__main__":

@@ -138,6 +138,21 @@ pm_status pm_sgm_filter_paths(pm_shape shape, const uint8_t* ref, const void* co
                               uint64_t workspace_bytes, float* agg, float* depth,
                               pm_stream stream);
 
+/* Compact form of pm_sgm_filter_paths (no reference counterpart; the storage-precision
+ * variant).  The L recurrence of every path runs exactly as in pm_sgm, but each path stores only
+ * its excess over the cost, E = L - C' (0 <= E <= P2 on every step, sgm.cu:85-98), as 16-bit
+ * fixed point with step max(pi2, pi2 * sgm_q2) / 65534 in the workspace
+ * (pm_sgm_excess_workspace_size bytes: paths * W * H * D uint16, 16-byte aligned).  One pass
+ * then rebuilds agg = paths * C + step * sum of the levels (exact integer sum, one fma; 0 at
+ * invalid pixels, sgm.cu:101-104), writes it to agg (may be NULL) and filters it into depth.
+ * |agg - sum_r L_r| <= paths * step / 2 + FP32 rounding. */
+pm_status pm_sgm_excess_workspace_size(pm_shape shape, int32_t paths, uint64_t* bytes);
+pm_status pm_sgm_filter_excess(pm_shape shape, const uint8_t* ref, const void* cost,
+                               pm_cost_type cost_type, pm_sgm_params params, float var_scale,
+                               float idep_min, float dep_sample, void* workspace,
+                               uint64_t workspace_bytes, float* agg, float* depth,
+                               pm_stream stream);
+
 /* Winner-take-all, rejection rules and parabola subpixel fit -> depth [H][W].
  * Replaces filter_cost (motion_stereo/filter_cost.h:7-8, filter_cost.cu:3-70); depth is a
  * dense [H][W] map instead of a pitched one.  depth = 1 / (idep_min + s * dep_sample). */
@@ -205,15 +220,24 @@ typedef struct pm_batch_tuning {
 
 typedef struct pm_batch pm_batch;
 
+/* SGM storage of a batch.  EXACT: the paths update agg one after another, as pm_sgm (results
+ * equal pm_sgm + pm_filter_cost, bit for bit).  COMPACT: every path of every keyframe in one
+ * launch, storing 16-bit excess volumes in a batch-owned scratch (paths * W * H * D uint16
+ * per keyframe), then one combine + filter launch per lane class (results equal
+ * pm_sgm_filter_excess, bit for bit; a keyframe's agg may then be NULL). */
+typedef enum pm_sgm_storage { PM_SGM_EXACT = 0, PM_SGM_COMPACT = 1 } pm_sgm_storage;
+
 /* Copies the keyframe table and work lists to the device (allocates; call once). */
 pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count, pm_cost_type cost_type,
-                          pm_sgm_params params, float var_scale, pm_batch** out);
+                          pm_sgm_params params, float var_scale, pm_sgm_storage storage,
+                          pm_batch** out);
 void pm_batch_destroy(pm_batch* b);
 pm_status pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t);
 pm_status pm_batch_get_tuning(const pm_batch* b, pm_batch_tuning* t);
 /* Enqueues cost, SGM and filter for every keyframe on stream. */
 pm_status pm_batch_run(pm_batch* b, pm_stream stream);
-/* One stage of pm_batch_run: 0 = cost, 1 = SGM (all paths), 2 = filter. */
+/* One stage of pm_batch_run: 0 = cost, 1 = SGM (all paths), 2 = filter (COMPACT: combine +
+ * filter). */
 pm_status pm_batch_run_stage(pm_batch* b, int32_t stage, pm_stream stream);
 /* Times the candidate launch shapes on this batch and keeps the fastest (overwrites outputs). */
 pm_status pm_batch_autotune(pm_batch* b, pm_stream stream, int32_t reps, pm_batch_tuning* best);

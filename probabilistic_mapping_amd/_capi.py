@@ -22,6 +22,9 @@ PM_COST_F32 = 0
 PM_COST_F16 = 1
 PM_COST_BF16 = 2
 
+PM_SGM_EXACT = 0
+PM_SGM_COMPACT = 1
+
 PM_MAX_PLANES = 256
 PM_MAX_SOURCES = 16
 
@@ -109,6 +112,10 @@ SIGNATURES = {
     "pm_sgm_filter_paths": (_i32, [Shape, _vp, _vp, _i32, SgmParams, ctypes.c_float,
                                    ctypes.c_float, ctypes.c_float, _vp, ctypes.c_uint64, _vp,
                                    _vp, _vp]),
+    "pm_sgm_excess_workspace_size": (_i32, [Shape, _i32, ctypes.POINTER(ctypes.c_uint64)]),
+    "pm_sgm_filter_excess": (_i32, [Shape, _vp, _vp, _i32, SgmParams, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, _vp, ctypes.c_uint64, _vp,
+                                    _vp, _vp]),
     "pm_filter_cost": (_i32, [Shape, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp,
                               _vp]),
     "pm_wta": (_i32, [Shape, _vp, _i32, ctypes.c_float, ctypes.c_float, _vp, _vp]),
@@ -117,7 +124,7 @@ SIGNATURES = {
                              FusionState, _vp]),
     "pm_fill_f32": (_i32, [_vp, ctypes.c_size_t, ctypes.c_float, _vp]),
     "pm_batch_create": (_i32, [ctypes.POINTER(Keyframe), _i32, _i32, SgmParams, ctypes.c_float,
-                               ctypes.POINTER(_vp)]),
+                               _i32, ctypes.POINTER(_vp)]),
     "pm_batch_destroy": (None, [_vp]),
     "pm_batch_set_tuning": (_i32, [_vp, ctypes.POINTER(BatchTuning)]),
     "pm_batch_get_tuning": (_i32, [_vp, ctypes.POINTER(BatchTuning)]),

@@ -31,6 +31,8 @@ struct SgmClass {
   int kf_off = 0;            // offset of kf[] in the device int pool
   int first_off[8] = {};     // offsets of first[] per path in the pool
   int total[8] = {};
+  // compact form: one work list over (keyframe, path) segments
+  int seg_off = 0, seg_first_off = 0, nseg = 0, seg_total = 0;
 };
 
 struct FilterClass {
@@ -46,6 +48,8 @@ struct FilterClass {
 struct pm_batch {
   int nkf = 0;
   pm_cost_type ct = PM_COST_F32;
+  pm_sgm_storage storage = PM_SGM_EXACT;
+  uint16_t* d_ebuf = nullptr;   // compact form: excess volumes of every keyframe
   int paths = 4;
   int cus = 256;
   std::vector<pm::KfDesc> host;
@@ -88,6 +92,19 @@ pm_status run_cost(pm_batch* b, hipStream_t st) {
 }
 
 pm_status run_sgm(pm_batch* b, hipStream_t st) {
+  if (b->storage == PM_SGM_COMPACT) {
+    for (const SgmClass& c : b->sgm) {
+      if (c.seg_total == 0) continue;
+      if (!launch_sgm_excess_batch(b->ct, b->tune.sgm_prefetch, c.nv, c.full, b->d_kfs,
+                                   b->d_pool + c.seg_off, b->d_pool + c.seg_first_off, c.nseg,
+                                   c.seg_total,
+                                   grid_for(c.seg_total, b->tune.sgm_blocks_per_cu, b->cus), st))
+        return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch: unsupported sgm tuning");
+      const pm_status s = check_launch("sgm_excess_batch_kernel");
+      if (s != PM_OK) return s;
+    }
+    return PM_OK;
+  }
   for (int p = 0; p < b->paths; ++p) {
     for (const SgmClass& c : b->sgm) {
       const int total = c.total[p];
@@ -108,8 +125,12 @@ pm_status run_sgm(pm_batch* b, hipStream_t st) {
 pm_status run_filter(pm_batch* b, hipStream_t st) {
   for (const FilterClass& c : b->filt) {
     if (c.total == 0) continue;
-    launch_filter_batch(c.lp, b->d_kfs, b->d_pool + c.kf_off, b->d_pool + c.first_off,
-                        (int)c.kf.size(), c.total, c.total, st);
+    if (b->storage == PM_SGM_COMPACT)
+      launch_combine_excess_batch(b->ct, c.lp, b->d_kfs, b->d_pool + c.kf_off,
+                                  b->d_pool + c.first_off, (int)c.kf.size(), c.total, st);
+    else
+      launch_filter_batch(c.lp, b->d_kfs, b->d_pool + c.kf_off, b->d_pool + c.first_off,
+                          (int)c.kf.size(), c.total, c.total, st);
     const pm_status s = check_launch("filter_batch_kernel");
     if (s != PM_OK) return s;
   }
@@ -127,7 +148,8 @@ void drop_graph(pm_batch* b) {
 
 extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
                                      pm_cost_type cost_type, pm_sgm_params params,
-                                     float var_scale, pm_batch** out) {
+                                     float var_scale, pm_sgm_storage storage,
+                                     pm_batch** out) {
   clear_error();
   if (!out) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: null out");
   *out = nullptr;
@@ -137,6 +159,8 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: unknown cost type");
   if (params.paths != 4 && params.paths != 8)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: paths must be 4 or 8");
+  if (storage != PM_SGM_EXACT && storage != PM_SGM_COMPACT)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: unknown SGM storage");
   const float kn[6] = {params.pi1, params.pi2, params.tau_so, params.sgm_q1, params.sgm_q2,
                        var_scale};
   for (float v : kn)
@@ -146,6 +170,7 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
   b->nkf = count;
   b->ct = cost_type;
   b->paths = params.paths;
+  b->storage = storage;
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) == hipSuccess &&
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
@@ -165,7 +190,7 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
       delete b;
       return fail(PM_ERR_INVALID_ARGUMENT, who + "planes or num_sources out of range");
     }
-    if (!in.ref || !in.cost || !in.agg || !in.depth ||
+    if (!in.ref || !in.cost || (!in.agg && storage == PM_SGM_EXACT) || !in.depth ||
         !std::isfinite(in.idep_min) || !std::isfinite(in.dep_sample)) {
       delete b;
       return fail(PM_ERR_INVALID_ARGUMENT, who + "null buffer or non-finite planes");
@@ -195,6 +220,8 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     d.tau_i = (int)std::min(256.0f, std::ceil(params.tau_so));
     d.paths = params.paths;
     d.var_scale = var_scale;
+    d.estep = excess_step(d.P2n, d.P2s);
+    d.einv = d.estep > 0.0f ? 1.0f / d.estep : 0.0f;
   }
   // work lists, packed in one int pool
   std::vector<int> pool;
@@ -249,6 +276,22 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
       c.first_off[p] = put(first);
       c.total[p] = acc;
     }
+    // compact form: segments (keyframe, path) in keyframe order, so the waves in flight at
+    // any time cover all paths of a few keyframes (whose cost volumes stay cache-resident)
+    std::vector<int> seg, sfirst;
+    int acc = 0;
+    for (int kf : c.kf)
+      for (int p = 0; p < b->paths; ++p) {
+        const CostArgs& a = b->host[kf].c;
+        seg.push_back(kf);
+        seg.push_back(p);
+        sfirst.push_back(acc);
+        acc += (sgm_lines(a.W, a.H, kPathDx[p], kPathDy[p]) + 3) / 4;
+      }
+    c.seg_off = put(seg);
+    c.seg_first_off = put(sfirst);
+    c.nseg = (int)sfirst.size();
+    c.seg_total = acc;
   }
   for (FilterClass& c : b->filt) {
     c.kf_off = put(c.kf);
@@ -262,7 +305,22 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     c.first_off = put(first);
     c.total = acc;
   }
-  hipError_t e = hipMalloc(&b->d_kfs, sizeof(KfDesc) * count);
+  hipError_t e = hipSuccess;
+  if (storage == PM_SGM_COMPACT) {
+    // every keyframe's volumes start 16-byte aligned (8 uint16)
+    auto span = [&](const KfDesc& d) {
+      return ((size_t)b->paths * d.c.W * d.c.H * d.c.D + 7) / 8 * 8;
+    };
+    size_t total = 0;
+    for (const KfDesc& d : b->host) total += span(d);
+    e = hipMalloc(&b->d_ebuf, total * sizeof(uint16_t));
+    size_t off = 0;
+    for (KfDesc& d : b->host) {
+      d.ebuf = e == hipSuccess ? b->d_ebuf + off : nullptr;
+      off += span(d);
+    }
+  }
+  if (e == hipSuccess) e = hipMalloc(&b->d_kfs, sizeof(KfDesc) * count);
   if (e == hipSuccess) e = hipMalloc(&b->d_pool, sizeof(int) * std::max<size_t>(pool.size(), 1));
   if (e == hipSuccess)
     e = hipMemcpy(b->d_kfs, b->host.data(), sizeof(KfDesc) * count, hipMemcpyHostToDevice);
@@ -282,6 +340,7 @@ extern "C" void pm_batch_destroy(pm_batch* b) {
   drop_graph(b);
   if (b->d_kfs) (void)hipFree(b->d_kfs);
   if (b->d_pool) (void)hipFree(b->d_pool);
+  if (b->d_ebuf) (void)hipFree(b->d_ebuf);
   delete b;
 }
 

@@ -43,7 +43,19 @@ struct SgmArgs {
   int tau_i;        // integer form of the test: |dI| < tau_so  <=>  |dI| < ceil(tau_so)
   float tau_f;      // tau_so itself, for float reference intensities
   float* lout;      // per-path L volume [H][W][D] (differentiable form), else unused
+  uint16_t* eout;   // per-path excess volume [H][W][D] (compact form), else unused
+  float eq_inv;     // excess quantisation: q = rint(E * eq_inv), E in [0, max(P2n, P2s)]
 };
+
+// Compact per-path storage of the SGM excess E = L - C' (0 <= E <= P2 on every step): 16-bit
+// fixed point with step max(P2n, P2s) / kExcessMaxQ; kExcessInvalid marks an invalid pixel
+// (every plane, every path).
+constexpr uint32_t kExcessMaxQ = 0xFFFEu;
+constexpr uint32_t kExcessInvalid = 0xFFFFu;
+__host__ __device__ inline float excess_step(float P2n, float P2s) {
+  const float m = P2n > P2s ? P2n : P2s;
+  return m > 0.0f ? m / (float)kExcessMaxQ : 0.0f;
+}
 
 // One keyframe of a batch (device table entry).
 struct KfDesc {
@@ -54,6 +66,9 @@ struct KfDesc {
   float P1n, P2n, P1s, P2s;
   int tau_i, paths;
   float var_scale;
+  uint16_t* ebuf;    // compact form: paths x [H][W][D] excess volumes, else null
+  float estep;       // excess_step(P2n, P2s)
+  float einv;        // 1 / estep (0 when estep is 0)
 };
 
 __host__ __device__ inline int sgm_lines(int W, int H, int dx, int dy) {
@@ -86,6 +101,17 @@ void launch_filter_batch(int LP, const KfDesc* kfs, const int* kf, const int* fi
 void launch_combine_filter(int W, int H, int D, int paths, const float* lbuf, float* agg,
                            float var_scale, float idep_min, float dep_sample, float* depth,
                            hipStream_t st);
+
+// Compact form: agg = P C + step sum_r q_r (0 at invalid pixels) -> agg (skipped if NULL)
+// -> filter -> depth.
+void launch_combine_excess(pm_cost_type ct, int W, int H, int D, int paths, const void* cost,
+                           const uint16_t* ebuf, float step, float* agg, float var_scale,
+                           float idep_min, float dep_sample, float* depth, hipStream_t st);
+void launch_combine_excess_batch(pm_cost_type ct, int LP, const KfDesc* kfs, const int* kf,
+                                 const int* first, int n, int total, hipStream_t st);
+bool launch_sgm_excess_batch(pm_cost_type ct, int prefetch, int nv, bool full,
+                             const KfDesc* kfs, const int* seg, const int* first, int nseg,
+                             int total, int grid, hipStream_t st);
 
 // ---- host error plumbing -------------------------------------------------------------
 void set_error(const std::string& msg);

@@ -131,6 +131,185 @@ void launch_combine_filter(int W, int H, int D, int paths, const float* lbuf, fl
   }
 }
 
+// Four consecutive planes (d0 % 4 == 0, 4-element aligned) of a cost volume, as floats.
+template <typename CT>
+__device__ __forceinline__ void load_cost4(const CT* p, float (&v)[4]);
+template <>
+__device__ __forceinline__ void load_cost4<float>(const float* p, float (&v)[4]) {
+  const float4 t = *reinterpret_cast<const float4*>(p);
+  v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+}
+template <>
+__device__ __forceinline__ void load_cost4<__half>(const __half* p, float (&v)[4]) {
+  const uint2 t = *reinterpret_cast<const uint2*>(p);
+  v[0] = __half2float(__ushort_as_half((unsigned short)(t.x & 0xFFFFu)));
+  v[1] = __half2float(__ushort_as_half((unsigned short)(t.x >> 16)));
+  v[2] = __half2float(__ushort_as_half((unsigned short)(t.y & 0xFFFFu)));
+  v[3] = __half2float(__ushort_as_half((unsigned short)(t.y >> 16)));
+}
+template <>
+__device__ __forceinline__ void load_cost4<__hip_bfloat16>(const __hip_bfloat16* p,
+                                                           float (&v)[4]) {
+  const uint2 t = *reinterpret_cast<const uint2*>(p);
+  v[0] = __uint_as_float(t.x << 16);
+  v[1] = __uint_as_float(t.x & 0xFFFF0000u);
+  v[2] = __uint_as_float(t.y << 16);
+  v[3] = __uint_as_float(t.y & 0xFFFF0000u);
+}
+
+// Compact form: the aggregate is rebuilt from the cost and the per-path excess volumes,
+//   agg(p,d) = P C(p,d) + step * sum_r q_r(p,d)   (sum_r L_r = sum_r (C' + E_r), sgm.cu:98),
+// and 0 at invalid pixels (marked kExcessInvalid in path 0's volume; sgm.cu:101-104).  The sum
+// of the 16-bit levels is exact in 32-bit integers, so the only rounding beyond the levels
+// themselves is one fma.
+template <typename CT>
+__device__ __forceinline__ void excess_agg4(const CT* __restrict__ cost,
+                                            const uint16_t* __restrict__ ebuf, long V, long e,
+                                            int D, int d0, int paths, bool vec4, float step,
+                                            float (&v)[4]) {
+  const float INF = __uint_as_float(0x7F800000u);
+  if (vec4 && d0 + 3 < D) {
+    float c[4];
+    load_cost4<CT>(cost + e, c);
+    uint32_t s[4] = {0u, 0u, 0u, 0u};
+    bool bad[4];
+    for (int r = 0; r < paths; ++r) {
+      const uint2 u = *reinterpret_cast<const uint2*>(ebuf + (long)r * V + e);
+      const uint32_t q[4] = {u.x & 0xFFFFu, u.x >> 16, u.y & 0xFFFFu, u.y >> 16};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (r == 0) bad[j] = q[j] == kExcessInvalid;
+        s[j] += q[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      v[j] = bad[j] ? kCostInf : fmaf((float)s[j], step, (float)paths * c[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (d0 + j < D) {
+        uint32_t sum = 0;
+        for (int r = 0; r < paths; ++r) sum += ebuf[(long)r * V + e + j];
+        const bool bad = ebuf[e + j] == kExcessInvalid;
+        v[j] = bad ? kCostInf : fmaf((float)sum, step, (float)paths * load_cost<CT>(cost + e + j));
+      } else {
+        v[j] = INF;
+      }
+    }
+  }
+}
+
+template <int LP, typename CT>
+__device__ __forceinline__ void combine_excess_pixels(long N, int D, int paths, bool vec4,
+                                                      const CT* __restrict__ cost,
+                                                      const uint16_t* __restrict__ ebuf,
+                                                      float step, float* __restrict__ agg,
+                                                      float var_scale, float idep_min,
+                                                      float dep_sample,
+                                                      float* __restrict__ depth, long wave) {
+  const long V = N * D;
+  auto load = [&](long pc, bool own, int d0, float (&v)[4]) {
+    const long e = pc * D + d0;
+    excess_agg4<CT>(cost, ebuf, V, e, D, d0, paths, vec4, step, v);
+    if (own && agg) {
+      if (vec4 && d0 + 3 < D) {
+        *reinterpret_cast<float4*>(agg + e) = make_float4(v[0], v[1], v[2], v[3]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (d0 + j < D) agg[e + j] = v[j];
+      }
+    }
+  };
+  filter_pixels_impl<LP>(N, D, load, var_scale, idep_min, dep_sample, depth, wave);
+}
+
+template <int LP, typename CT>
+__global__ __launch_bounds__(256) void combine_excess_filter_kernel(
+    long N, int D, int paths, bool vec4, const CT* __restrict__ cost,
+    const uint16_t* __restrict__ ebuf, float step, float* __restrict__ agg, float var_scale,
+    float idep_min, float dep_sample, float* __restrict__ depth) {
+  const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  combine_excess_pixels<LP, CT>(N, D, paths, vec4, cost, ebuf, step, agg, var_scale, idep_min,
+                                dep_sample, depth, wave);
+}
+
+template <typename CT>
+void launch_combine_excess_t(int W, int H, int D, int paths, const CT* cost,
+                             const uint16_t* ebuf, float step, float* agg, float var_scale,
+                             float idep_min, float dep_sample, float* depth, hipStream_t st) {
+  const long N = (long)W * H;
+  const bool vec4 = (D % 4 == 0) && (uintptr_t)cost % 16 == 0 && (uintptr_t)ebuf % 8 == 0 &&
+                    (agg == nullptr || (uintptr_t)agg % 16 == 0);
+  const int LP = filter_lanes(D);
+  const unsigned blocks = (unsigned)filter_items(W, H, D);
+  switch (LP) {
+    case 4: combine_excess_filter_kernel<4, CT><<<blocks, 256, 0, st>>>(N, D, paths, vec4, cost, ebuf, step, agg, var_scale, idep_min, dep_sample, depth); break;
+    case 8: combine_excess_filter_kernel<8, CT><<<blocks, 256, 0, st>>>(N, D, paths, vec4, cost, ebuf, step, agg, var_scale, idep_min, dep_sample, depth); break;
+    case 16: combine_excess_filter_kernel<16, CT><<<blocks, 256, 0, st>>>(N, D, paths, vec4, cost, ebuf, step, agg, var_scale, idep_min, dep_sample, depth); break;
+    case 32: combine_excess_filter_kernel<32, CT><<<blocks, 256, 0, st>>>(N, D, paths, vec4, cost, ebuf, step, agg, var_scale, idep_min, dep_sample, depth); break;
+    default: combine_excess_filter_kernel<64, CT><<<blocks, 256, 0, st>>>(N, D, paths, vec4, cost, ebuf, step, agg, var_scale, idep_min, dep_sample, depth); break;
+  }
+}
+
+void launch_combine_excess(pm_cost_type ct, int W, int H, int D, int paths, const void* cost,
+                           const uint16_t* ebuf, float step, float* agg, float var_scale,
+                           float idep_min, float dep_sample, float* depth, hipStream_t st) {
+  switch (ct) {
+    case PM_COST_F32:
+      launch_combine_excess_t<float>(W, H, D, paths, (const float*)cost, ebuf, step, agg, var_scale, idep_min, dep_sample, depth, st);
+      break;
+    case PM_COST_F16:
+      launch_combine_excess_t<__half>(W, H, D, paths, (const __half*)cost, ebuf, step, agg, var_scale, idep_min, dep_sample, depth, st);
+      break;
+    default:
+      launch_combine_excess_t<__hip_bfloat16>(W, H, D, paths, (const __hip_bfloat16*)cost, ebuf, step, agg, var_scale, idep_min, dep_sample, depth, st);
+      break;
+  }
+}
+
+// Batched compact form: keyframes kf[j] of one LP class own items first[j] .. first[j+1]-1.
+template <int LP, typename CT>
+__global__ __launch_bounds__(256) void combine_excess_batch_kernel(const KfDesc* __restrict__ kfs,
+                                                                   const int* __restrict__ kf,
+                                                                   const int* __restrict__ first,
+                                                                   int n, int total) {
+  for (int i = blockIdx.x; i < total; i += gridDim.x) {
+    const int j = owner_of(first, n, i);
+    const KfDesc& k = kfs[kf[j]];
+    const long N = (long)k.c.W * k.c.H;
+    const int D = k.c.D;
+    const bool vec4 = (D % 4 == 0) && ((uintptr_t)k.cost % 16 == 0) &&
+                      ((uintptr_t)k.ebuf % 8 == 0) && ((uintptr_t)k.agg % 16 == 0);
+    const long wave = (long)(i - first[j]) * 4 + (threadIdx.x >> 6);
+    combine_excess_pixels<LP, CT>(N, D, k.paths, vec4, static_cast<const CT*>(k.cost), k.ebuf,
+                                  k.estep, k.agg, k.var_scale, k.c.idep_min, k.c.dep_sample,
+                                  k.depth, wave);
+  }
+}
+
+template <typename CT>
+void launch_combine_excess_batch_t(int LP, const KfDesc* kfs, const int* kf, const int* first,
+                                   int n, int total, hipStream_t st) {
+  switch (LP) {
+    case 4: combine_excess_batch_kernel<4, CT><<<total, 256, 0, st>>>(kfs, kf, first, n, total); break;
+    case 8: combine_excess_batch_kernel<8, CT><<<total, 256, 0, st>>>(kfs, kf, first, n, total); break;
+    case 16: combine_excess_batch_kernel<16, CT><<<total, 256, 0, st>>>(kfs, kf, first, n, total); break;
+    case 32: combine_excess_batch_kernel<32, CT><<<total, 256, 0, st>>>(kfs, kf, first, n, total); break;
+    default: combine_excess_batch_kernel<64, CT><<<total, 256, 0, st>>>(kfs, kf, first, n, total); break;
+  }
+}
+
+void launch_combine_excess_batch(pm_cost_type ct, int LP, const KfDesc* kfs, const int* kf,
+                                 const int* first, int n, int total, hipStream_t st) {
+  switch (ct) {
+    case PM_COST_F32: launch_combine_excess_batch_t<float>(LP, kfs, kf, first, n, total, st); break;
+    case PM_COST_F16: launch_combine_excess_batch_t<__half>(LP, kfs, kf, first, n, total, st); break;
+    default: launch_combine_excess_batch_t<__hip_bfloat16>(LP, kfs, kf, first, n, total, st); break;
+  }
+}
+
 template <int LP>
 __global__ __launch_bounds__(256) void filter_kernel(long N, int D, bool vec4,
                                                      const float* __restrict__ agg,

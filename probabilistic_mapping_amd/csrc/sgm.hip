@@ -65,6 +65,94 @@ __global__ __launch_bounds__(256) void sgm_paths_kernel(const SgmArgs base, int 
   }
 }
 
+// Compact form: every scanline direction of one keyframe in a single launch, each path
+// writing its excess E = L - C' as 16-bit levels to ebuf + path * V (combine_excess_filter
+// rebuilds the aggregate from the cost and these volumes).
+template <int NV, typename CT, bool FULL, int B>
+__global__ __launch_bounds__(256) void sgm_excess_kernel(const SgmArgs base, int paths,
+                                                         const CT* __restrict__ cost,
+                                                         uint16_t* __restrict__ ebuf) {
+  const int dxs[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+  const int dys[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int g = blockIdx.x * 4 + wave;
+  const size_t V = (size_t)base.W * base.H * base.D;
+  for (int path = 0; path < paths; ++path) {
+    const int n = sgm_lines(base.W, base.H, dxs[path], dys[path]);
+    if (g < n) {
+      SgmArgs a = base;
+      a.dx = dxs[path];
+      a.dy = dys[path];
+      a.nlines = n;
+      a.eout = ebuf + (size_t)path * V;
+      sgm_scanline<NV, CT, true, FULL, B, uint8_t, false, false, false, true>(a, g, cost,
+                                                                              nullptr);
+      return;
+    }
+    g -= n;
+  }
+}
+
+// Batched compact form: the work list of a class (same NV, FULL) is a sequence of segments,
+// one per (keyframe, path), each of 4-scanline items; segment j is keyframe seg[2j], path
+// seg[2j+1] and owns items first[j] .. first[j+1]-1.  Persistent blocks stride the list.
+template <int NV, typename CT, bool FULL, int B>
+__global__ __launch_bounds__(256) void sgm_excess_batch_kernel(const KfDesc* __restrict__ kfs,
+                                                               const int* __restrict__ seg,
+                                                               const int* __restrict__ first,
+                                                               int nseg, int total) {
+  const int dxs[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+  const int dys[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int i = blockIdx.x; i < total; i += gridDim.x) {
+    const int j = owner_of(first, nseg, i);
+    const KfDesc& k = kfs[seg[2 * j]];
+    const int path = seg[2 * j + 1];
+    SgmArgs a;
+    a.W = k.c.W; a.H = k.c.H; a.D = k.c.D; a.dx = dxs[path]; a.dy = dys[path];
+    a.nlines = sgm_lines(a.W, a.H, a.dx, a.dy);
+    a.ref = k.c.ref;
+    a.P1n = k.P1n; a.P2n = k.P2n; a.P1s = k.P1s; a.P2s = k.P2s; a.tau_i = k.tau_i;
+    a.eout = k.ebuf + (size_t)path * a.W * a.H * a.D;
+    a.eq_inv = k.einv;
+    const int line = (i - first[j]) * 4 + wave;
+    if (line < a.nlines)
+      sgm_scanline<NV, CT, true, FULL, B, uint8_t, false, false, false, true>(
+          a, line, static_cast<const CT*>(k.cost), nullptr);
+  }
+}
+
+template <typename CT, int B>
+bool sgm_excess_batch_b(int nv, bool full, const KfDesc* kfs, const int* seg, const int* first,
+                        int nseg, int total, int grid, hipStream_t st) {
+  if (nv == 1 && full) sgm_excess_batch_kernel<1, CT, true, B><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
+  else if (nv == 1) sgm_excess_batch_kernel<1, CT, false, B><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
+  else if (nv == 2 && full) sgm_excess_batch_kernel<2, CT, true, B><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
+  else if (nv == 2) sgm_excess_batch_kernel<2, CT, false, B><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
+  else if (nv == 4) sgm_excess_batch_kernel<4, CT, false, B><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
+  else return false;
+  return true;
+}
+
+template <typename CT>
+bool sgm_excess_batch_c(int prefetch, int nv, bool full, const KfDesc* kfs, const int* seg,
+                        const int* first, int nseg, int total, int grid, hipStream_t st) {
+  if (prefetch == 8) return sgm_excess_batch_b<CT, 8>(nv, full, kfs, seg, first, nseg, total, grid, st);
+  if (prefetch == 16) return sgm_excess_batch_b<CT, 16>(nv, full, kfs, seg, first, nseg, total, grid, st);
+  return false;
+}
+
+bool launch_sgm_excess_batch(pm_cost_type ct, int prefetch, int nv, bool full,
+                             const KfDesc* kfs, const int* seg, const int* first, int nseg,
+                             int total, int grid, hipStream_t st) {
+  switch (ct) {
+    case PM_COST_F32: return sgm_excess_batch_c<float>(prefetch, nv, full, kfs, seg, first, nseg, total, grid, st);
+    case PM_COST_F16: return sgm_excess_batch_c<__half>(prefetch, nv, full, kfs, seg, first, nseg, total, grid, st);
+    case PM_COST_BF16: return sgm_excess_batch_c<__hip_bfloat16>(prefetch, nv, full, kfs, seg, first, nseg, total, grid, st);
+    default: return false;
+  }
+}
+
 // Batched form of one path direction: the scanlines of every keyframe of a class (same NV,
 // FULL) form one work list of 4-scanline items; keyframe kf[j] owns items first[j] ..
 // first[j+1]-1.  Persistent blocks stride through the list.
@@ -187,6 +275,28 @@ pm_status launch_sgm_concurrent(const SgmArgs& a, int paths, const CT* cost, flo
   return check_launch("sgm_paths_kernel");
 }
 
+template <int NV, typename CT, bool FULL>
+void launch_excess_t(const SgmArgs& a, int paths, const CT* cost, uint16_t* ebuf,
+                     hipStream_t st) {
+  constexpr int B = PM_SGM_PREFETCH;
+  long lines = 0;
+  for (int path = 0; path < paths; ++path)
+    lines += sgm_lines(a.W, a.H, kPathDx[path], kPathDy[path]);
+  const unsigned blocks = (unsigned)((lines + 3) / 4);
+  sgm_excess_kernel<NV, CT, FULL, B><<<blocks, 256, 0, st>>>(a, paths, cost, ebuf);
+}
+
+template <typename CT>
+pm_status launch_sgm_excess(const SgmArgs& a, int paths, const CT* cost, uint16_t* ebuf,
+                            hipStream_t st) {
+  if (a.D == 64) launch_excess_t<1, CT, true>(a, paths, cost, ebuf, st);
+  else if (a.D == 128) launch_excess_t<2, CT, true>(a, paths, cost, ebuf, st);
+  else if (a.D < 64) launch_excess_t<1, CT, false>(a, paths, cost, ebuf, st);
+  else if (a.D < 128) launch_excess_t<2, CT, false>(a, paths, cost, ebuf, st);
+  else launch_excess_t<4, CT, false>(a, paths, cost, ebuf, st);
+  return check_launch("sgm_excess_kernel");
+}
+
 // Validates the common SGM arguments and fills the kernel arguments.
 pm_status sgm_args(const std::string& w, pm_shape shape, const uint8_t* ref,
                    pm_sgm_params params, SgmArgs* out) {
@@ -293,4 +403,53 @@ extern "C" pm_status pm_sgm_filter(pm_shape shape, const uint8_t* ref, const voi
                                 (hipStream_t)stream);
   if (s != PM_OK) return s;
   return pm_filter_cost(shape, agg, var_scale, idep_min, dep_sample, depth, stream);
+}
+
+extern "C" pm_status pm_sgm_excess_workspace_size(pm_shape shape, int32_t paths,
+                                                  uint64_t* bytes) {
+  clear_error();
+  if (!bytes) return fail(PM_ERR_INVALID_ARGUMENT, "pm_sgm_excess_workspace_size: null pointer");
+  if (shape.width <= 0 || shape.height <= 0 || shape.planes <= 0 || (paths != 4 && paths != 8))
+    return fail(PM_ERR_INVALID_ARGUMENT,
+                "pm_sgm_excess_workspace_size: bad shape or path count");
+  *bytes = (uint64_t)paths * shape.width * shape.height * shape.planes * sizeof(uint16_t);
+  return PM_OK;
+}
+
+extern "C" pm_status pm_sgm_filter_excess(pm_shape shape, const uint8_t* ref, const void* cost,
+                                          pm_cost_type cost_type, pm_sgm_params params,
+                                          float var_scale, float idep_min, float dep_sample,
+                                          void* workspace, uint64_t workspace_bytes, float* agg,
+                                          float* depth, pm_stream stream) {
+  clear_error();
+  const std::string w("pm_sgm_filter_excess");
+  if (!ref || !cost || !workspace || !depth)
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": null pointer");
+  SgmArgs a;
+  pm_status s = sgm_args(w, shape, ref, params, &a);
+  if (s != PM_OK) return s;
+  uint64_t need = 0;
+  s = pm_sgm_excess_workspace_size(shape, params.paths, &need);
+  if (s != PM_OK) return s;
+  if (workspace_bytes < need)
+    return fail(PM_ERR_INVALID_ARGUMENT,
+                w + ": workspace smaller than pm_sgm_excess_workspace_size");
+  if ((uintptr_t)workspace % 16 != 0)
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": workspace must be 16-byte aligned");
+  const float step = excess_step(a.P2n, a.P2s);
+  a.eq_inv = step > 0.0f ? 1.0f / step : 0.0f;
+  hipStream_t st = (hipStream_t)stream;
+  uint16_t* ebuf = static_cast<uint16_t*>(workspace);
+  switch (cost_type) {
+    case PM_COST_F32: s = launch_sgm_excess<float>(a, params.paths, (const float*)cost, ebuf, st); break;
+    case PM_COST_F16: s = launch_sgm_excess<__half>(a, params.paths, (const __half*)cost, ebuf, st); break;
+    case PM_COST_BF16:
+      s = launch_sgm_excess<__hip_bfloat16>(a, params.paths, (const __hip_bfloat16*)cost, ebuf, st);
+      break;
+    default: return fail(PM_ERR_INVALID_ARGUMENT, w + ": unknown cost type");
+  }
+  if (s != PM_OK) return s;
+  launch_combine_excess(cost_type, a.W, a.H, a.D, params.paths, cost, ebuf, step, agg,
+                        var_scale, idep_min, dep_sample, depth, st);
+  return check_launch("combine_excess_filter_kernel");
 }

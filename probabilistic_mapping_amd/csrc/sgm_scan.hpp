@@ -30,14 +30,17 @@ __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, 
 // the next block stay in flight while the current block computes.
 // RT: reference element type (uint8_t, or float for the differentiable form).  AGG: accumulate
 // into agg (else agg is untouched).  LOUT: also write this path's L to a.lout; with LMASK the
-// stored value is 0 at invalid pixels (the path's contribution to the aggregate).
+// stored value is 0 at invalid pixels (the path's contribution to the aggregate).  EQ: write
+// the path's excess E = L - C' to a.eout as 16-bit fixed point (kExcessInvalid at invalid
+// pixels); L itself is computed exactly as in the other forms.
 template <int NV, typename CT, bool FIRST, bool FULL, int B, typename RT = uint8_t,
-          bool AGG = true, bool LOUT = false, bool LMASK = false>
+          bool AGG = true, bool LOUT = false, bool LMASK = false, bool EQ = false>
 __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
                                              const CT* __restrict__ cost,
                                              float* __restrict__ agg) {
   const RT* __restrict__ ref = static_cast<const RT*>(a.ref);
   float* __restrict__ lout = a.lout;
+  uint16_t* __restrict__ eout = a.eout;
   const int lane = threadIdx.x & 63;
   int x, y;
   scanline_start(a, line, x, y);
@@ -123,6 +126,11 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       const float out = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);
       if (AGG && act[j]) agg[p + off[j]] = out;    // sgm.cu:101-108
       if (LOUT && act[j]) lout[p + off[j]] = (LMASK && invalid) ? kCostInf : Ln[j];
+      if (EQ && act[j]) {
+        // val >= m exactly (every candidate is >= m), so E >= 0; rounding to nearest
+        const uint32_t q = min(kExcessMaxQ, (uint32_t)fmaf(val - m, a.eq_inv, 0.5f));
+        eout[p + off[j]] = (uint16_t)(invalid ? kExcessInvalid : q);
+      }
     }
 #pragma unroll
     for (int j = 0; j < NV; ++j) L[j] = Ln[j];
@@ -163,6 +171,7 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       const float out = invalid ? kCostInf : (FIRST ? L[j] : g[j] + L[j]);
       if (AGG && act[j]) agg[p + off[j]] = out;
       if (LOUT && act[j]) lout[p + off[j]] = (LMASK && invalid) ? kCostInf : L[j];
+      if (EQ && act[j]) eout[p + off[j]] = (uint16_t)(invalid ? kExcessInvalid : 0u);
     }
   }
   int base = 1;

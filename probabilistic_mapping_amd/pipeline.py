@@ -143,6 +143,36 @@ def sgm_filter_paths(ref: torch.Tensor, cost: torch.Tensor, knobs: dict, paths: 
     return agg, depth
 
 
+def sgm_excess_workspace(H: int, W: int, D: int, paths: int, device="cuda") -> torch.Tensor:
+    """Workspace of ``sgm_filter_excess``: one uint16 excess volume per path."""
+    b = ctypes.c_uint64()
+    C.check(C.load().pm_sgm_excess_workspace_size(C.Shape(W, H, D), paths, ctypes.byref(b)))
+    return torch.empty((b.value // 2,), dtype=torch.int16, device=device)
+
+
+def sgm_filter_excess(ref: torch.Tensor, cost: torch.Tensor, knobs: dict, paths: int,
+                      idep_min: float, dep_sample: float, workspace: torch.Tensor | None = None,
+                      agg: torch.Tensor | None = None, depth: torch.Tensor | None = None,
+                      write_agg: bool = True):
+    """Compact SGM: every path in flight at once, each storing its 16-bit excess over the
+    cost; the aggregate is rebuilt from the cost and the excess volumes and filtered ->
+    (agg, depth)."""
+    H, W, D = cost.shape
+    if workspace is None:
+        workspace = sgm_excess_workspace(H, W, D, paths, cost.device)
+    if agg is None and write_agg:
+        agg = torch.empty((H, W, D), dtype=torch.float32, device=cost.device)
+    if depth is None:
+        depth = torch.empty((H, W), dtype=torch.float32, device=cost.device)
+    p = C.SgmParams(knobs["pi1"], knobs["pi2"], knobs["tau_so"], knobs["sgm_q1"],
+                    knobs["sgm_q2"], paths)
+    C.check(C.load().pm_sgm_filter_excess(
+        C.Shape(W, H, D), _ptr(ref), _ptr(cost), _cost_code(cost), p, float(knobs["var_scale"]),
+        float(idep_min), float(dep_sample), _ptr(workspace), workspace.numel() * 2,
+        _ptr(agg) if agg is not None else None, _ptr(depth), _stream()))
+    return agg, depth
+
+
 def filter_cost(agg: torch.Tensor, var_scale: float, idep_min: float, dep_sample: float,
                 out: torch.Tensor | None = None) -> torch.Tensor:
     H, W, D = agg.shape
@@ -206,7 +236,7 @@ class KeyframeSequence:
     """
 
     def __init__(self, images, poses, sources, K, planes, idep_min, idep_max, knobs, paths,
-                 cost_type="f32", median=False, device="cuda"):
+                 cost_type="f32", median=False, device="cuda", sgm_storage="exact"):
         self.device = torch.device(device)
         self.H, self.W = np.asarray(images[0]).shape
         self.D = planes
@@ -216,6 +246,8 @@ class KeyframeSequence:
         self.paths = paths
         self.cost_type = cost_type
         self.median = median
+        assert sgm_storage in ("exact", "compact"), sgm_storage
+        self.sgm_storage = sgm_storage
         self.K = np.asarray(K, np.float64)
         self.cam = (self.K[0, 0], self.K[1, 1], self.K[0, 2], self.K[1, 2])
         self.poses = [(np.asarray(R, np.float64), np.asarray(T, np.float64).reshape(3))
@@ -232,7 +264,10 @@ class KeyframeSequence:
         dt = COST_TYPES[cost_type][1]
         self.cost = torch.empty((H, W, D), dtype=dt, device=self.device)
         self.agg = torch.empty((H, W, D), dtype=torch.float32, device=self.device)
-        self.sgm_ws = sgm_workspace(H, W, D, paths, self.device)
+        if sgm_storage == "compact":
+            self.sgm_ws = sgm_excess_workspace(H, W, D, paths, self.device)
+        else:
+            self.sgm_ws = sgm_workspace(H, W, D, paths, self.device)
         self.dep = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.dep_raw = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.fused = torch.empty((H, W), dtype=torch.float32, device=self.device)
@@ -252,11 +287,16 @@ class KeyframeSequence:
         calc_cost(self.images[i], [self.quads[j] for j in srcs], self.homs[i], self.idep_min,
                   self.dep_sample, self.D, self.cost_type, out=self.cost)
         out = self.dep_raw if self.median else self.dep
-        sgm_filter_paths(self.images[i], self.cost, self.knobs, self.paths, self.idep_min,
-                         self.dep_sample, workspace=self.sgm_ws, agg=self.agg, depth=out)
+        self.sgm_stage(i, out)
         if self.median:
             median3x3(self.dep_raw, out=self.dep)
         return self.dep
+
+    def sgm_stage(self, i: int, out: torch.Tensor) -> None:
+        """SGM (every path at once) + filter of keyframe i's cost volume into ``out``."""
+        fn = sgm_filter_excess if self.sgm_storage == "compact" else sgm_filter_paths
+        fn(self.images[i], self.cost, self.knobs, self.paths, self.idep_min, self.dep_sample,
+           workspace=self.sgm_ws, agg=self.agg, depth=out)
 
     def fusion_stage(self, i: int) -> None:
         """One outputFusion step on ``self.dep`` (stereo_mapper.cpp:128-243)."""
@@ -299,9 +339,11 @@ class Batch:
     ``dep_sample``.  Output tensors ``cost``, ``agg`` and ``depth`` are allocated per keyframe.
     """
 
-    def __init__(self, keyframes, cost_type, knobs, paths):
+    def __init__(self, keyframes, cost_type, knobs, paths, storage="exact"):
         self.lib = C.load()
         self.cost_type = cost_type
+        self.storage = storage
+        st = {"exact": C.PM_SGM_EXACT, "compact": C.PM_SGM_COMPACT}[storage]
         code, dt = COST_TYPES[cost_type]
         n = len(keyframes)
         self.kfs = keyframes
@@ -329,7 +371,7 @@ class Batch:
         p = C.SgmParams(knobs["pi1"], knobs["pi2"], knobs["tau_so"], knobs["sgm_q1"],
                         knobs["sgm_q2"], paths)
         h = ctypes.c_void_p()
-        C.check(self.lib.pm_batch_create(arr, n, code, p, float(knobs["var_scale"]),
+        C.check(self.lib.pm_batch_create(arr, n, code, p, float(knobs["var_scale"]), st,
                                          ctypes.byref(h)))
         self._h = h
 
@@ -378,12 +420,12 @@ class BatchedSequence(KeyframeSequence):
     def __init__(self, *args, **kw):
         super().__init__(*args, **kw)
         H, W, D = self.H, self.W, self.D
-        self.cost = self.agg = None   # per keyframe buffers live in the batch
+        self.cost = self.agg = self.sgm_ws = None   # per keyframe buffers live in the batch
         torch.cuda.empty_cache()
         specs = [{"ref": self.images[i], "quads": [self.quads[j] for j in self.sources[i]],
                   "homs": self.homs[i], "planes": D, "idep_min": self.idep_min,
                   "dep_sample": self.dep_sample} for i in range(len(self.images))]
-        self.batch = Batch(specs, self.cost_type, self.knobs, self.paths)
+        self.batch = Batch(specs, self.cost_type, self.knobs, self.paths, self.sgm_storage)
 
     def depth_of(self, i: int) -> torch.Tensor:
         return self.batch.outputs[i]["depth"]

@@ -41,11 +41,15 @@ def uneven():
     return specs
 
 
-def _single(pipeline, kf, cost_type, paths):
+def _single(pipeline, kf, cost_type, paths, storage="exact"):
     cost = pipeline.calc_cost(kf["ref"], kf["quads"], kf["homs"], kf["idep_min"],
                               kf["dep_sample"], kf["planes"], cost_type)
-    agg = pipeline.sgm(kf["ref"], cost, KNOBS, paths)
-    dep = pipeline.filter_cost(agg, KNOBS["var_scale"], kf["idep_min"], kf["dep_sample"])
+    if storage == "compact":
+        agg, dep = pipeline.sgm_filter_excess(kf["ref"], cost, KNOBS, paths, kf["idep_min"],
+                                              kf["dep_sample"])
+    else:
+        agg = pipeline.sgm(kf["ref"], cost, KNOBS, paths)
+        dep = pipeline.filter_cost(agg, KNOBS["var_scale"], kf["idep_min"], kf["dep_sample"])
     return {"cost": cost, "agg": agg, "depth": dep}
 
 
@@ -55,11 +59,15 @@ def _assert_equal(batch, expect):
             assert torch.equal(got[k], want[k]), (i, k)
 
 
-@pytest.mark.parametrize("cost_type,paths", [("f32", 8), ("f16", 8), ("bf16", 4)])
-def test_batch_equals_single_keyframe(uneven, cost_type, paths):
+@pytest.mark.parametrize("cost_type,paths,storage", [("f32", 8, "exact"), ("f16", 8, "exact"),
+                                                      ("bf16", 4, "exact"),
+                                                      ("f16", 8, "compact"),
+                                                      ("bf16", 4, "compact"),
+                                                      ("f32", 8, "compact")])
+def test_batch_equals_single_keyframe(uneven, cost_type, paths, storage):
     from probabilistic_mapping_amd import pipeline
-    expect = [_single(pipeline, kf, cost_type, paths) for kf in uneven]
-    batch = pipeline.Batch(uneven, cost_type, KNOBS, paths)
+    expect = [_single(pipeline, kf, cost_type, paths, storage) for kf in uneven]
+    batch = pipeline.Batch(uneven, cost_type, KNOBS, paths, storage)
     batch.run()
     torch.cuda.synchronize()
     _assert_equal(batch, expect)
@@ -79,10 +87,11 @@ def test_batch_equals_single_keyframe(uneven, cost_type, paths):
                     _assert_equal(batch, expect)
 
 
-def test_batch_autotune_and_graph_replay(uneven):
+@pytest.mark.parametrize("storage", ["exact", "compact"])
+def test_batch_autotune_and_graph_replay(uneven, storage):
     from probabilistic_mapping_amd import pipeline
-    expect = [_single(pipeline, kf, "f16", 8) for kf in uneven]
-    batch = pipeline.Batch(uneven, "f16", KNOBS, 8)
+    expect = [_single(pipeline, kf, "f16", 8, storage) for kf in uneven]
+    batch = pipeline.Batch(uneven, "f16", KNOBS, 8, storage)
     best = batch.autotune(reps=2)
     print(f"[autotune] {json.dumps(best)}", flush=True)
     assert best == batch.tuning
@@ -115,18 +124,20 @@ def test_batch_rejects_bad_input(uneven):
         b.launch()   # nothing captured
 
 
-@pytest.mark.parametrize("cost_type", ["f16", "bf16"])
-def test_storage_precision_error_report(trajectory_scene, cost_type):
-    """Cost volume stored in FP16 / BF16, SGM and fusion accumulated in FP32, FP64 homography
-    setup: errors of the whole batched pipeline against the FP64 oracle on the first 5
-    keyframes of the 752x480x64, 5-source, 8-path sequence."""
+@pytest.mark.parametrize("cost_type,storage", [("f16", "exact"), ("bf16", "exact"),
+                                                ("f16", "compact"), ("bf16", "compact")])
+def test_storage_precision_error_report(trajectory_scene, cost_type, storage):
+    """Cost volume stored in FP16 / BF16, SGM and fusion accumulated in FP32 (compact: the
+    per-path excess stored as 16-bit levels), FP64 homography setup: errors of the whole
+    batched pipeline against the FP64 oracle on the first 5 keyframes of the 752x480x64,
+    5-source, 8-path sequence."""
     from probabilistic_mapping_amd.pipeline import BatchedSequence
     s = trajectory_scene
     n = 5
     args = (s["images"][:n], s["poses"][:n], [[j for j in src if j < n] or [1]
                                                for src in s["sources"][:n]],
             s["scene"].K, s["planes"], s["idep_min"], s["idep_max"], s["knobs"], s["paths"])
-    g = BatchedSequence(*args, cost_type=cost_type)
+    g = BatchedSequence(*args, cost_type=cost_type, sgm_storage=storage)
     o = OracleSequence(*args, precision="f64")
     g.reset()
     g.batch.run()
@@ -141,7 +152,7 @@ def test_storage_precision_error_report(trajectory_scene, cost_type):
             st["state"] = parity.state_parity({k: v.cpu().numpy() for k, v in g.state.items()},
                                               o.state)
         report.append(st)
-        _report(f"{cost_type} kf{i}", st)
+        _report(f"{cost_type} {storage} kf{i}", st)
     depth_p999 = max(r["depth"]["rel"]["p999"] for r in report)
     if cost_type == "f16":
         # the accuracy bar of the FP16-storage form: <= 1e-3 relative depth error

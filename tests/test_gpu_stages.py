@@ -241,3 +241,69 @@ def test_concurrent_paths_equal_sequential(dev, small_scene, cost_type, planes):
     agg_c, dep_c = dev.sgm_filter_paths(ref, cost, knobs, 8, s["idep_min"], ds)
     assert torch.equal(agg_a, agg_c)
     assert torch.equal(dep_a, dep_c)
+
+
+def _excess_step(knobs):
+    p2 = max(np.float32(knobs["pi2"]), np.float32(knobs["pi2"]) * np.float32(knobs["sgm_q2"]))
+    return float(np.float32(p2) / np.float32(65534.0))
+
+
+@pytest.mark.parametrize("cost_type,paths,planes", [("f32", 8, 64), ("f16", 8, 64),
+                                                    ("bf16", 4, 96), ("f16", 8, 200),
+                                                    ("f32", 4, 5), ("f16", 4, 128)])
+def test_compact_sgm_against_exact(dev, small_scene, cost_type, paths, planes):
+    """Compact SGM (16-bit per-path excess) against the exact form on the same cost volume.
+    Both run the same FP32 L recurrence; the compact aggregate P C + step sum q differs from
+    the FP32 sum of L only by the level rounding (<= step / 2 per path) and FP32 rounding of
+    the two sums; invalid pixels are 0 in both, and the filter decisions agree."""
+    s = small_scene
+    K, homs, _, _ = _small_inputs(s)
+    ds = (s["idep_max"] - s["idep_min"]) / (planes - 1)
+    ref = torch.from_numpy(s["ref"]).cuda()
+    quads = [dev.make_quad(torch.from_numpy(im).cuda()) for im in s["srcs"]]
+    cost = dev.calc_cost(ref, quads, homs, s["idep_min"], ds, planes, cost_type)
+    knobs = dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+    agg_e, dep_e = dev.sgm_filter_paths(ref, cost, knobs, paths, s["idep_min"], ds)
+    agg_c, dep_c = dev.sgm_filter_excess(ref, cost, knobs, paths, s["idep_min"], ds)
+    ge, gc = agg_e.cpu().numpy().astype(np.float64), agg_c.cpu().numpy().astype(np.float64)
+    step = _excess_step(knobs)
+    ulp = np.spacing(np.abs(ge).astype(np.float32)).astype(np.float64)
+    bound = paths * step / 2 + 4 * paths * ulp
+    err = np.abs(gc - ge)
+    _report(f"compact vs exact {cost_type} P={paths} D={planes}",
+            {"max_abs": float(err.max()), "max_over_bound": float((err / bound).max())})
+    assert np.all(err <= bound)
+    inval = cost.float().cpu().numpy().min(axis=2) < 0
+    assert np.all(gc[inval] == 0.0)
+    st = parity.depth_parity(dep_c.cpu().numpy(), dep_e.cpu().numpy())
+    assert st["category_mismatch"] <= 1e-3
+    assert st["rel"]["p999"] <= 1e-4
+    # depth-only call gives the same depth
+    _, dep_d = dev.sgm_filter_excess(ref, cost, knobs, paths, s["idep_min"], ds, write_agg=False)
+    assert torch.equal(dep_c, dep_d)
+
+
+def test_compact_sgm_zero_penalties_is_exact(dev, small_scene):
+    """P1 = P2 = 0: every excess is 0 (L = C), so the compact aggregate is P C exactly."""
+    s = small_scene
+    K, homs, D, ds = _small_inputs(s)
+    ref = torch.from_numpy(s["ref"]).cuda()
+    quads = [dev.make_quad(torch.from_numpy(im).cuda()) for im in s["srcs"]]
+    cost = dev.calc_cost(ref, quads, homs, s["idep_min"], ds, D, "f16")
+    knobs = dict(pi1=0.0, pi2=0.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+    agg, _ = dev.sgm_filter_excess(ref, cost, knobs, 8, s["idep_min"], ds)
+    c = cost.float().cpu().numpy()
+    expect = np.where(c.min(axis=2, keepdims=True) < 0, 0.0, 8.0 * c).astype(np.float32)
+    assert np.array_equal(agg.cpu().numpy(), expect)
+
+
+def test_compact_sgm_rejects_small_workspace(dev, small_scene):
+    from probabilistic_mapping_amd import _capi as C
+    s = small_scene
+    ref = torch.from_numpy(s["ref"]).cuda()
+    H, W = s["ref"].shape
+    cost = torch.zeros((H, W, 16), dtype=torch.float16, device="cuda")
+    knobs = dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+    ws = torch.empty((10,), dtype=torch.int16, device="cuda")
+    with pytest.raises(C.PmError):
+        dev.sgm_filter_excess(ref, cost, knobs, 8, 0.2, 0.01, workspace=ws)
