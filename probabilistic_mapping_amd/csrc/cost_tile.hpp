@@ -2,6 +2,8 @@
 // the cost adjoint.  Reference: ADCalcCostKernel, motion_stereo/calc_cost.cu:11-210.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace pm {
@@ -29,6 +31,90 @@ __device__ __forceinline__ float warp_sample(const uint64_t* __restrict__ quad, 
   const float top = fmaf(ax, __high2float(lo), __low2float(lo));
   const float dv = fmaf(ax, __high2float(hi), __low2float(hi));
   return valid ? fmaf(ay, dv, top) : __uint_as_float(0x7F800000u);
+}
+
+// warp_sample for a warp known to be valid and inside the image (see tile_inside_mask): the
+// same arithmetic without the clamp and the validity test, so it returns the same value.
+__device__ __forceinline__ float warp_sample_inside(const uint64_t* __restrict__ quad, uint32_t W,
+                                                    float X, float Y, float w) {
+  const float rw = __builtin_amdgcn_rcpf(w);
+  const float U = X * rw;
+  const float V = Y * rw;
+  const uint32_t off = __umul24((uint32_t)V, W) * 8u + (uint32_t)U * 8u;
+  const uint2 t = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(quad) + off);
+  const float ax = __builtin_amdgcn_fractf(U), ay = __builtin_amdgcn_fractf(V);
+  const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
+  const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
+  const float top = fmaf(ax, __high2float(lo), __low2float(lo));
+  const float dv = fmaf(ax, __high2float(hi), __low2float(hi));
+  return fmaf(ay, dv, top);
+}
+
+// Conservative whole-tile validity, per source.  Every warp a cost_tile wave evaluates for one
+// (plane, source) is a linear-fractional map of the tile's pixel coordinates: the centre warps
+// H (u, r, 1) + t idep over the rectangle u in [u_lo, u_hi], r in [r_lo, r_hi], and the two
+// garbled warps (x + c_X, x + c_Y, x + c_Z) over x = h11 u + h12 v + h13 in [x_min, x_max].
+// Where the denominator w is positive at the corners it is positive on the whole box, and U, V
+// take their extremes at the corners; so if every corner has w >= kInsideW and U, V inside the
+// image by kInsideMargin pixels, every warp of the tile is valid with room to spare for FP32
+// rounding (< 4e-3 px at w >= kInsideW for |coordinates| < 2^16).  Then the validity tests can
+// be skipped: warp_sample_inside returns what warp_sample would.
+// Lane l tests corner l & 7 of source l >> 3 (sources 0..7); tile_inside_mask returns a
+// wave-uniform mask whose bit s says source s passed for the plane at idep.
+constexpr float kInsideW = 0.25f;
+constexpr float kInsideMargin = 1.0f / 32.0f;
+
+struct InsideCorner {
+  float X0, Y0, Z0;   // the corner's warp at idep = 0
+  float tx, ty, tz;   // its idep slope (t of the lane's source)
+  bool live;          // lane tests a real source
+};
+
+template <typename CA>
+__device__ __forceinline__ InsideCorner tile_inside_setup(const CA& a, float u_lo, float u_hi,
+                                                          float r_lo, float r_hi, float v_lo,
+                                                          float v_hi) {
+  const int lane = threadIdx.x & 63;
+  const int s = lane >> 3, k = lane & 7;
+  InsideCorner c;
+  c.live = s < a.S && s < 8;
+  const int ss = c.live ? s : 0;
+  const float* hm = a.hm[ss];
+  const float h11 = hm[0], h12 = hm[1], h13 = hm[2], h21 = hm[3], h22 = hm[4], h23 = hm[5],
+              h31 = hm[6], h32 = hm[7], h33 = hm[8];
+  c.tx = hm[9]; c.ty = hm[10]; c.tz = hm[11];
+  if (k < 4) {
+    const float u = (k & 1) ? u_hi : u_lo, r = (k & 2) ? r_hi : r_lo;
+    c.X0 = h11 * u + h12 * r + h13;
+    c.Y0 = h21 * u + h22 * r + h23;
+    c.Z0 = h31 * u + h32 * r + h33;
+  } else {
+    const float x00 = h11 * u_lo + h12 * v_lo + h13, x10 = h11 * u_hi + h12 * v_lo + h13;
+    const float x01 = h11 * u_lo + h12 * v_hi + h13, x11 = h11 * u_hi + h12 * v_hi + h13;
+    const float xmin = fminf(fminf(x00, x10), fminf(x01, x11));
+    const float xmax = fmaxf(fmaxf(x00, x10), fmaxf(x01, x11));
+    const float x = (k & 1) ? xmax : xmin;
+    if (k < 6) {   // term 4: (x + h11, x + h21, x + h31)
+      c.X0 = x + h11; c.Y0 = x + h21; c.Z0 = x + h31;
+    } else {       // term 8: (x + h11 - h12, x + h11 - h22, x + h11 - h32)
+      c.X0 = x + (h11 - h12); c.Y0 = x + (h11 - h22); c.Z0 = x + (h11 - h32);
+    }
+  }
+  return c;
+}
+
+__device__ __forceinline__ uint32_t tile_inside_mask(const InsideCorner& c, float idep,
+                                                     float Wm1, float Hm1) {
+  const float X = fmaf(c.tx, idep, c.X0), Y = fmaf(c.ty, idep, c.Y0);
+  const float Z = fmaf(c.tz, idep, c.Z0);
+  const float U = X / Z, V = Y / Z;
+  const bool ok = !c.live || (Z >= kInsideW && U >= kInsideMargin && U <= Wm1 - kInsideMargin &&
+                              V >= kInsideMargin && V <= Hm1 - kInsideMargin);
+  const uint64_t bad = __ballot(!ok);
+  uint32_t m = 0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) m |= ((bad >> (8 * s)) & 0xFFull) == 0 ? (1u << s) : 0u;
+  return m;
 }
 
 // warp_sample with the sample's derivatives with respect to the warped point (X, Y, w):
@@ -134,10 +220,14 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
 
   // vector stores need D and the chunk start to be multiples of 4 (16-byte / 8-byte aligned)
   const bool vec = (D % 4 == 0) && (dper % 4 == 0);
+  const InsideCorner corner =
+      tile_inside_setup(a, (float)(strip * 62 - 1), (float)(strip * 62 + 62), (float)(y0 - 1),
+                        (float)(y0 + TY), (float)y0, (float)(y0 + TY - 1));
   for (int dc0 = dbeg; dc0 < dend; dc0 += DC) {
   const int dcn = min(DC, dend - dc0);
   for (int d = dc0; d < dc0 + dcn; ++d) {
     const float idep = a.idep_min + (float)d * a.dep_sample;
+    const uint32_t inside = tile_inside_mask(corner, idep, Wm1, Hm1);
     float acc[TY], cnt[TY];
 #pragma unroll
     for (int t = 0; t < TY; ++t) { acc[t] = 0.0f; cnt[t] = 0.0f; }
@@ -156,41 +246,55 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
       const float g4x = xu + (h11 + tx), g4y = xu + (h21 + ty), g4z = xu + (h31 + tz);
       const float g8x = xu + (h11 - h12 + tx), g8y = xu + (h11 - h22 + ty),
                   g8z = xu + (h11 - h32 + tz);
-      // window over centre rows r-2, r-1: |I-S| (A, row r-1 only), its left-neighbour copy (B),
-      // up terms (Up)
-      float A1 = 0.0f, B0 = 0.0f, B1 = 0.0f, Up0 = 0.0f, Up1 = 0.0f;
+      // one pass over the centre rows; INSIDE: every warp of the tile is valid (no tests)
+      auto pass = [&](auto inside_tag) {
+        constexpr bool INSIDE = decltype(inside_tag)::value;
+        auto sample = [&](float X, float Y, float w) {
+          return INSIDE ? warp_sample_inside(q, (uint32_t)W, X, Y, w)
+                        : warp_sample(q, (uint32_t)W, Wm1, Hm1, X, Y, w);
+        };
+        // window over centre rows r-2, r-1: |I-S| (A, row r-1 only), its left-neighbour copy
+        // (B), up terms (Up)
+        float A1 = 0.0f, B0 = 0.0f, B1 = 0.0f, Up0 = 0.0f, Up1 = 0.0f;
 #pragma unroll
-      for (int k = 0; k < TY + 2; ++k) {   // centre row r = y0 - 1 + k
-        const float fr = (float)(y0 - 1 + k);
-        const float S = warp_sample(q, (uint32_t)W, Wm1, Hm1, fmaf(h12, fr, Xu),
-                                    fmaf(h22, fr, Yu), fmaf(h32, fr, Zu));
-        const float A2 = fabsf(I[k + 2] - S);     // |I(u,r) - S(u,r)|
-        const float Up2 = fabsf(I[k + 4] - S);    // |I(u,r+2) - S(u,r)|
-        const float B2 = wave_shr1z(A2);          // the same at (u-1, r)
-        if (k >= 2) {
-          const int t = k - 2;                    // output row v = r - 1
-          const float fv = (float)(y0 + t);
-          const float Dn = fabsf(I[k] - S);       // |I(u,r-2) - S(u,r)| = term 2 of row v
-          const float G4 = fabsf(IR[t + 1] - warp_sample(q, (uint32_t)W, Wm1, Hm1,
-                                                         fmaf(h12, fv, g4x), fmaf(h12, fv, g4y),
-                                                         fmaf(h12, fv, g4z)));
-          const float G8 = fabsf(IR[t] - warp_sample(q, (uint32_t)W, Wm1, Hm1,
-                                                     fmaf(h12, fv, g8x), fmaf(h12, fv, g8y),
-                                                     fmaf(h12, fv, g8z)));
-          float T = A1 + B0;                      // terms 0, 5
-          T += B1;                                // term 3
-          T += wave_shl1z(A2);                    // term 6
-          T += B2;                                // term 7
-          T += Up0 + Dn;                          // terms 1, 2
-          T += G4 + G8;                           // terms 4, 8
-          const bool ok = T < INF;
-          acc[t] += ok ? T : 0.0f;
-          cnt[t] += ok ? 1.0f : 0.0f;
+        for (int k = 0; k < TY + 2; ++k) {   // centre row r = y0 - 1 + k
+          const float fr = (float)(y0 - 1 + k);
+          const float S = sample(fmaf(h12, fr, Xu), fmaf(h22, fr, Yu), fmaf(h32, fr, Zu));
+          const float A2 = fabsf(I[k + 2] - S);     // |I(u,r) - S(u,r)|
+          const float Up2 = fabsf(I[k + 4] - S);    // |I(u,r+2) - S(u,r)|
+          const float B2 = wave_shr1z(A2);          // the same at (u-1, r)
+          if (k >= 2) {
+            const int t = k - 2;                    // output row v = r - 1
+            const float fv = (float)(y0 + t);
+            const float Dn = fabsf(I[k] - S);       // |I(u,r-2) - S(u,r)| = term 2 of row v
+            const float G4 = fabsf(IR[t + 1] - sample(fmaf(h12, fv, g4x), fmaf(h12, fv, g4y),
+                                                      fmaf(h12, fv, g4z)));
+            const float G8 = fabsf(IR[t] - sample(fmaf(h12, fv, g8x), fmaf(h12, fv, g8y),
+                                                  fmaf(h12, fv, g8z)));
+            float T = A1 + B0;                      // terms 0, 5
+            T += B1;                                // term 3
+            T += wave_shl1z(A2);                    // term 6
+            T += B2;                                // term 7
+            T += Up0 + Dn;                          // terms 1, 2
+            T += G4 + G8;                           // terms 4, 8
+            if (INSIDE) {
+              acc[t] += T;
+              cnt[t] += 1.0f;
+            } else {
+              const bool ok = T < INF;
+              acc[t] += ok ? T : 0.0f;
+              cnt[t] += ok ? 1.0f : 0.0f;
+            }
+          }
+          A1 = A2;
+          B0 = B1; B1 = B2;
+          Up0 = Up1; Up1 = Up2;
         }
-        A1 = A2;
-        B0 = B1; B1 = B2;
-        Up0 = Up1; Up1 = Up2;
-      }
+      };
+      if ((inside >> s) & 1u)
+        pass(std::true_type{});
+      else
+        pass(std::false_type{});
     }
 #pragma unroll
     for (int t = 0; t < TY; ++t) {   // mean over sources of (sum of 9 terms)/9, or -1
