@@ -55,8 +55,8 @@ struct pm_batch {
   std::vector<pm::KfDesc> host;
   pm::KfDesc* d_kfs = nullptr;
   int* d_pool = nullptr;
-  int cost_first_off[2] = {0, 0};   // tile rows 2, 4
-  int cost_total[2] = {0, 0};
+  int cost_first_off[3] = {0, 0, 0};   // tile rows 2, 4, 8
+  int cost_total[3] = {0, 0, 0};
   std::vector<pm::SgmClass> sgm;
   std::vector<pm::FilterClass> filt;
   pm_batch_tuning tune{4, 8, 16, 0, 0};
@@ -74,14 +74,15 @@ int grid_for(int total, int blocks_per_cu, int cus) {
 }
 
 bool valid_tuning(const pm_batch_tuning& t) {
-  return (t.cost_tile_rows == 2 || t.cost_tile_rows == 4) &&
+  return (t.cost_tile_rows == 2 || t.cost_tile_rows == 4 ||
+          (t.cost_tile_rows == 8 && t.cost_stage_planes == 8)) &&
          (t.cost_stage_planes == 8 || t.cost_stage_planes == 16) &&
          (t.sgm_prefetch == 8 || t.sgm_prefetch == 16) && t.cost_blocks_per_cu >= 0 &&
          t.sgm_blocks_per_cu >= 0 && t.cost_blocks_per_cu <= 64 && t.sgm_blocks_per_cu <= 64;
 }
 
 pm_status run_cost(pm_batch* b, hipStream_t st) {
-  const int ti = b->tune.cost_tile_rows == 2 ? 0 : 1;
+  const int ti = b->tune.cost_tile_rows == 2 ? 0 : (b->tune.cost_tile_rows == 4 ? 1 : 2);
   const int total = b->cost_total[ti];
   if (total == 0) return PM_OK;
   if (!launch_cost_batch(b->ct, b->tune.cost_tile_rows, b->tune.cost_stage_planes, b->d_kfs,
@@ -230,12 +231,12 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     pool.insert(pool.end(), v.begin(), v.end());
     return off;
   };
-  for (int ti = 0; ti < 2; ++ti) {
+  for (int ti = 0; ti < 3; ++ti) {
     std::vector<int> first(count);
     int acc = 0;
     for (int k = 0; k < count; ++k) {
       first[k] = acc;
-      acc += cost_items(b->host[k].c.W, b->host[k].c.H, ti == 0 ? 2 : 4);
+      acc += cost_items(b->host[k].c.W, b->host[k].c.H, 2 << ti);
     }
     b->cost_first_off[ti] = put(first);
     b->cost_total[ti] = acc;
@@ -445,11 +446,12 @@ extern "C" pm_status pm_batch_autotune(pm_batch* b, pm_stream stream, int32_t re
   };
   pm_batch_tuning cur = b->tune, win = b->tune;
   float best_ms = 1e30f;
-  const int rows[2] = {2, 4}, planes[2] = {8, 16}, bpc[4] = {0, 4, 8, 16};
+  const int rows[3] = {2, 4, 8}, planes[2] = {8, 16}, bpc[4] = {0, 4, 8, 16};
   for (int r : rows)
     for (int p : planes)
       for (int g : bpc) {
         if (status != PM_OK) break;
+        if (r == 8 && p == 16) continue;   // 128 KB of LDS per block: not built
         b->tune = cur;
         b->tune.cost_tile_rows = r;
         b->tune.cost_stage_planes = p;

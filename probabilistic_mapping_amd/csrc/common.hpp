@@ -21,6 +21,10 @@ constexpr int kWave = 64;
 // (sgm.cu:119-131), 4-7 the diagonal extension.
 constexpr int kPathDx[8] = {1, -1, 0, 0, 1, -1, 1, -1};
 constexpr int kPathDy[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+// The same tables packed two bits per path (value + 1), for device code indexing them with a
+// run-time path (a local array would live in scratch memory).
+__host__ __device__ inline int path_dx(int p) { return ((0x2252 >> (2 * p)) & 3) - 1; }
+__host__ __device__ inline int path_dy(int p) { return ((0x8225 >> (2 * p)) & 3) - 1; }
 
 // ---- per-keyframe stage arguments -------------------------------------------------------
 // Plane-sweep cost inputs of one keyframe (passed by value to the single-keyframe kernels,

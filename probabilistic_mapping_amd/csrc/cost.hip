@@ -74,7 +74,8 @@ __global__ __launch_bounds__(256) void cost_batch_kernel(const KfDesc* __restric
 }
 
 int cost_items(int W, int H, int tile_rows) {
-  return tile_rows == 2 ? cost_tiles<2>(W, H) : cost_tiles<4>(W, H);
+  return tile_rows == 2 ? cost_tiles<2>(W, H)
+                        : (tile_rows == 4 ? cost_tiles<4>(W, H) : cost_tiles<8>(W, H));
 }
 
 template <typename CT>
@@ -88,6 +89,8 @@ bool launch_cost_batch_t(int tile_rows, int stage_planes, const KfDesc* kfs, con
     cost_batch_kernel<CT, 4, 8><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
   else if (tile_rows == 4 && stage_planes == 16)
     cost_batch_kernel<CT, 4, 16><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+  else if (tile_rows == 8 && stage_planes == 8)   // 64 KB of LDS per block
+    cost_batch_kernel<CT, 8, 8><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
   else
     return false;
   return true;

@@ -26,6 +26,11 @@
 
 namespace pm {
 
+// Steps per prefetch block: at most 16 cost values per lane per buffer, so the wider plane
+// layouts (NV = 2, 4) keep the ping-pong buffers in registers instead of spilling.
+template <int NV, int B = PM_SGM_PREFETCH>
+constexpr int sgm_block() { return B * NV > 16 ? 16 / NV : B; }
+
 template <int NV, typename CT, bool FIRST, bool FULL, int B>
 __global__ __launch_bounds__(256) void sgm_path_kernel(const SgmArgs a,
                                                        const CT* __restrict__ cost,
@@ -45,24 +50,19 @@ template <int NV, typename CT, bool FULL, int B>
 __global__ __launch_bounds__(256) void sgm_paths_kernel(const SgmArgs base, int paths,
                                                         const CT* __restrict__ cost,
                                                         float* __restrict__ lbuf) {
-  const int dxs[8] = {1, -1, 0, 0, 1, -1, 1, -1};
-  const int dys[8] = {0, 0, 1, -1, 1, -1, -1, 1};
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int g = blockIdx.x * 4 + wave;
-  const size_t V = (size_t)base.W * base.H * base.D;
-  for (int path = 0; path < paths; ++path) {
-    const int n = sgm_lines(base.W, base.H, dxs[path], dys[path]);
-    if (g < n) {
-      SgmArgs a = base;
-      a.dx = dxs[path];
-      a.dy = dys[path];
-      a.nlines = n;
-      a.lout = lbuf + (size_t)path * V;
-      sgm_scanline<NV, CT, true, FULL, B, uint8_t, false, true, true>(a, g, cost, nullptr);
-      return;
-    }
-    g -= n;
+  int g = blockIdx.x * 4 + wave, path = 0, n = 0;
+  for (; path < paths; ++path, g -= n) {
+    n = sgm_lines(base.W, base.H, path_dx(path), path_dy(path));
+    if (g < n) break;
   }
+  if (path == paths) return;
+  SgmArgs a = base;
+  a.dx = path_dx(path);
+  a.dy = path_dy(path);
+  a.nlines = n;
+  a.lout = lbuf + (size_t)path * ((size_t)base.W * base.H * base.D);
+  sgm_scanline<NV, CT, true, FULL, B, uint8_t, false, true, true>(a, g, cost, nullptr);
 }
 
 // Compact form: every scanline direction of one keyframe in a single launch, each path
@@ -72,25 +72,19 @@ template <int NV, typename CT, bool FULL, int B>
 __global__ __launch_bounds__(256) void sgm_excess_kernel(const SgmArgs base, int paths,
                                                          const CT* __restrict__ cost,
                                                          uint16_t* __restrict__ ebuf) {
-  const int dxs[8] = {1, -1, 0, 0, 1, -1, 1, -1};
-  const int dys[8] = {0, 0, 1, -1, 1, -1, -1, 1};
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int g = blockIdx.x * 4 + wave;
-  const size_t V = (size_t)base.W * base.H * base.D;
-  for (int path = 0; path < paths; ++path) {
-    const int n = sgm_lines(base.W, base.H, dxs[path], dys[path]);
-    if (g < n) {
-      SgmArgs a = base;
-      a.dx = dxs[path];
-      a.dy = dys[path];
-      a.nlines = n;
-      a.eout = ebuf + (size_t)path * V;
-      sgm_scanline<NV, CT, true, FULL, B, uint8_t, false, false, false, true>(a, g, cost,
-                                                                              nullptr);
-      return;
-    }
-    g -= n;
+  int g = blockIdx.x * 4 + wave, path = 0, n = 0;
+  for (; path < paths; ++path, g -= n) {
+    n = sgm_lines(base.W, base.H, path_dx(path), path_dy(path));
+    if (g < n) break;
   }
+  if (path == paths) return;
+  SgmArgs a = base;
+  a.dx = path_dx(path);
+  a.dy = path_dy(path);
+  a.nlines = n;
+  a.eout = ebuf + (size_t)path * ((size_t)base.W * base.H * base.D);
+  sgm_scanline<NV, CT, true, FULL, B, uint8_t, false, false, false, true>(a, g, cost, nullptr);
 }
 
 // Batched compact form: the work list of a class (same NV, FULL) is a sequence of segments,
@@ -101,15 +95,13 @@ __global__ __launch_bounds__(256) void sgm_excess_batch_kernel(const KfDesc* __r
                                                                const int* __restrict__ seg,
                                                                const int* __restrict__ first,
                                                                int nseg, int total) {
-  const int dxs[8] = {1, -1, 0, 0, 1, -1, 1, -1};
-  const int dys[8] = {0, 0, 1, -1, 1, -1, -1, 1};
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (int i = blockIdx.x; i < total; i += gridDim.x) {
     const int j = owner_of(first, nseg, i);
     const KfDesc& k = kfs[seg[2 * j]];
     const int path = seg[2 * j + 1];
     SgmArgs a;
-    a.W = k.c.W; a.H = k.c.H; a.D = k.c.D; a.dx = dxs[path]; a.dy = dys[path];
+    a.W = k.c.W; a.H = k.c.H; a.D = k.c.D; a.dx = path_dx(path); a.dy = path_dy(path);
     a.nlines = sgm_lines(a.W, a.H, a.dx, a.dy);
     a.ref = k.c.ref;
     a.P1n = k.P1n; a.P2n = k.P2n; a.P1s = k.P1s; a.P2s = k.P2s; a.tau_i = k.tau_i;
@@ -129,7 +121,7 @@ bool sgm_excess_batch_b(int nv, bool full, const KfDesc* kfs, const int* seg, co
   else if (nv == 1) sgm_excess_batch_kernel<1, CT, false, B><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
   else if (nv == 2 && full) sgm_excess_batch_kernel<2, CT, true, B><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
   else if (nv == 2) sgm_excess_batch_kernel<2, CT, false, B><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
-  else if (nv == 4) sgm_excess_batch_kernel<4, CT, false, B><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
+  else if (nv == 4) sgm_excess_batch_kernel<4, CT, false, sgm_block<4, B>()><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total);
   else return false;
   return true;
 }
@@ -193,7 +185,7 @@ bool sgm_batch_path_b(int nv, bool full, const KfDesc* kfs, const int* kf, const
   else if (nv == 1) sgm_batch_path_t<1, CT, false, B>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
   else if (nv == 2 && full) sgm_batch_path_t<2, CT, true, B>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
   else if (nv == 2) sgm_batch_path_t<2, CT, false, B>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
-  else if (nv == 4) sgm_batch_path_t<4, CT, false, B>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
+  else if (nv == 4) sgm_batch_path_t<4, CT, false, sgm_block<4, B>()>(kfs, kf, first, n, total, grid, first_path, dx, dy, st);
   else return false;
   return true;
 }
@@ -227,7 +219,7 @@ bool launch_sgm_batch_path(pm_cost_type ct, int prefetch, int nv, bool full, con
 template <int NV, typename CT, bool FULL>
 pm_status launch_paths(const SgmArgs& base, int paths, const CT* cost, float* agg,
                        hipStream_t st) {
-  constexpr int B = PM_SGM_PREFETCH;
+  constexpr int B = sgm_block<NV>();
   for (int path = 0; path < paths; ++path) {
     SgmArgs a = base;
     a.dx = kPathDx[path];
@@ -256,7 +248,7 @@ pm_status launch_sgm(const SgmArgs& a, int paths, const CT* cost, float* agg, hi
 template <int NV, typename CT, bool FULL>
 void launch_paths_concurrent(const SgmArgs& a, int paths, const CT* cost, float* lbuf,
                              hipStream_t st) {
-  constexpr int B = PM_SGM_PREFETCH;
+  constexpr int B = sgm_block<NV>();
   long lines = 0;
   for (int path = 0; path < paths; ++path)
     lines += sgm_lines(a.W, a.H, kPathDx[path], kPathDy[path]);
@@ -278,7 +270,7 @@ pm_status launch_sgm_concurrent(const SgmArgs& a, int paths, const CT* cost, flo
 template <int NV, typename CT, bool FULL>
 void launch_excess_t(const SgmArgs& a, int paths, const CT* cost, uint16_t* ebuf,
                      hipStream_t st) {
-  constexpr int B = PM_SGM_PREFETCH;
+  constexpr int B = sgm_block<NV>();
   long lines = 0;
   for (int path = 0; path < paths; ++path)
     lines += sgm_lines(a.W, a.H, kPathDx[path], kPathDy[path]);

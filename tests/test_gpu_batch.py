@@ -72,8 +72,8 @@ def test_batch_equals_single_keyframe(uneven, cost_type, paths, storage):
     torch.cuda.synchronize()
     _assert_equal(batch, expect)
     # every launch shape gives the same bits
-    for rows in (2, 4):
-        for planes in (8, 16):
+    for rows in (2, 4, 8):
+        for planes in ((8,) if rows == 8 else (8, 16)):
             for pref in (8, 16):
                 for bpc in (0, 4):
                     for o in batch.outputs:
@@ -120,6 +120,8 @@ def test_batch_rejects_bad_input(uneven):
     b = pipeline.Batch(uneven[:1], "f32", KNOBS, 4)
     with pytest.raises(C.PmError):
         b.tuning = dict(cost_tile_rows=3)
+    with pytest.raises(C.PmError):
+        b.tuning = dict(cost_tile_rows=8, cost_stage_planes=16)
     with pytest.raises(C.PmError):
         b.launch()   # nothing captured
 
