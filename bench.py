@@ -162,8 +162,7 @@ def measure_plain(args, torch, s, cost_type, storage):
         for i in range(nkf):
             e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
             e[0].record(stream)
-            pl.calc_cost(seq.images[i], [seq.quads[j] for j in seq.sources[i]], seq.homs[i],
-                         seq.idep_min, seq.dep_sample, seq.D, seq.cost_type, out=seq.cost)
+            seq.cost_stage(i)
             e[1].record(stream)
             seq.sgm_stage(i, seq.dep_raw if seq.median else seq.dep)
             e[2].record(stream)

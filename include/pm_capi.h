@@ -107,7 +107,7 @@ pm_status pm_make_quad(const uint8_t* image, uint64_t* quad, int32_t width, int3
  * idep of plane d = idep_min + d * dep_sample (the reference has idep_min = 0).
  * src_quads and homs are HOST arrays of num_sources entries (the quads are device buffers).
  * cost: device [H][W][D] of cost_type; -1 where no source saw the (pixel, plane).
- * width * height <= 2^29 (texel byte offsets are 32-bit). */
+ * width * height < 2^29 (texel byte offsets are 32-bit). */
 pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t num_sources,
                        const uint64_t* const* src_quads, const pm_homography* homs,
                        float idep_min, float dep_sample, void* cost, pm_cost_type cost_type,

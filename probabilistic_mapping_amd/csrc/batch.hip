@@ -182,7 +182,7 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     const pm_keyframe& in = kfs[k];
     const std::string who = "pm_batch_create: keyframe " + std::to_string(k) + ": ";
     if (in.width <= 0 || in.height <= 0 || in.width > 65535 || in.height > 65535 ||
-        (long)in.width * in.height > (1L << 29)) {
+        (long)in.width * in.height >= (1L << 29)) {
       delete b;
       return fail(PM_ERR_INVALID_ARGUMENT, who + "bad image size");
     }

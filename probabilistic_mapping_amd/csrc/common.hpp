@@ -16,6 +16,8 @@ constexpr float kFloatEps = 1e-5f;   // FLOAT_EPS, motion_stereo/parameters.h:6
 constexpr float kDepInf = 1000.0f;   // DEP_INF, parameters.h:28
 constexpr float kCostInf = 0.0f;     // COST_INF, parameters.h:29
 constexpr int kWave = 64;
+// Dword 3 of a raw buffer resource on gfx9 (32-bit data format; no swizzle, no index stride).
+constexpr int kBufferDword3 = 0x00020000;
 
 // Scanline directions in aggregation order: 0-3 are the reference's paths
 // (sgm.cu:119-131), 4-7 the diagonal extension.

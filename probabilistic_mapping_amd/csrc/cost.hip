@@ -133,7 +133,7 @@ extern "C" pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t nu
   clear_error();
   if (!ref || !cost) return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: null pointer");
   if (shape.width <= 0 || shape.height <= 0 || shape.width > 65535 || shape.height > 65535 ||
-      (long)shape.width * shape.height > (1L << 29))
+      (long)shape.width * shape.height >= (1L << 29))
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: bad image size");
   if (shape.planes <= 0 || shape.planes > PM_MAX_PLANES)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: planes must be in 1..256");

@@ -8,13 +8,26 @@
 
 namespace pm {
 
+// A measurement frame's quad texture ([H][W] texels of 8 bytes) read through a buffer resource:
+// wave-uniform base and size in scalar registers, a 32-bit byte offset per lane, and the
+// hardware range check (a read past W H 8 bytes returns 0 instead of touching memory).  W H < 2^29
+// is checked on the host, so the byte size fits the 32-bit record count.
+struct QuadTex {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ QuadTex(const uint64_t* q, uint32_t W, uint32_t H)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(q), 0, (int)(W * H * 8u),
+                                            kBufferDword3)) {}
+  __device__ __forceinline__ uint2 at(uint32_t byte_off) const {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_off, 0, 0);
+    return make_uint2(v[0], v[1]);
+  }
+};
+
 // Bilinear sample of the quad texture at (U, V) with the warp's validity (calc_cost.cu:85-93);
 // returns +inf for an invalid warp.  Validity is "the clamp to the image changes nothing" and
 // w >= 0, so a NaN warp (0/0) is invalid; an invalid warp still reads an in-bounds texel (the
-// clamped position), so every gather of a row can be in flight at once.  The byte offset fits
-// 32 bits (W H <= 2^29 is checked on the host), which lets the gather use the scalar base
-// address + 32-bit vector offset form.
-__device__ __forceinline__ float warp_sample(const uint64_t* __restrict__ quad, uint32_t W,
+// clamped position), so every gather of a row can be in flight at once.
+__device__ __forceinline__ float warp_sample(const QuadTex& quad, uint32_t W,
                                              float Wm1, float Hm1, float X, float Y, float w) {
   const float rw = __builtin_amdgcn_rcpf(w);
   const float U = X * rw;
@@ -24,7 +37,7 @@ __device__ __forceinline__ float warp_sample(const uint64_t* __restrict__ quad, 
   const bool valid = (w >= 0.0f) & (Uc == U) & (Vc == V);
   // non-negative, so truncation is floor; W, y <= 65535 fit the 24-bit multiply
   const uint32_t off = __umul24((uint32_t)Vc, W) * 8u + (uint32_t)Uc * 8u;
-  const uint2 t = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(quad) + off);
+  const uint2 t = quad.at(off);
   const float ax = __builtin_amdgcn_fractf(Uc), ay = __builtin_amdgcn_fractf(Vc);
   const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
   const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
@@ -35,13 +48,13 @@ __device__ __forceinline__ float warp_sample(const uint64_t* __restrict__ quad, 
 
 // warp_sample for a warp known to be valid and inside the image (see tile_inside_mask): the
 // same arithmetic without the clamp and the validity test, so it returns the same value.
-__device__ __forceinline__ float warp_sample_inside(const uint64_t* __restrict__ quad, uint32_t W,
+__device__ __forceinline__ float warp_sample_inside(const QuadTex& quad, uint32_t W,
                                                     float X, float Y, float w) {
   const float rw = __builtin_amdgcn_rcpf(w);
   const float U = X * rw;
   const float V = Y * rw;
   const uint32_t off = __umul24((uint32_t)V, W) * 8u + (uint32_t)U * 8u;
-  const uint2 t = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(quad) + off);
+  const uint2 t = quad.at(off);
   const float ax = __builtin_amdgcn_fractf(U), ay = __builtin_amdgcn_fractf(V);
   const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
   const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
@@ -125,7 +138,7 @@ __device__ __forceinline__ uint32_t tile_inside_mask(const InsideCorner& c, floa
 struct SampleGrad {
   float s, a, b, c;
 };
-__device__ __forceinline__ SampleGrad warp_sample_grad(const uint64_t* __restrict__ quad,
+__device__ __forceinline__ SampleGrad warp_sample_grad(const QuadTex& quad,
                                                        uint32_t W, float Wm1, float Hm1,
                                                        float X, float Y, float w) {
   const float rw = __builtin_amdgcn_rcpf(w);
@@ -135,7 +148,7 @@ __device__ __forceinline__ SampleGrad warp_sample_grad(const uint64_t* __restric
   const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
   const bool valid = (w >= 0.0f) & (Uc == U) & (Vc == V);
   const uint32_t off = __umul24((uint32_t)Vc, W) * 8u + (uint32_t)Uc * 8u;
-  const uint2 t = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(quad) + off);
+  const uint2 t = quad.at(off);
   const float ax = __builtin_amdgcn_fractf(Uc), ay = __builtin_amdgcn_fractf(Vc);
   const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
   const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
@@ -236,7 +249,7 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
       const float h11 = hm[0], h12 = hm[1], h13 = hm[2], h21 = hm[3], h22 = hm[4],
                   h23 = hm[5], h31 = hm[6], h32 = hm[7], h33 = hm[8];
       const float tx = hm[9] * idep, ty = hm[10] * idep, tz = hm[11] * idep;
-      const uint64_t* q = a.quad[s];
+      const QuadTex q(a.quad[s], (uint32_t)W, (uint32_t)H);
       // column parts of H (u, r, 1) + t idep
       const float Xu = h11 * fu + h13 + tx;
       const float Yu = h21 * fu + h23 + ty;
@@ -260,23 +273,24 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
         for (int k = 0; k < TY + 2; ++k) {   // centre row r = y0 - 1 + k
           const float fr = (float)(y0 - 1 + k);
           const float S = sample(fmaf(h12, fr, Xu), fmaf(h22, fr, Yu), fmaf(h32, fr, Zu));
-          const float A2 = fabsf(I[k + 2] - S);     // |I(u,r) - S(u,r)|
-          const float Up2 = fabsf(I[k + 4] - S);    // |I(u,r+2) - S(u,r)|
+          // signed differences; the |.| is taken where they are summed (a source modifier)
+          const float A2 = I[k + 2] - S;            // I(u,r) - S(u,r)
+          const float Up2 = I[k + 4] - S;           // I(u,r+2) - S(u,r)
           const float B2 = wave_shr1z(A2);          // the same at (u-1, r)
           if (k >= 2) {
             const int t = k - 2;                    // output row v = r - 1
             const float fv = (float)(y0 + t);
-            const float Dn = fabsf(I[k] - S);       // |I(u,r-2) - S(u,r)| = term 2 of row v
-            const float G4 = fabsf(IR[t + 1] - sample(fmaf(h12, fv, g4x), fmaf(h12, fv, g4y),
-                                                      fmaf(h12, fv, g4z)));
-            const float G8 = fabsf(IR[t] - sample(fmaf(h12, fv, g8x), fmaf(h12, fv, g8y),
-                                                  fmaf(h12, fv, g8z)));
-            float T = A1 + B0;                      // terms 0, 5
-            T += B1;                                // term 3
-            T += wave_shl1z(A2);                    // term 6
-            T += B2;                                // term 7
-            T += Up0 + Dn;                          // terms 1, 2
-            T += G4 + G8;                           // terms 4, 8
+            const float Dn = I[k] - S;              // I(u,r-2) - S(u,r): term 2 of row v
+            const float G4 = IR[t + 1] - sample(fmaf(h12, fv, g4x), fmaf(h12, fv, g4y),
+                                                fmaf(h12, fv, g4z));
+            const float G8 = IR[t] - sample(fmaf(h12, fv, g8x), fmaf(h12, fv, g8y),
+                                            fmaf(h12, fv, g8z));
+            float T = fabsf(A1) + fabsf(B0);        // terms 0, 5
+            T += fabsf(B1);                         // term 3
+            T += fabsf(wave_shl1z(A2));             // term 6
+            T += fabsf(B2);                         // term 7
+            T += fabsf(Up0) + fabsf(Dn);            // terms 1, 2
+            T += fabsf(G4) + fabsf(G8);             // terms 4, 8
             if (INSIDE) {
               acc[t] += T;
               cnt[t] += 1.0f;

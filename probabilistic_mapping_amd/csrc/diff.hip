@@ -458,7 +458,7 @@ __global__ __launch_bounds__(256) void cost_adjoint_kernel(const CostAdjArgs A) 
       const float* hm = a.hm[s];
       const float h11 = hm[0], h12 = hm[1], h13 = hm[2], h21 = hm[3], h22 = hm[4],
                   h23 = hm[5], h31 = hm[6], h32 = hm[7], h33 = hm[8];
-      const uint64_t* q = a.quad[s];
+      const QuadTex q(a.quad[s], (uint32_t)W, (uint32_t)H);
       // per-lane sums: centre samples (X, Y, w and their products with the row), garbled
       // samples, translation
       float cX = 0, cXr = 0, cY = 0, cYr = 0, cW = 0, cWr = 0;
@@ -747,7 +747,7 @@ extern "C" pm_status pm_diff_create(pm_shape shape, int32_t num_sources, pm_sgm_
   if (!out) return fail(PM_ERR_INVALID_ARGUMENT, "pm_diff_create: null out");
   *out = nullptr;
   if (shape.width <= 0 || shape.height <= 0 || shape.width > 65535 || shape.height > 65535 ||
-      (long)shape.width * shape.height > (1L << 29))
+      (long)shape.width * shape.height >= (1L << 29))
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_diff_create: bad image size");
   if (shape.planes < 2 || shape.planes > PM_MAX_PLANES)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_diff_create: planes must be in 2..256");

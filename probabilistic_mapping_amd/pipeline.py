@@ -280,12 +280,30 @@ class KeyframeSequence:
         self.first = True
         self.pre = 0
         self.prev_pose = None
+        self.prev_index = None
+
+    def _cost_args(self, i: int):
+        """ctypes arguments of keyframe i's pm_calc_cost (built once; the stream is added per
+        call)."""
+        if not hasattr(self, "_cost_cache"):
+            self._cost_cache = {}
+        if i not in self._cost_cache or self._cost_cache[i][-1] is not self.cost:
+            srcs = self.sources[i]
+            qarr = (ctypes.c_void_p * max(len(srcs), 1))(*[self.quads[j].data_ptr() for j in srcs])
+            code = COST_TYPES[self.cost_type][0]
+            self._cost_cache[i] = (C.Shape(self.W, self.H, self.D), _ptr(self.images[i]),
+                                   len(srcs), qarr, _hom_array(self.homs[i]),
+                                   ctypes.c_float(self.idep_min), ctypes.c_float(self.dep_sample),
+                                   _ptr(self.cost), code, self.cost)
+        return self._cost_cache[i][:-1]
+
+    def cost_stage(self, i: int) -> None:
+        """Plane-sweep cost of keyframe i over its sources into ``self.cost``."""
+        C.check(C.load().pm_calc_cost(*self._cost_args(i), _stream()))
 
     def depth_stage(self, i: int) -> torch.Tensor:
         """cost -> SGM + filter (-> median) for keyframe i; returns the filtered depth."""
-        srcs = self.sources[i]
-        calc_cost(self.images[i], [self.quads[j] for j in srcs], self.homs[i], self.idep_min,
-                  self.dep_sample, self.D, self.cost_type, out=self.cost)
+        self.cost_stage(i)
         out = self.dep_raw if self.median else self.dep
         self.sgm_stage(i, out)
         if self.median:
@@ -294,9 +312,22 @@ class KeyframeSequence:
 
     def sgm_stage(self, i: int, out: torch.Tensor) -> None:
         """SGM (every path at once) + filter of keyframe i's cost volume into ``out``."""
-        fn = sgm_filter_excess if self.sgm_storage == "compact" else sgm_filter_paths
-        fn(self.images[i], self.cost, self.knobs, self.paths, self.idep_min, self.dep_sample,
-           workspace=self.sgm_ws, agg=self.agg, depth=out)
+        if not hasattr(self, "_sgm_cache"):
+            self._sgm_cache = {}
+        key = (i, out.data_ptr(), self.cost.data_ptr())
+        if key not in self._sgm_cache:
+            k = self.knobs
+            p = C.SgmParams(k["pi1"], k["pi2"], k["tau_so"], k["sgm_q1"], k["sgm_q2"], self.paths)
+            elt = 2 if self.sgm_storage == "compact" else 4
+            self._sgm_cache[key] = (
+                C.Shape(self.W, self.H, self.D), _ptr(self.images[i]), _ptr(self.cost),
+                _cost_code(self.cost), p, ctypes.c_float(k["var_scale"]),
+                ctypes.c_float(self.idep_min), ctypes.c_float(self.dep_sample),
+                _ptr(self.sgm_ws), ctypes.c_uint64(self.sgm_ws.numel() * elt),
+                _ptr(self.agg) if self.agg is not None else None, _ptr(out))
+        fn = (C.load().pm_sgm_filter_excess if self.sgm_storage == "compact"
+              else C.load().pm_sgm_filter_paths)
+        C.check(fn(*self._sgm_cache[key], _stream()))
 
     def fusion_stage(self, i: int) -> None:
         """One outputFusion step on ``self.dep`` (stereo_mapper.cpp:128-243)."""
@@ -307,13 +338,25 @@ class KeyframeSequence:
             for s in self.states:
                 s["alpha"].zero_()
             self.prev_pose = (R, t)
+            self.prev_index = i
             self.fused.zero_()
             return
         self.fused.copy_(self.dep)
-        r, tt = pose_p2c(R, t, *self.prev_pose)
-        depth_fuse(self.fused, self.zbuf, self.states[self.pre], self.states[self.pre ^ 1], r,
-                   tt, self.cam, table=self.table)
+        # the pm_depth_fuse arguments of (keyframe, previous keyframe, state parity), built once
+        key = (i, self.prev_index, self.pre)
+        if not hasattr(self, "_fuse_cache"):
+            self._fuse_cache = {}
+        if key not in self._fuse_cache:
+            r, tt = pose_p2c(R, t, *self.prev_pose)
+            pre, cur = self.states[self.pre], self.states[self.pre ^ 1]
+            rr = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(r, np.float32).reshape(9)])
+            ttc = (ctypes.c_float * 3)(*[float(v) for v in np.asarray(tt, np.float32).reshape(3)])
+            self._fuse_cache[key] = (self.W, self.H, C.Camera(*[float(c) for c in self.cam]), rr,
+                                     ttc, _ptr(self.fused), _ptr(self.zbuf), _ptr(self.table),
+                                     _state(pre), _state(cur))
+        C.check(C.load().pm_depth_fuse(*self._fuse_cache[key], _stream()))
         self.prev_pose = (R, t)
+        self.prev_index = i
         self.pre ^= 1
 
     def run_keyframe(self, i: int) -> None:
