@@ -144,7 +144,8 @@ pm_status pm_sgm_filter_paths(pm_shape shape, const uint8_t* ref, const void* co
  * fixed point with step max(pi2, pi2 * sgm_q2) / 65534 in the workspace
  * (pm_sgm_excess_workspace_size bytes: paths * W * H * D uint16, 16-byte aligned).  One pass
  * then rebuilds agg = paths * C + step * sum of the levels (exact integer sum, one fma; 0 at
- * invalid pixels, sgm.cu:101-104), writes it to agg (may be NULL) and filters it into depth.
+ * invalid pixels, those with a negative cost on any plane, sgm.cu:33-38, 101-104), writes it
+ * to agg (may be NULL) and filters it into depth.
  * |agg - sum_r L_r| <= paths * step / 2 + FP32 rounding. */
 pm_status pm_sgm_excess_workspace_size(pm_shape shape, int32_t paths, uint64_t* bytes);
 pm_status pm_sgm_filter_excess(pm_shape shape, const uint8_t* ref, const void* cost,

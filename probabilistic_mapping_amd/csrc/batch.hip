@@ -187,7 +187,8 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
       return fail(PM_ERR_INVALID_ARGUMENT, who + "bad image size");
     }
     if (in.planes < 1 || in.planes > PM_MAX_PLANES || in.num_sources < 0 ||
-        in.num_sources > PM_MAX_SOURCES) {
+        in.num_sources > PM_MAX_SOURCES ||
+        (uint64_t)in.width * in.height * in.planes * 4 >= (1ull << 32)) {
       delete b;
       return fail(PM_ERR_INVALID_ARGUMENT, who + "planes or num_sources out of range");
     }

@@ -54,10 +54,8 @@ struct SgmArgs {
 };
 
 // Compact per-path storage of the SGM excess E = L - C' (0 <= E <= P2 on every step): 16-bit
-// fixed point with step max(P2n, P2s) / kExcessMaxQ; kExcessInvalid marks an invalid pixel
-// (every plane, every path).
+// fixed point with step max(P2n, P2s) / kExcessMaxQ.
 constexpr uint32_t kExcessMaxQ = 0xFFFEu;
-constexpr uint32_t kExcessInvalid = 0xFFFFu;
 __host__ __device__ inline float excess_step(float P2n, float P2s) {
   const float m = P2n > P2s ? P2n : P2s;
   return m > 0.0f ? m / (float)kExcessMaxQ : 0.0f;
@@ -131,7 +129,74 @@ pm_status check_launch(const char* what);
 bool homography_fp64(const double K[9], const double R_l[9], const double T_l[3],
                      const double R_r[9], const double T_r[3], double H[9], double t[3]);
 
+// Global-memory views of device pointers that arrive as generic pointers (e.g. read from a
+// descriptor table): loads and stores through them use the scalar-base + 32-bit vector offset
+// global instructions instead of flat ones with 64-bit vector addresses.
+template <typename T>
+using gptr = T __attribute__((address_space(1)))*;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+  return (gptr<T>)p;
+}
+
+// A device array read and written through a raw buffer resource: the base and size live in
+// scalar registers, an access is a 32-bit per-lane byte offset plus a 32-bit wave-uniform one
+// (no per-lane 64-bit address arithmetic), and the hardware range check turns an access past
+// `bytes` into a zero read / dropped write.  Arrays must be smaller than 4 GiB.
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Buf(const void* p, uint32_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes,
+                                            kBufferDword3)) {}
+  __device__ __forceinline__ float f32(uint32_t voff, uint32_t soff) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+  }
+  __device__ __forceinline__ uint32_t u16(uint32_t voff, uint32_t soff) const {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+  }
+  __device__ __forceinline__ void put_f32(float v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+  }
+  __device__ __forceinline__ void put_u16(uint32_t v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)voff, (int)soff, 0);
+  }
+};
+// A cost element through a Buf (byte offsets).
+template <typename CT>
+__device__ __forceinline__ float buf_cost(const Buf& b, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float buf_cost<float>(const Buf& b, uint32_t voff, uint32_t soff) {
+  return b.f32(voff, soff);
+}
+template <>
+__device__ __forceinline__ float buf_cost<__half>(const Buf& b, uint32_t voff, uint32_t soff) {
+  return __half2float(__ushort_as_half((unsigned short)b.u16(voff, soff)));
+}
+template <>
+__device__ __forceinline__ float buf_cost<__hip_bfloat16>(const Buf& b, uint32_t voff,
+                                                         uint32_t soff) {
+  return __uint_as_float(b.u16(voff, soff) << 16);
+}
+
+// A wave-uniform float read from memory, moved to a scalar register.
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // ---- cost storage ---------------------------------------------------------------------
+template <typename CT>
+__device__ __forceinline__ float load_cost(gptr<const CT> p);
+template <>
+__device__ __forceinline__ float load_cost<float>(gptr<const float> p) { return *p; }
+template <>
+__device__ __forceinline__ float load_cost<__half>(gptr<const __half> p) {
+  return __half2float(__ushort_as_half(*(gptr<const unsigned short>)p));
+}
+template <>
+__device__ __forceinline__ float load_cost<__hip_bfloat16>(gptr<const __hip_bfloat16> p) {
+  return __uint_as_float((uint32_t)(*(gptr<const unsigned short>)p) << 16);
+}
+
 template <typename CT>
 __device__ __forceinline__ float load_cost(const CT* p);
 template <>
@@ -186,17 +251,39 @@ __device__ __forceinline__ float wave_shl1z(float v) {
                                                                0xF, 0xF, true));
 }
 
-// Minimum over the 64 lanes of a non-negative float, returned wave-uniform.
+// Minimum over the 64 lanes of a non-negative float, in every lane.  Inside each row of 16
+// lanes: four DPP stages (quad_perm [1,0,3,2], [2,3,0,1], row_ror:4, row_ror:8) that fold into
+// v_min_u32_dpp (every source lane exists, so bound_ctrl changes nothing); across rows: the
+// gfx950 permlane32 / permlane16 swaps exchange the row minima.  Ten VALU instructions, no
+// readlane.
+template <int CTRL>
+__device__ __forceinline__ uint32_t umin_dpp(uint32_t u) {
+  return min(u, (uint32_t)__builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, true));
+}
 __device__ __forceinline__ float wave_min_nonneg(float v) {
   uint32_t u = __float_as_uint(v);
-  u = min(u, dpp_u32<0xB1>(u, u));    // quad_perm [1,0,3,2]
-  u = min(u, dpp_u32<0x4E>(u, u));    // quad_perm [2,3,0,1]
-  u = min(u, dpp_u32<0x124>(u, u));   // row_ror:4
-  u = min(u, dpp_u32<0x128>(u, u));   // row_ror:8
-  const uint32_t a = __builtin_amdgcn_readlane(u, 0), b = __builtin_amdgcn_readlane(u, 16);
-  const uint32_t c = __builtin_amdgcn_readlane(u, 32), d = __builtin_amdgcn_readlane(u, 48);
-  return __uint_as_float(min(min(a, b), min(c, d)));
+  u = umin_dpp<0xB1>(u);
+  u = umin_dpp<0x4E>(u);
+  u = umin_dpp<0x124>(u);
+  u = umin_dpp<0x128>(u);
+  const auto h = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+  u = min(h[0], h[1]);
+  const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+  return __uint_as_float(min(r[0], r[1]));
 }
+
+// min(lane i-1's `lo`, lane i+1's `hi`) for the neighbouring planes of the SGM recurrence; a
+// missing neighbour (lane 0's left, lane 63's right) is 0xFFFFFFFF, above the bit pattern of
+// every non-negative float, so it never wins.
+__device__ __forceinline__ uint32_t shr1_max(float lo) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)__float_as_uint(lo), 0x138, 0xF, 0xF,
+                                               false);
+}
+__device__ __forceinline__ uint32_t shl1_max(float hi) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)__float_as_uint(hi), 0x130, 0xF, 0xF,
+                                               false);
+}
+
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block i runs on XCD i % 8).  This
 // remaps the launch index so that each XCD receives one contiguous range of logical blocks,

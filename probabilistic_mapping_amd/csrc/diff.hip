@@ -751,6 +751,9 @@ extern "C" pm_status pm_diff_create(pm_shape shape, int32_t num_sources, pm_sgm_
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_diff_create: bad image size");
   if (shape.planes < 2 || shape.planes > PM_MAX_PLANES)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_diff_create: planes must be in 2..256");
+  if ((uint64_t)shape.width * shape.height * shape.planes * 4 >= (1ull << 32))
+    return fail(PM_ERR_INVALID_ARGUMENT,
+                "pm_diff_create: W H D must be below 2^30 (32-bit offsets)");
   if (num_sources < 1 || num_sources > PM_MAX_SOURCES)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_diff_create: num_sources must be in 1..16");
   if (sgm.paths != 4 && sgm.paths != 8)

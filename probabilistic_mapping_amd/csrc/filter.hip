@@ -158,46 +158,47 @@ __device__ __forceinline__ void load_cost4<__hip_bfloat16>(const __hip_bfloat16*
 }
 
 // Compact form: the aggregate is rebuilt from the cost and the per-path excess volumes,
-//   agg(p,d) = P C(p,d) + step * sum_r q_r(p,d)   (sum_r L_r = sum_r (C' + E_r), sgm.cu:98),
-// and 0 at invalid pixels (marked kExcessInvalid in path 0's volume; sgm.cu:101-104).  The sum
-// of the 16-bit levels is exact in 32-bit integers, so the only rounding beyond the levels
-// themselves is one fma.
+//   agg(p,d) = P C(p,d) + step * sum_r q_r(p,d)   (sum_r L_r = sum_r (C' + E_r), sgm.cu:98).
+// The sum of the 16-bit levels is exact in 32-bit integers, so the only rounding beyond the
+// levels themselves is one fma.  Returns whether any of the lane's planes has a negative cost
+// (the pixel is then invalid and its aggregate 0, sgm.cu:33-38, 101-104).
 template <typename CT>
-__device__ __forceinline__ void excess_agg4(const CT* __restrict__ cost,
+__device__ __forceinline__ bool excess_agg4(const CT* __restrict__ cost,
                                             const uint16_t* __restrict__ ebuf, long V, long e,
                                             int D, int d0, int paths, bool vec4, float step,
                                             float (&v)[4]) {
   const float INF = __uint_as_float(0x7F800000u);
+  bool neg = false;
   if (vec4 && d0 + 3 < D) {
     float c[4];
     load_cost4<CT>(cost + e, c);
     uint32_t s[4] = {0u, 0u, 0u, 0u};
-    bool bad[4];
     for (int r = 0; r < paths; ++r) {
       const uint2 u = *reinterpret_cast<const uint2*>(ebuf + (long)r * V + e);
       const uint32_t q[4] = {u.x & 0xFFFFu, u.x >> 16, u.y & 0xFFFFu, u.y >> 16};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (r == 0) bad[j] = q[j] == kExcessInvalid;
-        s[j] += q[j];
-      }
+      for (int j = 0; j < 4; ++j) s[j] += q[j];
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      v[j] = bad[j] ? kCostInf : fmaf((float)s[j], step, (float)paths * c[j]);
+    for (int j = 0; j < 4; ++j) {
+      neg = neg || c[j] < 0.0f;
+      v[j] = fmaf((float)s[j], step, (float)paths * c[j]);
+    }
   } else {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (d0 + j < D) {
         uint32_t sum = 0;
         for (int r = 0; r < paths; ++r) sum += ebuf[(long)r * V + e + j];
-        const bool bad = ebuf[e + j] == kExcessInvalid;
-        v[j] = bad ? kCostInf : fmaf((float)sum, step, (float)paths * load_cost<CT>(cost + e + j));
+        const float c = load_cost<CT>(cost + e + j);
+        neg = neg || c < 0.0f;
+        v[j] = fmaf((float)sum, step, (float)paths * c);
       } else {
         v[j] = INF;
       }
     }
   }
+  return neg;
 }
 
 template <int LP, typename CT>
@@ -211,7 +212,14 @@ __device__ __forceinline__ void combine_excess_pixels(long N, int D, int paths, 
   const long V = N * D;
   auto load = [&](long pc, bool own, int d0, float (&v)[4]) {
     const long e = pc * D + d0;
-    excess_agg4<CT>(cost, ebuf, V, e, D, d0, paths, vec4, step, v);
+    int bad = excess_agg4<CT>(cost, ebuf, V, e, D, d0, paths, vec4, step, v) ? 1 : 0;
+    // invalid pixel: a negative cost on any plane of the LP lanes holding it
+#pragma unroll
+    for (int off = LP / 2; off >= 1; off >>= 1) bad |= __shfl_xor(bad, off, 64);
+    if (bad) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = d0 + j < D ? kCostInf : v[j];
+    }
     if (own && agg) {
       if (vec4 && d0 + 3 < D) {
         *reinterpret_cast<float4*>(agg + e) = make_float4(v[0], v[1], v[2], v[3]);

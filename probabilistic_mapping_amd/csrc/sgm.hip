@@ -104,9 +104,10 @@ __global__ __launch_bounds__(256) void sgm_excess_batch_kernel(const KfDesc* __r
     a.W = k.c.W; a.H = k.c.H; a.D = k.c.D; a.dx = path_dx(path); a.dy = path_dy(path);
     a.nlines = sgm_lines(a.W, a.H, a.dx, a.dy);
     a.ref = k.c.ref;
-    a.P1n = k.P1n; a.P2n = k.P2n; a.P1s = k.P1s; a.P2s = k.P2s; a.tau_i = k.tau_i;
+    a.P1n = uniform(k.P1n); a.P2n = uniform(k.P2n); a.P1s = uniform(k.P1s);
+    a.P2s = uniform(k.P2s); a.tau_i = __builtin_amdgcn_readfirstlane(k.tau_i);
     a.eout = k.ebuf + (size_t)path * a.W * a.H * a.D;
-    a.eq_inv = k.einv;
+    a.eq_inv = uniform(k.einv);
     const int line = (i - first[j]) * 4 + wave;
     if (line < a.nlines)
       sgm_scanline<NV, CT, true, FULL, B, uint8_t, false, false, false, true>(
@@ -161,7 +162,8 @@ __global__ __launch_bounds__(256) void sgm_batch_kernel(const KfDesc* __restrict
     a.W = k.c.W; a.H = k.c.H; a.D = k.c.D; a.dx = dx; a.dy = dy;
     a.nlines = sgm_lines(a.W, a.H, dx, dy);
     a.ref = k.c.ref;
-    a.P1n = k.P1n; a.P2n = k.P2n; a.P1s = k.P1s; a.P2s = k.P2s; a.tau_i = k.tau_i;
+    a.P1n = uniform(k.P1n); a.P2n = uniform(k.P2n); a.P1s = uniform(k.P1s);
+    a.P2s = uniform(k.P2s); a.tau_i = __builtin_amdgcn_readfirstlane(k.tau_i);
     const int line = (i - first[j]) * 4 + wave;
     if (line < a.nlines)
       sgm_scanline<NV, CT, FIRST, FULL, B>(a, line, static_cast<const CT*>(k.cost), k.agg);
@@ -298,6 +300,8 @@ pm_status sgm_args(const std::string& w, pm_shape shape, const uint8_t* ref,
     return fail(PM_ERR_INVALID_ARGUMENT, w + ": planes must be in 1..256");
   if (params.paths != 4 && params.paths != 8)
     return fail(PM_ERR_INVALID_ARGUMENT, w + ": paths must be 4 or 8");
+  if ((uint64_t)shape.width * shape.height * shape.planes * 4 >= (1ull << 32))
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": W H D must be below 2^30 (32-bit offsets)");
   const float kn[5] = {params.pi1, params.pi2, params.tau_so, params.sgm_q1, params.sgm_q2};
   for (float v : kn)
     if (!(std::isfinite(v) && v >= 0.0f))

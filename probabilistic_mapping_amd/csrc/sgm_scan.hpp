@@ -31,16 +31,21 @@ __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, 
 // RT: reference element type (uint8_t, or float for the differentiable form).  AGG: accumulate
 // into agg (else agg is untouched).  LOUT: also write this path's L to a.lout; with LMASK the
 // stored value is 0 at invalid pixels (the path's contribution to the aggregate).  EQ: write
-// the path's excess E = L - C' to a.eout as 16-bit fixed point (kExcessInvalid at invalid
-// pixels); L itself is computed exactly as in the other forms.
+// the path's excess E = L - C' to a.eout as 16-bit fixed point; L itself is computed exactly as
+// in the other forms.
 template <int NV, typename CT, bool FIRST, bool FULL, int B, typename RT = uint8_t,
           bool AGG = true, bool LOUT = false, bool LMASK = false, bool EQ = false>
 __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
                                              const CT* __restrict__ cost,
                                              float* __restrict__ agg) {
-  const RT* __restrict__ ref = static_cast<const RT*>(a.ref);
-  float* __restrict__ lout = a.lout;
-  uint16_t* __restrict__ eout = a.eout;
+  const gptr<const RT> ref = to_global(static_cast<const RT*>(a.ref));
+  // volumes through buffer resources: per step one uniform byte offset (the pixel's planes),
+  // per lane a constant one (its planes); the host keeps W H D 4 bytes below 4 GiB
+  const uint32_t V = (uint32_t)a.W * (uint32_t)a.H * (uint32_t)a.D;
+  const Buf bcost(cost, V * (uint32_t)sizeof(CT));
+  const Buf bagg(agg, AGG ? V * 4u : 0u);
+  const Buf blout(a.lout, LOUT ? V * 4u : 0u);
+  const Buf beout(a.eout, EQ ? V * 2u : 0u);
   const int lane = threadIdx.x & 63;
   int x, y;
   scanline_start(a, line, x, y);
@@ -60,6 +65,8 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
     act[j] = FULL || d < D;
     off[j] = FULL ? d : min(d, D - 1);
   }
+  // byte offsets of the lane's planes in a volume of elt-byte elements
+  auto lb = [&](int j, uint32_t elt) { return (uint32_t)off[j] * elt; };
   const float INF = __uint_as_float(0x7F800000u);
 
   // two register buffers of B steps each, used ping-pong: while one block computes, the
@@ -73,11 +80,11 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-      const long p = (long)(p0 + min(base + i, n - 1) * pstep) * D;
+      const uint32_t p = (uint32_t)(p0 + min(base + i, n - 1) * pstep) * (uint32_t)D;
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
-        c[i][j] = load_cost<CT>(cost + p + off[j]);
-        if (!FIRST && AGG) g[i][j] = agg[p + off[j]];
+        c[i][j] = buf_cost<CT>(bcost, lb(j, sizeof(CT)), p * (uint32_t)sizeof(CT));
+        if (!FIRST && AGG) g[i][j] = bagg.f32(lb(j, 4), p * 4u);
       }
     }
   };
@@ -97,7 +104,7 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
   };
   // one step k >= 1 from slot i of a block buffer
   auto step = [&](int k, int i, const float (&c)[B][NV], const float (&g)[B][NV]) {
-    const long p = (long)(p0 + k * pstep) * D;
+    const uint32_t p = (uint32_t)(p0 + k * pstep) * (uint32_t)D;
     bool neg = false;
 #pragma unroll
     for (int j = 0; j < NV; ++j) neg = neg || c[i][j] < 0.0f;
@@ -111,25 +118,27 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
     const bool sm = (smooth >> i) & 1ull;
     const float P1 = sm ? a.P1s : a.P1n;
     const float P2 = sm ? a.P2s : a.P2n;
-    const float lo = wave_shr1(L[NV - 1], INF);   // L(q, d0-1)
-    const float hi = wave_shl1(L[0], INF);        // L(q, d0+NV)
+    // L(q, d0-1) and L(q, d0+NV) from the neighbouring lanes (0xFFFFFFFF where none)
+    const uint32_t lo = shr1_max(L[NV - 1]);
+    const uint32_t hi = shl1_max(L[0]);
     const float mP2 = m + P2;
     float Ln[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-      const float left = j > 0 ? L[j - 1] : lo;
-      const float right = j + 1 < NV ? L[j + 1] : hi;
-      float val = umin(L[j], mP2);                 // sgm.cu:85
-      val = umin(val, umin(left, right) + P1);     // sgm.cu:86-93
+      const uint32_t left = j > 0 ? __float_as_uint(L[j - 1]) : lo;
+      const uint32_t right = j + 1 < NV ? __float_as_uint(L[j + 1]) : hi;
+      // min(L(q,d), L(q,d-1)+P1, L(q,d+1)+P1, m+P2) (sgm.cu:85-93)
+      const float val = umin(umin(L[j], mP2), __uint_as_float(min(left, right)) + P1);
       const float cc = invalid ? kCostInf : c[i][j];
       Ln[j] = act[j] ? (cc + val) - m : INF;       // sgm.cu:98
       const float out = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);
-      if (AGG && act[j]) agg[p + off[j]] = out;    // sgm.cu:101-108
-      if (LOUT && act[j]) lout[p + off[j]] = (LMASK && invalid) ? kCostInf : Ln[j];
+      if (AGG && act[j]) bagg.put_f32(out, lb(j, 4), p * 4u);   // sgm.cu:101-108
+      if (LOUT && act[j]) blout.put_f32((LMASK && invalid) ? kCostInf : Ln[j], lb(j, 4), p * 4u);
       if (EQ && act[j]) {
-        // val >= m exactly (every candidate is >= m), so E >= 0; rounding to nearest
+        // val >= m exactly (every candidate is >= m), so E >= 0; rounding to nearest.  At an
+        // invalid pixel the level is ignored (the combine finds the pixel from its cost).
         const uint32_t q = min(kExcessMaxQ, (uint32_t)fmaf(val - m, a.eq_inv, 0.5f));
-        eout[p + off[j]] = (uint16_t)(invalid ? kExcessInvalid : q);
+        beout.put_u16(q, lb(j, 2), p * 2u);
       }
     }
 #pragma unroll
@@ -153,12 +162,12 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
 
   // step 0: L = C' (sgm.cu:21-43)
   {
-    const long p = (long)p0 * D;
+    const uint32_t p = (uint32_t)p0 * (uint32_t)D;
     float c[NV], g[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-      c[j] = load_cost<CT>(cost + p + off[j]);
-      g[j] = (FIRST || !AGG) ? 0.0f : agg[p + off[j]];
+      c[j] = buf_cost<CT>(bcost, lb(j, sizeof(CT)), p * (uint32_t)sizeof(CT));
+      g[j] = (FIRST || !AGG) ? 0.0f : bagg.f32(lb(j, 4), p * 4u);
     }
     iq = ref_bits(ref[p0]);   // intensity of step 0
     bool neg = false;
@@ -169,9 +178,9 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
     for (int j = 0; j < NV; ++j) {
       L[j] = act[j] ? (invalid ? kCostInf : c[j]) : INF;
       const float out = invalid ? kCostInf : (FIRST ? L[j] : g[j] + L[j]);
-      if (AGG && act[j]) agg[p + off[j]] = out;
-      if (LOUT && act[j]) lout[p + off[j]] = (LMASK && invalid) ? kCostInf : L[j];
-      if (EQ && act[j]) eout[p + off[j]] = (uint16_t)(invalid ? kExcessInvalid : 0u);
+      if (AGG && act[j]) bagg.put_f32(out, lb(j, 4), p * 4u);
+      if (LOUT && act[j]) blout.put_f32((LMASK && invalid) ? kCostInf : L[j], lb(j, 4), p * 4u);
+      if (EQ && act[j]) beout.put_u16(0u, lb(j, 2), p * 2u);
     }
   }
   int base = 1;
