@@ -149,7 +149,7 @@ def measure_plain(args, torch, s, cost_type, storage):
     seq = pl.KeyframeSequence(s["images"], s["poses"], s["sources"], s["scene"].K, s["planes"],
                               s["idep_min"], s["idep_max"], s["knobs"], s["paths"],
                               cost_type=cost_type, median=bool(args.median),
-                              sgm_storage=storage)
+                              sgm_storage=storage, write_agg=False)
     nkf = len(s["images"])
     for _ in range(args.warmup):
         seq.run()
@@ -191,7 +191,7 @@ def measure_batched(args, torch, s, cost_type, storage):
     seq = pl.BatchedSequence(s["images"], s["poses"], s["sources"], s["scene"].K, s["planes"],
                              s["idep_min"], s["idep_max"], s["knobs"], s["paths"],
                              cost_type=cost_type, median=bool(args.median),
-                             sgm_storage=storage)
+                             sgm_storage=storage, write_agg=False)
     nkf = len(s["images"])
     side = torch.cuda.Stream()
     with torch.cuda.stream(side):

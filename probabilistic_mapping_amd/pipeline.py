@@ -236,7 +236,8 @@ class KeyframeSequence:
     """
 
     def __init__(self, images, poses, sources, K, planes, idep_min, idep_max, knobs, paths,
-                 cost_type="f32", median=False, device="cuda", sgm_storage="exact"):
+                 cost_type="f32", median=False, device="cuda", sgm_storage="exact",
+                 write_agg=True):
         self.device = torch.device(device)
         self.H, self.W = np.asarray(images[0]).shape
         self.D = planes
@@ -248,6 +249,7 @@ class KeyframeSequence:
         self.median = median
         assert sgm_storage in ("exact", "compact"), sgm_storage
         self.sgm_storage = sgm_storage
+        self.write_agg = write_agg   # materialise the aggregate (the filter needs only depth)
         self.K = np.asarray(K, np.float64)
         self.cam = (self.K[0, 0], self.K[1, 1], self.K[0, 2], self.K[1, 2])
         self.poses = [(np.asarray(R, np.float64), np.asarray(T, np.float64).reshape(3))
@@ -263,7 +265,8 @@ class KeyframeSequence:
         H, W, D = self.H, self.W, self.D
         dt = COST_TYPES[cost_type][1]
         self.cost = torch.empty((H, W, D), dtype=dt, device=self.device)
-        self.agg = torch.empty((H, W, D), dtype=torch.float32, device=self.device)
+        self.agg = (torch.empty((H, W, D), dtype=torch.float32, device=self.device)
+                    if write_agg else None)
         if sgm_storage == "compact":
             self.sgm_ws = sgm_excess_workspace(H, W, D, paths, self.device)
         else:
@@ -382,7 +385,7 @@ class Batch:
     ``dep_sample``.  Output tensors ``cost``, ``agg`` and ``depth`` are allocated per keyframe.
     """
 
-    def __init__(self, keyframes, cost_type, knobs, paths, storage="exact"):
+    def __init__(self, keyframes, cost_type, knobs, paths, storage="exact", write_agg=True):
         self.lib = C.load()
         self.cost_type = cost_type
         self.storage = storage
@@ -396,8 +399,11 @@ class Batch:
             H, W = kf["ref"].shape
             D = kf["planes"]
             dev = kf["ref"].device
+            # the exact storage accumulates into agg; compact may skip writing it
+            keep = write_agg or storage == "exact"
             out = {"cost": torch.empty((H, W, D), dtype=dt, device=dev),
-                   "agg": torch.empty((H, W, D), dtype=torch.float32, device=dev),
+                   "agg": (torch.empty((H, W, D), dtype=torch.float32, device=dev)
+                           if keep else None),
                    "depth": torch.empty((H, W), dtype=torch.float32, device=dev)}
             self.outputs.append(out)
             k = arr[i]
@@ -409,8 +415,9 @@ class Batch:
                 k.src_quads[s] = q.data_ptr()
                 k.homs[s].h[:] = [float(v) for v in np.asarray(h, np.float32).reshape(9)]
                 k.homs[s].t[:] = [float(v) for v in np.asarray(t, np.float32).reshape(3)]
-            k.cost, k.agg, k.depth = (out["cost"].data_ptr(), out["agg"].data_ptr(),
-                                      out["depth"].data_ptr())
+            k.cost = out["cost"].data_ptr()
+            k.agg = out["agg"].data_ptr() if out["agg"] is not None else None
+            k.depth = out["depth"].data_ptr()
         p = C.SgmParams(knobs["pi1"], knobs["pi2"], knobs["tau_so"], knobs["sgm_q1"],
                         knobs["sgm_q2"], paths)
         h = ctypes.c_void_p()
@@ -468,7 +475,8 @@ class BatchedSequence(KeyframeSequence):
         specs = [{"ref": self.images[i], "quads": [self.quads[j] for j in self.sources[i]],
                   "homs": self.homs[i], "planes": D, "idep_min": self.idep_min,
                   "dep_sample": self.dep_sample} for i in range(len(self.images))]
-        self.batch = Batch(specs, self.cost_type, self.knobs, self.paths, self.sgm_storage)
+        self.batch = Batch(specs, self.cost_type, self.knobs, self.paths, self.sgm_storage,
+                           write_agg=self.write_agg)
 
     def depth_of(self, i: int) -> torch.Tensor:
         return self.batch.outputs[i]["depth"]
