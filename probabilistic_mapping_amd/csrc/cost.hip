@@ -29,6 +29,8 @@
 
 namespace pm {
 
+// Tile count below which a single-keyframe launch splits each tile's planes over two blocks.
+constexpr long kCostSplitTiles = 4096;
 
 __global__ __launch_bounds__(256) void make_quad_kernel(const uint8_t* __restrict__ img,
                                                         uint64_t* __restrict__ quad, int W,
@@ -49,10 +51,12 @@ __global__ __launch_bounds__(256) void make_quad_kernel(const uint8_t* __restric
 }
 
 template <typename CT, int TY, int DC>
-__global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, CT* __restrict__ cost) {
+__global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, CT* __restrict__ cost,
+                                                        int npg) {
   __shared__ float stage[4][TY * DC * 64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  cost_tile<CT, TY, DC>(a, xcd_block(blockIdx.x, gridDim.x), cost, stage[wave]);
+  cost_tile<CT, TY, DC>(a, xcd_block(blockIdx.x, gridDim.x), cost, stage[wave], nullptr,
+                        nullptr, npg);
 }
 
 // Batched form: the tiles of every keyframe of a batch form one work list (keyframe k owns
@@ -157,17 +161,22 @@ extern "C" pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t nu
   }
   constexpr int TY = PM_COST_TILE_ROWS;
   constexpr int DC = PM_COST_STAGE_PLANES;
-  const unsigned blocks = (unsigned)cost_tiles<TY>(a.W, a.H);
+  const long tiles = cost_tiles<TY>(a.W, a.H);
+  // one keyframe's tiles are about 1.5 rounds of resident blocks at 752x480: splitting each
+  // tile's planes over two blocks (at least 4 planes per wave, 4-aligned) leaves a short tail
+  const int npg = (tiles < kCostSplitTiles && a.D % 32 == 0) ? 2 : 1;
+  const unsigned blocks = (unsigned)(tiles * npg);
   hipStream_t st = (hipStream_t)stream;
   switch (cost_type) {
     case PM_COST_F32:
-      cost_tile_kernel<float, TY, DC><<<blocks, 256, 0, st>>>(a, (float*)cost);
+      cost_tile_kernel<float, TY, DC><<<blocks, 256, 0, st>>>(a, (float*)cost, npg);
       break;
     case PM_COST_F16:
-      cost_tile_kernel<__half, TY, DC><<<blocks, 256, 0, st>>>(a, (__half*)cost);
+      cost_tile_kernel<__half, TY, DC><<<blocks, 256, 0, st>>>(a, (__half*)cost, npg);
       break;
     case PM_COST_BF16:
-      cost_tile_kernel<__hip_bfloat16, TY, DC><<<blocks, 256, 0, st>>>(a, (__hip_bfloat16*)cost);
+      cost_tile_kernel<__hip_bfloat16, TY, DC><<<blocks, 256, 0, st>>>(a, (__hip_bfloat16*)cost,
+                                                                        npg);
       break;
     default:
       return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: unknown cost type");

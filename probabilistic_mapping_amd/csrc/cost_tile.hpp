@@ -205,20 +205,24 @@ __device__ __forceinline__ void store_quad<__hip_bfloat16>(__hip_bfloat16* dst, 
 
 // RT: reference element type (uint8_t, or float for the differentiable form).  CNT: also write
 // the number of sources that saw each (pixel, plane) to cnt (uint8 [H][W][D]), staged in ctile.
+// Work item blk is tile blk / npg, plane group blk % npg: the tile's planes split over npg
+// blocks of 4 waves (npg > 1 gives a single keyframe's launch enough blocks to fill the chip).
 template <typename CT, int TY, int DC, typename RT = uint8_t, bool CNT = false>
 __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __restrict__ cost,
                                           float* __restrict__ tile,
                                           uint8_t* __restrict__ cnt_out = nullptr,
-                                          uint8_t* __restrict__ ctile = nullptr) {
+                                          uint8_t* __restrict__ ctile = nullptr, int npg = 1) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int pg = blk % npg;
+  blk /= npg;
   const int nstrips = (a.W + 61) / 62;
   const int strip = blk % nstrips;
   const int y0 = (blk / nstrips) * TY;
   const int W = a.W, H = a.H, D = a.D;
   const int u = strip * 62 - 1 + lane;
-  const int dper = (D + 3) / 4;
-  const int dbeg = wave * dper, dend = min(D, dbeg + dper);
+  const int dper = (D + 4 * npg - 1) / (4 * npg);
+  const int dbeg = (pg * 4 + wave) * dper, dend = min(D, dbeg + dper);
   const float Wm1 = (float)(W - 1), Hm1 = (float)(H - 1);
   const float INF = __uint_as_float(0x7F800000u);
   const float fu = (float)u;
