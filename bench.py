@@ -36,6 +36,11 @@ METRIC = "fused depth frames/s @752x480x64 planes, plain vs FP16-batched; % HBM 
 WORKLOAD = ("752x480 frames, 64 planes (idep 0.125-2.0), 5 sources, 8-path SGM, filter, "
             "Gaussian x Beta fusion, 20-keyframe sequence per step")
 COST_BYTES = {"f32": 4, "f16": 2, "bf16": 2}
+# HBM bytes per keyframe of the SGM pass (paths launch + combine/filter launch) on this
+# workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales measured on a 1 GiB device
+# copy), profiles/sgm_pass_752x480x64_s5_p8.md.  Keyed by (form, cost storage, SGM storage).
+SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2215e9,
+                    ("batched", "f16", "compact"): 1.1560e9}
 
 
 def parse():
@@ -54,6 +59,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the attainable-bandwidth copy (0 disables it)")
+    ap.add_argument("--tuning", default=None,
+                    help="batched form: launch shapes as JSON (pm_batch_tuning fields) instead "
+                         "of running the autotuner")
     return ap.parse_args()
 
 
@@ -181,7 +189,8 @@ def measure_plain(args, torch, s, cost_type, storage):
     return {"value": args.steps * nkf / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
             "stages": stages, "sgm_bytes": sgm_pass_bytes(seq, cost_type),
             "form": "plain", "cost_storage": cost_type, "sgm_storage": storage,
-            "keyframes_per_step": nkf}
+            "keyframes_per_step": nkf,
+            "traffic": SGM_PASS_TRAFFIC.get(("plain", cost_type, storage))}
 
 
 def measure_batched(args, torch, s, cost_type, storage):
@@ -195,7 +204,11 @@ def measure_batched(args, torch, s, cost_type, storage):
     nkf = len(s["images"])
     side = torch.cuda.Stream()
     with torch.cuda.stream(side):
-        tuning = seq.batch.autotune(3)
+        if args.tuning:
+            seq.batch.tuning = json.loads(args.tuning)
+            tuning = seq.batch.tuning
+        else:
+            tuning = seq.batch.autotune(3)
         seq.batch.capture()
         torch.cuda.synchronize()
 
@@ -236,7 +249,8 @@ def measure_batched(args, torch, s, cost_type, storage):
     return {"value": args.steps * nkf / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
             "stages": stages, "sgm_bytes": sgm_pass_bytes(seq, cost_type),
             "form": "batched (20 keyframes per hipGraph replay)", "cost_storage": cost_type,
-            "sgm_storage": storage, "keyframes_per_step": nkf, "tuning": tuning}
+            "sgm_storage": storage, "keyframes_per_step": nkf, "tuning": tuning,
+            "traffic": SGM_PASS_TRAFFIC.get(("batched", cost_type, storage))}
 
 
 def report(args, main: dict, extra: dict | None, copy, base) -> dict:
@@ -261,7 +275,7 @@ def report(args, main: dict, extra: dict | None, copy, base) -> dict:
                      "kernel": "sgm pass (all paths in one launch + combine/filter)",
                      "achieved": sg["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sg["frac"], "algorithmic_bytes": main["sgm_bytes"],
-                     "traffic": None},
+                     "traffic": main["traffic"]},
         "stages": main["stages"],
         "attainable_copy": copy,
         "cpu_baseline": base,
@@ -272,7 +286,11 @@ def report(args, main: dict, extra: dict | None, copy, base) -> dict:
         out["fp16_batched"] = {k: extra[k] for k in ("value", "ms_per_step", "form",
                                                      "cost_storage", "sgm_storage", "stages",
                                                      "tuning")}
-        out["fp16_batched"]["roofline_frac"] = extra["stages"]["sgm"]["frac"]
+        out["fp16_batched"]["roofline"] = {
+            "bound": "hbm", "kernel": "sgm pass (compact paths launch + combine/filter)",
+            "achieved": extra["stages"]["sgm"]["achieved_gbs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": extra["stages"]["sgm"]["frac"],
+            "algorithmic_bytes": extra["sgm_bytes"], "traffic": extra["traffic"]}
         out["fp16_batched_vs_plain"] = extra["value"] / main["value"]
     return out
 
