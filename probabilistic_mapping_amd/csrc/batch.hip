@@ -73,10 +73,15 @@ int grid_for(int total, int blocks_per_cu, int cus) {
   return std::max(8, std::min(round_up(total, 8), round_up(blocks_per_cu * cus, 8)));
 }
 
+// (tile rows, LDS staging planes) pairs with a compiled cost kernel; LDS per block is
+// 4 waves x rows x planes x 64 lanes x 4 bytes <= 64 KB.
+bool cost_shape_built(int rows, int planes) {
+  return ((rows == 2 || rows == 4) && (planes == 8 || planes == 16)) ||
+         (rows == 8 && (planes == 4 || planes == 8));
+}
+
 bool valid_tuning(const pm_batch_tuning& t) {
-  return (t.cost_tile_rows == 2 || t.cost_tile_rows == 4 ||
-          (t.cost_tile_rows == 8 && t.cost_stage_planes == 8)) &&
-         (t.cost_stage_planes == 8 || t.cost_stage_planes == 16) &&
+  return cost_shape_built(t.cost_tile_rows, t.cost_stage_planes) &&
          (t.sgm_prefetch == 8 || t.sgm_prefetch == 16) && t.cost_blocks_per_cu >= 0 &&
          t.sgm_blocks_per_cu >= 0 && t.cost_blocks_per_cu <= 64 && t.sgm_blocks_per_cu <= 64;
 }
@@ -447,12 +452,12 @@ extern "C" pm_status pm_batch_autotune(pm_batch* b, pm_stream stream, int32_t re
   };
   pm_batch_tuning cur = b->tune, win = b->tune;
   float best_ms = 1e30f;
-  const int rows[3] = {2, 4, 8}, planes[2] = {8, 16}, bpc[4] = {0, 4, 8, 16};
+  const int rows[3] = {2, 4, 8}, planes[3] = {4, 8, 16}, bpc[4] = {0, 4, 8, 16};
   for (int r : rows)
     for (int p : planes)
       for (int g : bpc) {
         if (status != PM_OK) break;
-        if (r == 8 && p == 16) continue;   // 128 KB of LDS per block: not built
+        if (!cost_shape_built(r, p)) continue;
         b->tune = cur;
         b->tune.cost_tile_rows = r;
         b->tune.cost_stage_planes = p;

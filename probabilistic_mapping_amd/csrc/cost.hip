@@ -95,6 +95,8 @@ bool launch_cost_batch_t(int tile_rows, int stage_planes, const KfDesc* kfs, con
     cost_batch_kernel<CT, 4, 16><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
   else if (tile_rows == 8 && stage_planes == 8)   // 64 KB of LDS per block
     cost_batch_kernel<CT, 8, 8><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+  else if (tile_rows == 8 && stage_planes == 4)
+    cost_batch_kernel<CT, 8, 4><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
   else
     return false;
   return true;

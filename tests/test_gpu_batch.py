@@ -73,7 +73,7 @@ def test_batch_equals_single_keyframe(uneven, cost_type, paths, storage):
     _assert_equal(batch, expect)
     # every launch shape gives the same bits
     for rows in (2, 4, 8):
-        for planes in ((8,) if rows == 8 else (8, 16)):
+        for planes in ((4, 8) if rows == 8 else (8, 16)):
             for pref in (8, 16):
                 for bpc in (0, 4):
                     for o in batch.outputs:
