@@ -209,7 +209,8 @@ typedef struct pm_keyframe {
 } pm_keyframe;
 
 /* Launch shapes (picked by pm_batch_autotune): cost tile rows 2, 4 or 8 with an LDS staging
- * chunk of 8 or 16 planes (4 or 8 with 8 rows), SGM prefetch depth 8 or 16 steps, and persistent grids of blocks_per_cu x
+ * chunk of 8 or 16 planes (4 or 8 with 8 rows; 2 rows only when no keyframe has more than 10
+ * sources), SGM prefetch depth 8 or 16 steps, and persistent grids of blocks_per_cu x
  * (compute units) blocks (0 = one block per work item). */
 typedef struct pm_batch_tuning {
   int32_t cost_tile_rows;

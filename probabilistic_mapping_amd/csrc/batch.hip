@@ -52,6 +52,7 @@ struct pm_batch {
   uint16_t* d_ebuf = nullptr;   // compact form: excess volumes of every keyframe
   int paths = 4;
   int cus = 256;
+  int max_sources = 0;          // over the keyframes (bounds the cost tile rows)
   std::vector<pm::KfDesc> host;
   pm::KfDesc* d_kfs = nullptr;
   int* d_pool = nullptr;
@@ -80,8 +81,14 @@ bool cost_shape_built(int rows, int planes) {
          (rows == 8 && (planes == 4 || planes == 8));
 }
 
-bool valid_tuning(const pm_batch_tuning& t) {
+// A tile of r rows holds the warp records of cost_tile_max_sources(r) sources.
+bool cost_rows_fit(int rows, int max_sources) {
+  return max_sources <= cost_tile_max_sources(rows);
+}
+
+bool valid_tuning(const pm_batch_tuning& t, int max_sources) {
   return cost_shape_built(t.cost_tile_rows, t.cost_stage_planes) &&
+         cost_rows_fit(t.cost_tile_rows, max_sources) &&
          (t.sgm_prefetch == 8 || t.sgm_prefetch == 16) && t.cost_blocks_per_cu >= 0 &&
          t.sgm_blocks_per_cu >= 0 && t.cost_blocks_per_cu <= 64 && t.sgm_blocks_per_cu <= 64;
 }
@@ -205,6 +212,7 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     KfDesc& d = b->host[k];
     d = KfDesc{};
     d.c.W = in.width; d.c.H = in.height; d.c.D = in.planes; d.c.S = in.num_sources;
+    b->max_sources = std::max(b->max_sources, in.num_sources);
     d.c.idep_min = in.idep_min; d.c.dep_sample = in.dep_sample; d.c.ref = in.ref;
     for (int s = 0; s < in.num_sources; ++s) {
       if (!in.src_quads[s]) {
@@ -354,7 +362,8 @@ extern "C" void pm_batch_destroy(pm_batch* b) {
 extern "C" pm_status pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t) {
   clear_error();
   if (!b || !t) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_set_tuning: null argument");
-  if (!valid_tuning(*t)) return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_set_tuning: bad values");
+  if (!valid_tuning(*t, b->max_sources))
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_set_tuning: bad values");
   b->tune = *t;
   drop_graph(b);   // a captured graph holds the old launch shapes
   return PM_OK;
@@ -457,7 +466,7 @@ extern "C" pm_status pm_batch_autotune(pm_batch* b, pm_stream stream, int32_t re
     for (int p : planes)
       for (int g : bpc) {
         if (status != PM_OK) break;
-        if (!cost_shape_built(r, p)) continue;
+        if (!cost_shape_built(r, p) || !cost_rows_fit(r, b->max_sources)) continue;
         b->tune = cur;
         b->tune.cost_tile_rows = r;
         b->tune.cost_stage_planes = p;

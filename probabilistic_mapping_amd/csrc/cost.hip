@@ -53,6 +53,7 @@ __global__ __launch_bounds__(256) void make_quad_kernel(const uint8_t* __restric
 template <typename CT, int TY, int DC>
 __global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, CT* __restrict__ cost,
                                                         int npg) {
+  static_assert(cost_tile_max_sources(TY) >= PM_MAX_SOURCES, "tile too short for the sources");
   __shared__ float stage[4][TY * DC * 64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   cost_tile<CT, TY, DC>(a, xcd_block(blockIdx.x, gridDim.x), cost, stage[wave], nullptr,

@@ -8,59 +8,100 @@
 
 namespace pm {
 
-// A measurement frame's quad texture ([H][W] texels of 8 bytes) read through a buffer resource:
-// wave-uniform base and size in scalar registers, a 32-bit byte offset per lane, and the
-// hardware range check (a read past W H 8 bytes returns 0 instead of touching memory).  W H < 2^29
-// is checked on the host, so the byte size fits the 32-bit record count.
+// Structured buffer load of one 8-byte record (LLVM's intrinsic for the idxen form of
+// buffer_load_dwordx2; clang has no builtin for it): address = base + vindex * stride + voffset.
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+__device__ u32x2_t struct_buffer_load_b64(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
+                                          int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
+
+// A measurement frame's quad texture ([H][W] texels of 8 bytes) read as a structured buffer of
+// W H records with an 8-byte stride: wave-uniform base and size in scalar registers, a texel
+// index per lane (one 24-bit multiply-add from the cell's row and column), and the hardware
+// range check (an index past W H returns 0 instead of touching memory).  W H < 2^29 is checked
+// on the host.
 struct QuadTex {
   __amdgpu_buffer_rsrc_t r;
   __device__ __forceinline__ QuadTex(const uint64_t* q, uint32_t W, uint32_t H)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(q), 0, (int)(W * H * 8u),
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(q), 8, (int)(W * H),
                                             kBufferDword3)) {}
-  __device__ __forceinline__ uint2 at(uint32_t byte_off) const {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_off, 0, 0);
+  __device__ __forceinline__ uint2 at(uint32_t texel) const {
+    const u32x2_t v = struct_buffer_load_b64(r, (int)texel, 0, 0, 0);
     return make_uint2(v[0], v[1]);
   }
 };
 
-// Bilinear sample of the quad texture at (U, V) with the warp's validity (calc_cost.cu:85-93);
-// returns +inf for an invalid warp.  Validity is "the clamp to the image changes nothing" and
-// w >= 0, so a NaN warp (0/0) is invalid; an invalid warp still reads an in-bounds texel (the
-// clamped position), so every gather of a row can be in flight at once.
-__device__ __forceinline__ float warp_sample(const QuadTex& quad, uint32_t W,
-                                             float Wm1, float Hm1, float X, float Y, float w) {
-  const float rw = __builtin_amdgcn_rcpf(w);
-  const float U = X * rw;
-  const float V = Y * rw;
-  const float Uc = __builtin_amdgcn_fmed3f(U, 0.0f, Wm1);
-  const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
-  const bool valid = (w >= 0.0f) & (Uc == U) & (Vc == V);
-  // non-negative, so truncation is floor; W, y <= 65535 fit the 24-bit multiply
-  const uint32_t off = __umul24((uint32_t)Vc, W) * 8u + (uint32_t)Uc * 8u;
-  const uint2 t = quad.at(off);
-  const float ax = __builtin_amdgcn_fractf(Uc), ay = __builtin_amdgcn_fractf(Vc);
-  const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
-  const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
-  const float top = fmaf(ax, __high2float(lo), __low2float(lo));
-  const float dv = fmaf(ax, __high2float(hi), __low2float(hi));
-  return valid ? fmaf(ay, dv, top) : __uint_as_float(0x7F800000u);
-}
-
-// warp_sample for a warp known to be valid and inside the image (see tile_inside_mask): the
-// same arithmetic without the clamp and the validity test, so it returns the same value.
-__device__ __forceinline__ float warp_sample_inside(const QuadTex& quad, uint32_t W,
-                                                    float X, float Y, float w) {
-  const float rw = __builtin_amdgcn_rcpf(w);
-  const float U = X * rw;
-  const float V = Y * rw;
-  const uint32_t off = __umul24((uint32_t)V, W) * 8u + (uint32_t)U * 8u;
-  const uint2 t = quad.at(off);
-  const float ax = __builtin_amdgcn_fractf(U), ay = __builtin_amdgcn_fractf(V);
+// Bilinear interpolation of texel t = (q00, q10 - q00, q01 - q00, q11 - q10 - q01 + q00) at
+// weights (ax, ay).  The differences are integers in [-510, 510], exact in FP16.
+__device__ __forceinline__ float bilinear(uint2 t, float ax, float ay) {
   const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
   const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
   const float top = fmaf(ax, __high2float(lo), __low2float(lo));
   const float dv = fmaf(ax, __high2float(hi), __low2float(hi));
   return fmaf(ay, dv, top);
+}
+
+// Bilinear sample of the quad texture at the warped point (U, V) with the warp's validity
+// (calc_cost.cu:85-93); returns +inf for an invalid warp.  Validity is "the clamp to the image
+// changes nothing" and w >= 0, so a NaN warp (0/0) is invalid; an invalid warp still reads an
+// in-bounds texel (the clamped position), so every gather of a row can be in flight at once.
+__device__ __forceinline__ float sample_uv(const QuadTex& quad, uint32_t W, float Wm1,
+                                           float Hm1, float U, float V, float w) {
+  const float Uc = __builtin_amdgcn_fmed3f(U, 0.0f, Wm1);
+  const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
+  const bool valid = (w >= 0.0f) & (Uc == U) & (Vc == V);
+  // non-negative, so truncation is floor; W, y <= 65535 fit the 24-bit multiply
+  const uint2 t = quad.at(__umul24((uint32_t)Vc, W) + (uint32_t)Uc);
+  const float s = bilinear(t, __builtin_amdgcn_fractf(Uc), __builtin_amdgcn_fractf(Vc));
+  return valid ? s : __uint_as_float(0x7F800000u);
+}
+
+// sample_uv for a point known to be valid and inside the image (see tile_inside_mask): the
+// same arithmetic without the clamp and the validity test, so it returns the same value.
+__device__ __forceinline__ float sample_uv_inside(const QuadTex& quad, uint32_t W, float U,
+                                                  float V) {
+  const uint2 t = quad.at(__umul24((uint32_t)V, W) + (uint32_t)U);
+  return bilinear(t, __builtin_amdgcn_fractf(U), __builtin_amdgcn_fractf(V));
+}
+
+// A gather in flight: the texel of the cell of (U, V), its bilinear weights and, for the
+// checked form, the warp's validity.  gather_uv issues the load; gather_value consumes it, so
+// a caller can put independent work between the two.
+struct Gather {
+  uint2 t;
+  float ax, ay;
+  bool ok;
+};
+template <bool INSIDE>
+__device__ __forceinline__ Gather gather_uv(const QuadTex& quad, uint32_t W, float Wm1,
+                                            float Hm1, float U, float V, float w) {
+  Gather g;
+  if (INSIDE) {   // sample_uv_inside
+    g.t = quad.at(__umul24((uint32_t)V, W) + (uint32_t)U);
+    g.ax = __builtin_amdgcn_fractf(U);
+    g.ay = __builtin_amdgcn_fractf(V);
+    g.ok = true;
+  } else {        // sample_uv
+    const float Uc = __builtin_amdgcn_fmed3f(U, 0.0f, Wm1);
+    const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
+    g.ok = (w >= 0.0f) & (Uc == U) & (Vc == V);
+    g.t = quad.at(__umul24((uint32_t)Vc, W) + (uint32_t)Uc);
+    g.ax = __builtin_amdgcn_fractf(Uc);
+    g.ay = __builtin_amdgcn_fractf(Vc);
+  }
+  return g;
+}
+template <bool INSIDE>
+__device__ __forceinline__ float gather_value(const Gather& g) {
+  const float s = bilinear(g.t, g.ax, g.ay);
+  return (INSIDE || g.ok) ? s : __uint_as_float(0x7F800000u);
+}
+
+// Sample of the warp (X, Y, w): (U, V) = (X, Y) / w.
+__device__ __forceinline__ float warp_sample(const QuadTex& quad, uint32_t W,
+                                             float Wm1, float Hm1, float X, float Y, float w) {
+  const float rw = __builtin_amdgcn_rcpf(w);
+  return sample_uv(quad, W, Wm1, Hm1, X * rw, Y * rw, w);
 }
 
 // Conservative whole-tile validity, per source.  Every warp a cost_tile wave evaluates for one
@@ -71,7 +112,7 @@ __device__ __forceinline__ float warp_sample_inside(const QuadTex& quad, uint32_
 // take their extremes at the corners; so if every corner has w >= kInsideW and U, V inside the
 // image by kInsideMargin pixels, every warp of the tile is valid with room to spare for FP32
 // rounding (< 4e-3 px at w >= kInsideW for |coordinates| < 2^16).  Then the validity tests can
-// be skipped: warp_sample_inside returns what warp_sample would.
+// be skipped: sample_uv_inside returns what sample_uv would.
 // Lane l tests corner l & 7 of source l >> 3 (sources 0..7); tile_inside_mask returns a
 // wave-uniform mask whose bit s says source s passed for the plane at idep.
 constexpr float kInsideW = 0.25f;
@@ -120,7 +161,9 @@ __device__ __forceinline__ uint32_t tile_inside_mask(const InsideCorner& c, floa
                                                      float Wm1, float Hm1) {
   const float X = fmaf(c.tx, idep, c.X0), Y = fmaf(c.ty, idep, c.Y0);
   const float Z = fmaf(c.tz, idep, c.Z0);
-  const float U = X / Z, V = Y / Z;
+  // rcp is within 1 ulp: far inside the kInsideMargin / kInsideW slack
+  const float rz = __builtin_amdgcn_rcpf(Z);
+  const float U = X * rz, V = Y * rz;
   const bool ok = !c.live || (Z >= kInsideW && U >= kInsideMargin && U <= Wm1 - kInsideMargin &&
                               V >= kInsideMargin && V <= Hm1 - kInsideMargin);
   const uint64_t bad = __ballot(!ok);
@@ -147,8 +190,7 @@ __device__ __forceinline__ SampleGrad warp_sample_grad(const QuadTex& quad,
   const float Uc = __builtin_amdgcn_fmed3f(U, 0.0f, Wm1);
   const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
   const bool valid = (w >= 0.0f) & (Uc == U) & (Vc == V);
-  const uint32_t off = __umul24((uint32_t)Vc, W) * 8u + (uint32_t)Uc * 8u;
-  const uint2 t = quad.at(off);
+  const uint2 t = quad.at(__umul24((uint32_t)Vc, W) + (uint32_t)Uc);
   const float ax = __builtin_amdgcn_fractf(Uc), ay = __builtin_amdgcn_fractf(Vc);
   const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
   const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
@@ -240,80 +282,126 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
   const InsideCorner corner =
       tile_inside_setup(a, (float)(strip * 62 - 1), (float)(strip * 62 + 62), (float)(y0 - 1),
                         (float)(y0 + TY), (float)y0, (float)(y0 + TY - 1));
+  // lane s holds source s's (h11, h12, h13, h21), (h22, h23, h31, h32), (h33, t1, t2, t3)
+  float4 ha = make_float4(0.0f, 0.0f, 0.0f, 0.0f), hb = ha, hc = ha;
+  if (lane < a.S) {
+    const float4* hp = reinterpret_cast<const float4*>(a.hm[lane]);
+    ha = hp[0]; hb = hp[1]; hc = hp[2];
+  }
   for (int dc0 = dbeg; dc0 < dend; dc0 += DC) {
   const int dcn = min(DC, dend - dc0);
+  // Per-(plane, source) constants of the warps.  At the start of each plane, lane s computes
+  // source s's record and parks it in the plane's own output slot of the tile (12 floats per
+  // source from the slot's start, so S <= TY * 64 / 12); the plane's results overwrite it once
+  // the plane is done.  With x = h11 u + h12 v + h13 and (tx, ty, tz) = t idep:
+  //   r0 = (h13 + tx, h23 + ty, h33 + tz, h13 + h31 + tz)   centre warps; w of term 4 at u = 0
+  //   r1 = (h13 + h11 - h32 + tz, k4x, k4y, k8x)            w of term 8 at u = 0
+  //   r2 = (k8y, -, -, -)
+  // Term 4 = (x + h11 + tx, x + h21 + ty, x + h31 + tz) has U = 1 + k4x / w, V = 1 + k4y / w with
+  // k4x = h11 - h31 + tx - tz, k4y = h21 - h31 + ty - tz; term 8 = (x + h11 - h12 + tx,
+  // x + h11 - h22 + ty, x + h11 - h32 + tz) has k8x = h32 - h12 + tx - tz, k8y = h32 - h22 + ty - tz.
+  auto rec = [&](int j, int f) {   // float4 f of plane slot j
+    return reinterpret_cast<float4*>(tile + ((f >> 4) * DC + j) * 64 + (f & 15) * 4);
+  };
   for (int d = dc0; d < dc0 + dcn; ++d) {
     const float idep = a.idep_min + (float)d * a.dep_sample;
+    const int j = d - dc0;
+    if (lane < a.S) {
+      const float tx = hc.y * idep, ty = hc.z * idep, tz = hc.w * idep;
+      const float dxz = tx - tz, dyz = ty - tz;
+      *rec(j, 3 * lane) = make_float4(ha.z + tx, hb.y + ty, hc.x + tz, (ha.z + hb.z) + tz);
+      *rec(j, 3 * lane + 1) = make_float4((ha.z + (ha.x - hb.w)) + tz, (ha.x - hb.z) + dxz,
+                                          (ha.w - hb.z) + dyz, (hb.w - ha.y) + dxz);
+      *rec(j, 3 * lane + 2) = make_float4((hb.w - hb.x) + dyz, 0.0f, 0.0f, 0.0f);
+    }
+    __builtin_amdgcn_wave_barrier();
     const uint32_t inside = tile_inside_mask(corner, idep, Wm1, Hm1);
     float acc[TY], cnt[TY];
+    int n_inside = 0;   // sources that took the inside pass (each counts on every row)
 #pragma unroll
     for (int t = 0; t < TY; ++t) { acc[t] = 0.0f; cnt[t] = 0.0f; }
     for (int s = 0; s < a.S; ++s) {
       const float* hm = a.hm[s];
-      const float h11 = hm[0], h12 = hm[1], h13 = hm[2], h21 = hm[3], h22 = hm[4],
-                  h23 = hm[5], h31 = hm[6], h32 = hm[7], h33 = hm[8];
-      const float tx = hm[9] * idep, ty = hm[10] * idep, tz = hm[11] * idep;
+      const float h11 = hm[0], h12 = hm[1], h21 = hm[3], h22 = hm[4], h31 = hm[6], h32 = hm[7];
+      const float4 r0 = *rec(j, 3 * s), r1 = *rec(j, 3 * s + 1), r2 = *rec(j, 3 * s + 2);
       const QuadTex q(a.quad[s], (uint32_t)W, (uint32_t)H);
-      // column parts of H (u, r, 1) + t idep
-      const float Xu = h11 * fu + h13 + tx;
-      const float Yu = h21 * fu + h23 + ty;
-      const float Zu = h31 * fu + h33 + tz;
-      // garbled warps, offsets from x = h11 u + h12 v + h13 (calc_cost.cu:49-51, 69-71)
-      const float xu = h11 * fu + h13;
-      const float g4x = xu + (h11 + tx), g4y = xu + (h21 + ty), g4z = xu + (h31 + tz);
-      const float g8x = xu + (h11 - h12 + tx), g8y = xu + (h11 - h22 + ty),
-                  g8z = xu + (h11 - h32 + tz);
+      // column parts of H (u, r, 1) + t idep, and of the garbled warps' w
+      const float Xu = fmaf(fu, h11, r0.x);
+      const float Yu = fmaf(fu, h21, r0.y);
+      const float Zu = fmaf(fu, h31, r0.z);
+      const float w4u = fmaf(fu, h11, r0.w);
+      const float w8u = fmaf(fu, h11, r1.x);
+      const float k4x = r1.y, k4y = r1.z, k8x = r1.w, k8y = r2.x;
       // one pass over the centre rows; INSIDE: every warp of the tile is valid (no tests)
       auto pass = [&](auto inside_tag) {
         constexpr bool INSIDE = decltype(inside_tag)::value;
-        auto sample = [&](float X, float Y, float w) {
-          return INSIDE ? warp_sample_inside(q, (uint32_t)W, X, Y, w)
-                        : warp_sample(q, (uint32_t)W, Wm1, Hm1, X, Y, w);
+        auto centre = [&](int k) {   // the centre warp of row y0 - 1 + k
+          const float fr = (float)(y0 - 1 + k);
+          const float w = fmaf(h32, fr, Zu);
+          const float rw = __builtin_amdgcn_rcpf(w);
+          return gather_uv<INSIDE>(q, (uint32_t)W, Wm1, Hm1, fmaf(h12, fr, Xu) * rw,
+                                   fmaf(h22, fr, Yu) * rw, w);
         };
-        // window over centre rows r-2, r-1: |I-S| (A, row r-1 only), its left-neighbour copy
-        // (B), up terms (Up)
-        float A1 = 0.0f, B0 = 0.0f, B1 = 0.0f, Up0 = 0.0f, Up1 = 0.0f;
+        auto garbled = [&](int t, float wu, float kx, float ky) {   // output row y0 + t
+          const float w = fmaf(h12, (float)(y0 + t), wu);
+          const float rw = __builtin_amdgcn_rcpf(w);
+          return gather_uv<INSIDE>(q, (uint32_t)W, Wm1, Hm1, fmaf(kx, rw, 1.0f),
+                                   fmaf(ky, rw, 1.0f), w);
+        };
+        // Row k's gathers are issued one row ahead (before row k-1's arithmetic), and the
+        // scheduling barrier keeps the compiler from sinking them back to their uses: each
+        // row's loads are in flight while the previous row computes.
+        Gather gc[TY + 2], g4[TY], g8[TY];
+        gc[0] = centre(0);
+        // window over centre rows r-2, r-1: I-S (A0, A1) and the up terms (Up0, Up1)
+        float A0 = 0.0f, A1 = 0.0f, Up0 = 0.0f, Up1 = 0.0f;
 #pragma unroll
         for (int k = 0; k < TY + 2; ++k) {   // centre row r = y0 - 1 + k
-          const float fr = (float)(y0 - 1 + k);
-          const float S = sample(fmaf(h12, fr, Xu), fmaf(h22, fr, Yu), fmaf(h32, fr, Zu));
+          if (k + 1 < TY + 2) {
+            gc[k + 1] = centre(k + 1);
+            if (k >= 1) {
+              g4[k - 1] = garbled(k - 1, w4u, k4x, k4y);
+              g8[k - 1] = garbled(k - 1, w8u, k8x, k8y);
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          const float S = gather_value<INSIDE>(gc[k]);
           // signed differences; the |.| is taken where they are summed (a source modifier)
           const float A2 = I[k + 2] - S;            // I(u,r) - S(u,r)
           const float Up2 = I[k + 4] - S;           // I(u,r+2) - S(u,r)
-          const float B2 = wave_shr1z(A2);          // the same at (u-1, r)
           if (k >= 2) {
             const int t = k - 2;                    // output row v = r - 1
-            const float fv = (float)(y0 + t);
             const float Dn = I[k] - S;              // I(u,r-2) - S(u,r): term 2 of row v
-            const float G4 = IR[t + 1] - sample(fmaf(h12, fv, g4x), fmaf(h12, fv, g4y),
-                                                fmaf(h12, fv, g4z));
-            const float G8 = IR[t] - sample(fmaf(h12, fv, g8x), fmaf(h12, fv, g8y),
-                                            fmaf(h12, fv, g8z));
-            float T = fabsf(A1) + fabsf(B0);        // terms 0, 5
-            T += fabsf(B1);                         // term 3
+            const float G4 = IR[t + 1] - gather_value<INSIDE>(g4[t]);
+            const float G8 = IR[t] - gather_value<INSIDE>(g8[t]);
+            // terms 5, 3, 7 are the left neighbour's |A| over rows v-1, v, v+1: one column sum
+            // moved one lane right
+            const float col = fabsf(A0) + fabsf(A1) + fabsf(A2);
+            float T = wave_shr1z(col) + fabsf(A1);  // terms 3, 5, 7, 0
             T += fabsf(wave_shl1z(A2));             // term 6
-            T += fabsf(B2);                         // term 7
             T += fabsf(Up0) + fabsf(Dn);            // terms 1, 2
             T += fabsf(G4) + fabsf(G8);             // terms 4, 8
             if (INSIDE) {
               acc[t] += T;
-              cnt[t] += 1.0f;
             } else {
               const bool ok = T < INF;
               acc[t] += ok ? T : 0.0f;
               cnt[t] += ok ? 1.0f : 0.0f;
             }
           }
-          A1 = A2;
-          B0 = B1; B1 = B2;
+          A0 = A1; A1 = A2;
           Up0 = Up1; Up1 = Up2;
         }
       };
-      if ((inside >> s) & 1u)
+      if ((inside >> s) & 1u) {
         pass(std::true_type{});
-      else
+        ++n_inside;
+      } else {
         pass(std::false_type{});
+      }
     }
+#pragma unroll
+    for (int t = 0; t < TY; ++t) cnt[t] += (float)n_inside;
 #pragma unroll
     for (int t = 0; t < TY; ++t) {   // mean over sources of (sum of 9 terms)/9, or -1
       tile[(t * DC + (d - dc0)) * 64 + lane] = cnt[t] > 0.0f ? acc[t] / (9.0f * cnt[t]) : -1.0f;

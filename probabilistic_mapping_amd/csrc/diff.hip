@@ -37,6 +37,7 @@ namespace pm {
 namespace {
 
 constexpr int kCostTy = 4, kCostDc = 8;   // forward cost tile
+static_assert(cost_tile_max_sources(kCostTy) >= PM_MAX_SOURCES, "tile too short for the sources");
 constexpr int kAdjTy = 2, kAdjDc = 8;     // adjoint cost tile
 constexpr int kReadoutBlocks = 1024;
 

@@ -126,6 +126,28 @@ def test_batch_rejects_bad_input(uneven):
         b.launch()   # nothing captured
 
 
+def test_batch_max_sources_tile_rows(uneven):
+    """A keyframe with 16 sources: 2-row cost tiles hold the warp records of at most 10
+    sources, so that shape is refused (and skipped by the autotuner); 4 and 8 rows run it."""
+    from probabilistic_mapping_amd import _capi as C, pipeline
+    kf = dict(uneven[3])
+    n = len(kf["quads"])
+    kf["quads"] = [kf["quads"][i % n] for i in range(16)]
+    kf["homs"] = [(h, (t * (1.0 + 0.03 * i)).astype(np.float32))
+                  for i, (h, t) in enumerate(kf["homs"][i % n] for i in range(16))]
+    expect = [_single(pipeline, kf, "f16", 8)]
+    b = pipeline.Batch([kf], "f16", KNOBS, 8)
+    with pytest.raises(C.PmError):
+        b.tuning = dict(cost_tile_rows=2, cost_stage_planes=8)
+    for rows, planes in ((4, 8), (8, 4)):
+        b.tuning = dict(cost_tile_rows=rows, cost_stage_planes=planes)
+        b.run()
+        torch.cuda.synchronize()
+        _assert_equal(b, expect)
+    best = b.autotune(reps=1)
+    assert best["cost_tile_rows"] != 2
+
+
 @pytest.mark.parametrize("cost_type,storage", [("f16", "exact"), ("bf16", "exact"),
                                                 ("f16", "compact"), ("bf16", "compact")])
 def test_storage_precision_error_report(trajectory_scene, cost_type, storage):

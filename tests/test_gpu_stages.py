@@ -63,6 +63,25 @@ def test_cost_small_scene(dev, small_scene, cost_type):
     assert st["rel"]["max"] <= bound
 
 
+def test_cost_max_sources(dev, small_scene):
+    """16 measurement frames (the most a keyframe takes): every source's warp record fits."""
+    s = small_scene
+    K, homs, D, ds = _small_inputs(s)
+    n = len(s["srcs"])
+    srcs = [s["srcs"][i % n] for i in range(16)]
+    # distinct homographies per source: a slightly different translation for each repeat
+    hs = [(h, (t * (1.0 + 0.03 * i)).astype(np.float32)) for i, (h, t) in
+          enumerate(homs[i % n] for i in range(16))]
+    ref = torch.from_numpy(s["ref"]).cuda()
+    quads = [dev.make_quad(torch.from_numpy(im).cuda()) for im in srcs]
+    g = dev.calc_cost(ref, quads, hs, s["idep_min"], ds, D, "f32").cpu().numpy()
+    o = oracle.calc_cost(s["ref"], srcs, hs, s["idep_min"], ds, D, "f64")
+    st = parity.cost_parity(g, o)
+    _report("cost 16 sources", st)
+    assert st["validity_mismatch"] <= 1e-4
+    assert st["rel"]["max"] <= 1e-3
+
+
 def test_cost_all_invalid_when_behind_camera(dev, small_scene):
     """w < 0 for every plane (source behind the reference) leaves the whole volume at -1."""
     s = small_scene
