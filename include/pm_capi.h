@@ -226,8 +226,17 @@ typedef struct pm_batch pm_batch;
  * equal pm_sgm + pm_filter_cost, bit for bit).  COMPACT: every path of every keyframe in one
  * launch, storing 16-bit excess volumes in a batch-owned scratch (paths * W * H * D uint16
  * per keyframe), then one combine + filter launch per lane class (results equal
- * pm_sgm_filter_excess, bit for bit; a keyframe's agg may then be NULL). */
-typedef enum pm_sgm_storage { PM_SGM_EXACT = 0, PM_SGM_COMPACT = 1 } pm_sgm_storage;
+ * pm_sgm_filter_excess, bit for bit; a keyframe's agg may then be NULL).  SWEEP: the
+ * horizontal paths as COMPACT; the downward paths advance together row by row and store one
+ * 16-bit volume of their summed excess (level step = downward paths x the excess step); the
+ * upward paths do the same and rebuild the aggregate in registers, then filter on the spot
+ * (3 W H D uint16 + 6 W D floats of scratch per keyframe; at most 128 planes; agg, if given,
+ * receives the rebuilt aggregate). */
+typedef enum pm_sgm_storage {
+  PM_SGM_EXACT = 0,
+  PM_SGM_COMPACT = 1,
+  PM_SGM_SWEEP = 2
+} pm_sgm_storage;
 
 /* Copies the keyframe table and work lists to the device (allocates; call once). */
 pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count, pm_cost_type cost_type,

@@ -24,6 +24,7 @@ PM_COST_BF16 = 2
 
 PM_SGM_EXACT = 0
 PM_SGM_COMPACT = 1
+PM_SGM_SWEEP = 2
 
 PM_MAX_PLANES = 256
 PM_MAX_SOURCES = 16

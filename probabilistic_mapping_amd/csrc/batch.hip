@@ -31,8 +31,11 @@ struct SgmClass {
   int kf_off = 0;            // offset of kf[] in the device int pool
   int first_off[8] = {};     // offsets of first[] per path in the pool
   int total[8] = {};
-  // compact form: one work list over (keyframe, path) segments
+  // compact form: one work list over (keyframe, path) segments (sweep form: the horizontal
+  // paths only)
   int seg_off = 0, seg_first_off = 0, nseg = 0, seg_total = 0;
+  // sweep form: workgroups (keyframe, band) and the most row blocks of any keyframe
+  int sw_first_off = 0, sw_total = 0, sw_blocks = 0;
 };
 
 struct FilterClass {
@@ -49,7 +52,8 @@ struct pm_batch {
   int nkf = 0;
   pm_cost_type ct = PM_COST_F32;
   pm_sgm_storage storage = PM_SGM_EXACT;
-  uint16_t* d_ebuf = nullptr;   // compact form: excess volumes of every keyframe
+  uint16_t* d_ebuf = nullptr;   // compact / sweep form: excess volumes of every keyframe
+  float* d_sweep = nullptr;     // sweep form: row-boundary path states of every keyframe
   int paths = 4;
   int cus = 256;
   int max_sources = 0;          // over the keyframes (bounds the cost tile rows)
@@ -104,8 +108,22 @@ pm_status run_cost(pm_batch* b, hipStream_t st) {
   return check_launch("cost_batch_kernel");
 }
 
+// Sweep form: every row block of the downward (up = false) or upward sweep, in order.
+pm_status run_sweeps(pm_batch* b, bool up, hipStream_t st) {
+  for (const SgmClass& c : b->sgm)
+    for (int rb = 0; rb < c.sw_blocks; ++rb) {
+      if (!launch_sgm_sweep_batch(b->ct, c.nv, c.full, up, b->d_kfs, b->d_pool + c.kf_off,
+                                  b->d_pool + c.sw_first_off, (int)c.kf.size(), c.sw_total, rb,
+                                  st))
+        return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch: unsupported sweep class");
+      const pm_status s = check_launch("sgm_sweep_kernel");
+      if (s != PM_OK) return s;
+    }
+  return PM_OK;
+}
+
 pm_status run_sgm(pm_batch* b, hipStream_t st) {
-  if (b->storage == PM_SGM_COMPACT) {
+  if (b->storage == PM_SGM_COMPACT || b->storage == PM_SGM_SWEEP) {
     for (const SgmClass& c : b->sgm) {
       if (c.seg_total == 0) continue;
       if (!launch_sgm_excess_batch(b->ct, b->tune.sgm_prefetch, c.nv, c.full, b->d_kfs,
@@ -116,7 +134,7 @@ pm_status run_sgm(pm_batch* b, hipStream_t st) {
       const pm_status s = check_launch("sgm_excess_batch_kernel");
       if (s != PM_OK) return s;
     }
-    return PM_OK;
+    return b->storage == PM_SGM_SWEEP ? run_sweeps(b, false, st) : PM_OK;
   }
   for (int p = 0; p < b->paths; ++p) {
     for (const SgmClass& c : b->sgm) {
@@ -136,6 +154,7 @@ pm_status run_sgm(pm_batch* b, hipStream_t st) {
 }
 
 pm_status run_filter(pm_batch* b, hipStream_t st) {
+  if (b->storage == PM_SGM_SWEEP) return run_sweeps(b, true, st);   // combines and filters
   for (const FilterClass& c : b->filt) {
     if (c.total == 0) continue;
     if (b->storage == PM_SGM_COMPACT)
@@ -172,7 +191,7 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: unknown cost type");
   if (params.paths != 4 && params.paths != 8)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: paths must be 4 or 8");
-  if (storage != PM_SGM_EXACT && storage != PM_SGM_COMPACT)
+  if (storage != PM_SGM_EXACT && storage != PM_SGM_COMPACT && storage != PM_SGM_SWEEP)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch_create: unknown SGM storage");
   const float kn[6] = {params.pi1, params.pi2, params.tau_so, params.sgm_q1, params.sgm_q2,
                        var_scale};
@@ -203,6 +222,12 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
         (uint64_t)in.width * in.height * in.planes * 4 >= (1ull << 32)) {
       delete b;
       return fail(PM_ERR_INVALID_ARGUMENT, who + "planes or num_sources out of range");
+    }
+    if (storage == PM_SGM_SWEEP &&
+        (in.planes > 128 || (uint64_t)in.width * in.height * in.planes * 6 >= (1ull << 32))) {
+      delete b;
+      return fail(PM_ERR_INVALID_ARGUMENT, who + "the sweep storage takes at most 128 planes "
+                                                 "and W H D 6 bytes below 4 GiB");
     }
     if (!in.ref || !in.cost || (!in.agg && storage == PM_SGM_EXACT) || !in.depth ||
         !std::isfinite(in.idep_min) || !std::isfinite(in.dep_sample)) {
@@ -295,8 +320,9 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     // any time cover all paths of a few keyframes (whose cost volumes stay cache-resident)
     std::vector<int> seg, sfirst;
     int acc = 0;
+    const int seg_paths = storage == PM_SGM_SWEEP ? 2 : b->paths;   // sweep: L->R, R->L
     for (int kf : c.kf)
-      for (int p = 0; p < b->paths; ++p) {
+      for (int p = 0; p < seg_paths; ++p) {
         const CostArgs& a = b->host[kf].c;
         seg.push_back(kf);
         seg.push_back(p);
@@ -307,6 +333,16 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     c.seg_first_off = put(sfirst);
     c.nseg = (int)sfirst.size();
     c.seg_total = acc;
+    std::vector<int> wfirst(c.kf.size());
+    int wacc = 0;
+    for (size_t j = 0; j < c.kf.size(); ++j) {
+      const CostArgs& a = b->host[c.kf[j]].c;
+      wfirst[j] = wacc;
+      wacc += sweep_bands(a.W);
+      c.sw_blocks = std::max(c.sw_blocks, sweep_blocks(a.H));
+    }
+    c.sw_first_off = put(wfirst);
+    c.sw_total = wacc;
   }
   for (FilterClass& c : b->filt) {
     c.kf_off = put(c.kf);
@@ -321,10 +357,11 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     c.total = acc;
   }
   hipError_t e = hipSuccess;
-  if (storage == PM_SGM_COMPACT) {
-    // every keyframe's volumes start 16-byte aligned (8 uint16)
+  if (storage == PM_SGM_COMPACT || storage == PM_SGM_SWEEP) {
+    // every keyframe's volumes start 16-byte aligned (8 uint16); the sweep form keeps 3
+    const size_t slots = storage == PM_SGM_SWEEP ? 3 : (size_t)b->paths;
     auto span = [&](const KfDesc& d) {
-      return ((size_t)b->paths * d.c.W * d.c.H * d.c.D + 7) / 8 * 8;
+      return (slots * d.c.W * d.c.H * d.c.D + 7) / 8 * 8;
     };
     size_t total = 0;
     for (const KfDesc& d : b->host) total += span(d);
@@ -332,6 +369,18 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     size_t off = 0;
     for (KfDesc& d : b->host) {
       d.ebuf = e == hipSuccess ? b->d_ebuf + off : nullptr;
+      off += span(d);
+    }
+  }
+  if (e == hipSuccess && storage == PM_SGM_SWEEP) {
+    // two halves of 3 x [W][D] row-boundary states per keyframe (16-byte aligned)
+    auto span = [&](const KfDesc& d) { return ((size_t)6 * d.c.W * d.c.D + 3) / 4 * 4; };
+    size_t total = 0;
+    for (const KfDesc& d : b->host) total += span(d);
+    e = hipMalloc(&b->d_sweep, total * sizeof(float));
+    size_t off = 0;
+    for (KfDesc& d : b->host) {
+      d.sweep = e == hipSuccess ? b->d_sweep + off : nullptr;
       off += span(d);
     }
   }
@@ -356,6 +405,7 @@ extern "C" void pm_batch_destroy(pm_batch* b) {
   if (b->d_kfs) (void)hipFree(b->d_kfs);
   if (b->d_pool) (void)hipFree(b->d_pool);
   if (b->d_ebuf) (void)hipFree(b->d_ebuf);
+  if (b->d_sweep) (void)hipFree(b->d_sweep);
   delete b;
 }
 

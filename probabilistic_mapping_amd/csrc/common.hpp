@@ -74,10 +74,21 @@ struct KfDesc {
   float P1n, P2n, P1s, P2s;
   int tau_i, paths;
   float var_scale;
-  uint16_t* ebuf;    // compact form: paths x [H][W][D] excess volumes, else null
+  uint16_t* ebuf;    // compact form: paths x [H][W][D] excess volumes; sweep form: 3 volumes
   float estep;       // excess_step(P2n, P2s)
   float einv;        // 1 / estep (0 when estep is 0)
+  float* sweep;      // sweep form: two halves of 3 x [W][D] row-boundary path states
 };
+
+// Sweep form of the SGM pass (batched): row blocks of kSweepRows rows per launch; a workgroup
+// of kSweepWaves waves owns kSweepBand columns and recomputes kSweepRows - 1 halo columns on
+// each side at a block's first row, shrinking by one per row (the diagonal paths' reach).
+constexpr int kSweepRows = 16;
+constexpr int kSweepBand = 64;
+constexpr int kSweepWaves = 16;
+constexpr int kSweepCols = (kSweepBand + 2 * (kSweepRows - 1) + kSweepWaves - 1) / kSweepWaves;
+__host__ __device__ inline int sweep_bands(int W) { return (W + kSweepBand - 1) / kSweepBand; }
+__host__ __device__ inline int sweep_blocks(int H) { return (H + kSweepRows - 1) / kSweepRows; }
 
 __host__ __device__ inline int sgm_lines(int W, int H, int dx, int dy) {
   return (dx != 0 && dy != 0) ? W + H - 1 : (dx != 0 ? H : W);
@@ -117,6 +128,11 @@ void launch_combine_excess(pm_cost_type ct, int W, int H, int D, int paths, cons
                            float idep_min, float dep_sample, float* depth, hipStream_t st);
 void launch_combine_excess_batch(pm_cost_type ct, int LP, const KfDesc* kfs, const int* kf,
                                  const int* first, int n, int total, hipStream_t st);
+// Sweep form: row block rb of the downward (up = false) or upward sweep over the keyframes
+// kf[j] of one lane class, keyframe kf[j] owning workgroups first[j] .. first[j+1]-1.
+bool launch_sgm_sweep_batch(pm_cost_type ct, int nv, bool full, bool up, const KfDesc* kfs,
+                            const int* kf, const int* first, int n, int total, int rb,
+                            hipStream_t st);
 bool launch_sgm_excess_batch(pm_cost_type ct, int prefetch, int nv, bool full,
                              const KfDesc* kfs, const int* seg, const int* first, int nseg,
                              int total, int grid, hipStream_t st);

@@ -247,7 +247,8 @@ class KeyframeSequence:
         self.paths = paths
         self.cost_type = cost_type
         self.median = median
-        assert sgm_storage in ("exact", "compact"), sgm_storage
+        # "sweep" exists only in the batched form (BatchedSequence)
+        assert sgm_storage in ("exact", "compact", "sweep"), sgm_storage
         self.sgm_storage = sgm_storage
         self.write_agg = write_agg   # materialise the aggregate (the filter needs only depth)
         self.K = np.asarray(K, np.float64)
@@ -269,8 +270,10 @@ class KeyframeSequence:
                     if write_agg else None)
         if sgm_storage == "compact":
             self.sgm_ws = sgm_excess_workspace(H, W, D, paths, self.device)
-        else:
+        elif sgm_storage == "exact":
             self.sgm_ws = sgm_workspace(H, W, D, paths, self.device)
+        else:
+            self.sgm_ws = None
         self.dep = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.dep_raw = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.fused = torch.empty((H, W), dtype=torch.float32, device=self.device)
@@ -315,6 +318,8 @@ class KeyframeSequence:
 
     def sgm_stage(self, i: int, out: torch.Tensor) -> None:
         """SGM (every path at once) + filter of keyframe i's cost volume into ``out``."""
+        if self.sgm_storage == "sweep":
+            raise ValueError("the sweep SGM storage runs in the batched form only")
         if not hasattr(self, "_sgm_cache"):
             self._sgm_cache = {}
         key = (i, out.data_ptr(), self.cost.data_ptr())
@@ -389,7 +394,8 @@ class Batch:
         self.lib = C.load()
         self.cost_type = cost_type
         self.storage = storage
-        st = {"exact": C.PM_SGM_EXACT, "compact": C.PM_SGM_COMPACT}[storage]
+        st = {"exact": C.PM_SGM_EXACT, "compact": C.PM_SGM_COMPACT,
+              "sweep": C.PM_SGM_SWEEP}[storage]
         code, dt = COST_TYPES[cost_type]
         n = len(keyframes)
         self.kfs = keyframes

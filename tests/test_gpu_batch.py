@@ -110,6 +110,72 @@ def test_batch_autotune_and_graph_replay(uneven, storage):
         _assert_equal(batch, expect)
 
 
+def _excess_step(knobs):
+    p2 = max(np.float32(knobs["pi2"]), np.float32(knobs["pi2"]) * np.float32(knobs["sgm_q2"]))
+    return float(np.float32(p2) / np.float32(65534.0))
+
+
+@pytest.mark.parametrize("cost_type,paths", [("f16", 8), ("f32", 8), ("bf16", 4), ("f16", 4)])
+def test_batch_sweep_against_exact(uneven, cost_type, paths):
+    """Sweep SGM storage against the exact batch on the same keyframes (D = 32, 96, 128, 64:
+    both lane classes, full and partial).  Every path runs the same FP32 recurrence; the sweep
+    aggregate P C + step (q_LR + q_RL) + np step q_down + (upward excess) differs from the FP32
+    sum of the paths' L by the level rounding (step / 2 for each horizontal path, np step / 2
+    for the downward sum) and FP32 rounding of the sums.  Invalid pixels are 0 in both, and the
+    filter decisions agree."""
+    from probabilistic_mapping_amd import pipeline
+    ex = pipeline.Batch(uneven, cost_type, KNOBS, paths, "exact")
+    ex.run()
+    sw = pipeline.Batch(uneven, cost_type, KNOBS, paths, "sweep")
+    sw.run()
+    dn = pipeline.Batch(uneven, cost_type, KNOBS, paths, "sweep", write_agg=False)
+    dn.run()
+    torch.cuda.synchronize()
+    step = _excess_step(KNOBS)
+    n_down = 3 if paths == 8 else 1
+    for i, (a, b, c) in enumerate(zip(ex.outputs, sw.outputs, dn.outputs)):
+        assert torch.equal(a["cost"], b["cost"])
+        ge = a["agg"].cpu().numpy().astype(np.float64)
+        gs = b["agg"].cpu().numpy().astype(np.float64)
+        ulp = np.spacing(np.abs(ge).astype(np.float32)).astype(np.float64)
+        bound = (2 + n_down) * step / 2 + 4 * paths * ulp
+        err = np.abs(gs - ge)
+        _report(f"sweep vs exact kf{i} {cost_type} P={paths}",
+                {"max_abs": float(err.max()), "max_over_bound": float((err / bound).max())})
+        assert np.all(err <= bound), i
+        inval = a["cost"].float().cpu().numpy().min(axis=2) < 0
+        assert np.all(gs[inval] == 0.0)
+        st = parity.depth_parity(b["depth"].cpu().numpy(), a["depth"].cpu().numpy())
+        assert st["category_mismatch"] <= 1e-3, (i, st)
+        assert st["rel"]["p999"] <= 1e-4, (i, st)
+        assert torch.equal(b["depth"], c["depth"])   # depth-only run: same depth
+
+
+def test_batch_sweep_graph_replay(uneven):
+    """The sweep form's ~2 x (row blocks) launches captured in one hipGraph replay to the
+    eager run's bits."""
+    from probabilistic_mapping_amd import pipeline
+    batch = pipeline.Batch(uneven, "f16", KNOBS, 8, "sweep")
+    batch.run()
+    torch.cuda.synchronize()
+    expect = [{k: v.clone() for k, v in o.items()} for o in batch.outputs]
+    best = batch.autotune(reps=1)
+    assert best == batch.tuning
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        batch.capture()
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for o in batch.outputs:
+            for v in o.values():
+                v.fill_(rep + 1)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            batch.launch()
+        side.synchronize()
+        _assert_equal(batch, expect)
+
+
 def test_batch_rejects_bad_input(uneven):
     from probabilistic_mapping_amd import _capi as C, pipeline
     bad = dict(uneven[0], planes=0)
@@ -124,6 +190,8 @@ def test_batch_rejects_bad_input(uneven):
         b.tuning = dict(cost_tile_rows=8, cost_stage_planes=16)
     with pytest.raises(C.PmError):
         b.launch()   # nothing captured
+    with pytest.raises(C.PmError):   # the sweep form takes at most 128 planes
+        pipeline.Batch([dict(uneven[0], planes=129)], "f16", KNOBS, 8, "sweep")
 
 
 def test_batch_max_sources_tile_rows(uneven):
