@@ -51,8 +51,9 @@ def parse():
     ap.add_argument("--cost-type", default="f32", choices=["f32", "f16", "bf16"],
                     help="cost storage of the plain / batched form (both: of the plain form)")
     ap.add_argument("--form", default="both", choices=["both", "plain", "batched", "diff"])
-    ap.add_argument("--sgm-storage", default=None, choices=["exact", "compact"],
-                    help="SGM per-path storage (default: exact for f32 cost, compact otherwise)")
+    ap.add_argument("--sgm-storage", default=None, choices=["exact", "compact", "sweep"],
+                    help="SGM per-path storage (default: exact for f32 cost, compact otherwise; "
+                         "sweep exists in the batched form only)")
     ap.add_argument("--median", type=int, default=0)
     ap.add_argument("--cpu-baseline-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample (0 disables it)")
@@ -381,6 +382,10 @@ def main():
     s = synth.trajectory_scene(args.seed)
     copy = attainable_copy(torch, args.copy_mib)
     storage = default_storage(args.cost_type, args.sgm_storage)
+    if storage == "sweep" and args.form != "batched":
+        print(json.dumps({"error": "the sweep SGM storage runs in the batched form only "
+                                   "(--form batched)"}))
+        sys.exit(2)
     extra = None
     if args.form == "batched":
         main_r = measure_batched(args, torch, s, args.cost_type, storage)
