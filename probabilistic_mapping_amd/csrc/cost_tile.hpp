@@ -41,32 +41,17 @@ __device__ __forceinline__ float bilinear(uint2 t, float ax, float ay) {
   return fmaf(ay, dv, top);
 }
 
-// Bilinear sample of the quad texture at the warped point (U, V) with the warp's validity
-// (calc_cost.cu:85-93); returns +inf for an invalid warp.  Validity is "the clamp to the image
-// changes nothing" and w >= 0, so a NaN warp (0/0) is invalid; an invalid warp still reads an
-// in-bounds texel (the clamped position), so every gather of a row can be in flight at once.
-__device__ __forceinline__ float sample_uv(const QuadTex& quad, uint32_t W, float Wm1,
-                                           float Hm1, float U, float V, float w) {
-  const float Uc = __builtin_amdgcn_fmed3f(U, 0.0f, Wm1);
-  const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
-  const bool valid = (w >= 0.0f) & (Uc == U) & (Vc == V);
-  // non-negative, so truncation is floor; W, y <= 65535 fit the 24-bit multiply
-  const uint2 t = quad.at(__umul24((uint32_t)Vc, W) + (uint32_t)Uc);
-  const float s = bilinear(t, __builtin_amdgcn_fractf(Uc), __builtin_amdgcn_fractf(Vc));
-  return valid ? s : __uint_as_float(0x7F800000u);
-}
-
-// sample_uv for a point known to be valid and inside the image (see tile_inside_mask): the
-// same arithmetic without the clamp and the validity test, so it returns the same value.
-__device__ __forceinline__ float sample_uv_inside(const QuadTex& quad, uint32_t W, float U,
-                                                  float V) {
-  const uint2 t = quad.at(__umul24((uint32_t)V, W) + (uint32_t)U);
-  return bilinear(t, __builtin_amdgcn_fractf(U), __builtin_amdgcn_fractf(V));
-}
-
-// A gather in flight: the texel of the cell of (U, V), its bilinear weights and, for the
-// checked form, the warp's validity.  gather_uv issues the load; gather_value consumes it, so
-// a caller can put independent work between the two.
+// Bilinear sample of the quad texture at the warped point (U, V), in two halves: gather_uv
+// issues the texel load, gather_value consumes it, so a caller can put independent work
+// between the two.
+// Checked form (INSIDE = false): the warp's validity (calc_cost.cu:85-93) is "the clamp to
+// the image changes nothing" and w >= 0, so a NaN warp (0/0) is invalid and gather_value
+// returns +inf for it; an invalid warp still reads an in-bounds texel (the clamped position),
+// so every gather of a row can be in flight at once.
+// Inside form (INSIDE = true): for a point known to be valid and inside the image (see
+// tile_inside_mask), the same arithmetic without the clamp and the validity test, so it
+// returns the same value.  Coordinates are non-negative there, so truncation is floor, and
+// W, y <= 65535 fit the 24-bit multiply.
 struct Gather {
   uint2 t;
   float ax, ay;
@@ -76,12 +61,12 @@ template <bool INSIDE>
 __device__ __forceinline__ Gather gather_uv(const QuadTex& quad, uint32_t W, float Wm1,
                                             float Hm1, float U, float V, float w) {
   Gather g;
-  if (INSIDE) {   // sample_uv_inside
+  if (INSIDE) {
     g.t = quad.at(__umul24((uint32_t)V, W) + (uint32_t)U);
     g.ax = __builtin_amdgcn_fractf(U);
     g.ay = __builtin_amdgcn_fractf(V);
     g.ok = true;
-  } else {        // sample_uv
+  } else {
     const float Uc = __builtin_amdgcn_fmed3f(U, 0.0f, Wm1);
     const float Vc = __builtin_amdgcn_fmed3f(V, 0.0f, Hm1);
     g.ok = (w >= 0.0f) & (Uc == U) & (Vc == V);
@@ -97,13 +82,6 @@ __device__ __forceinline__ float gather_value(const Gather& g) {
   return (INSIDE || g.ok) ? s : __uint_as_float(0x7F800000u);
 }
 
-// Sample of the warp (X, Y, w): (U, V) = (X, Y) / w.
-__device__ __forceinline__ float warp_sample(const QuadTex& quad, uint32_t W,
-                                             float Wm1, float Hm1, float X, float Y, float w) {
-  const float rw = __builtin_amdgcn_rcpf(w);
-  return sample_uv(quad, W, Wm1, Hm1, X * rw, Y * rw, w);
-}
-
 // Conservative whole-tile validity, per source.  Every warp a cost_tile wave evaluates for one
 // (plane, source) is a linear-fractional map of the tile's pixel coordinates: the centre warps
 // H (u, r, 1) + t idep over the rectangle u in [u_lo, u_hi], r in [r_lo, r_hi], and the two
@@ -112,7 +90,7 @@ __device__ __forceinline__ float warp_sample(const QuadTex& quad, uint32_t W,
 // take their extremes at the corners; so if every corner has w >= kInsideW and U, V inside the
 // image by kInsideMargin pixels, every warp of the tile is valid with room to spare for FP32
 // rounding (< 4e-3 px at w >= kInsideW for |coordinates| < 2^16).  Then the validity tests can
-// be skipped: sample_uv_inside returns what sample_uv would.
+// be skipped: gather_uv<true> returns what gather_uv<false> would.
 // Lane l tests corner l & 7 of source l >> 3 (sources 0..7); tile_inside_mask returns a
 // wave-uniform mask whose bit s says source s passed for the plane at idep.
 constexpr float kInsideW = 0.25f;
@@ -173,9 +151,9 @@ __device__ __forceinline__ uint32_t tile_inside_mask(const InsideCorner& c, floa
   return m;
 }
 
-// warp_sample with the sample's derivatives with respect to the warped point (X, Y, w):
-// S = B(X/w, Y/w), dS/dX = a, dS/dY = b, dS/dw = c.  The value is computed exactly as
-// warp_sample computes it.  Inside a texel cell the bilinear slopes are
+// Checked sample of the warp (X, Y, w), (U, V) = (X, Y) / w, with its derivatives with
+// respect to the warped point: S = B(X/w, Y/w), dS/dX = a, dS/dY = b, dS/dw = c.  The value
+// is computed as gather_uv<false> / gather_value compute it.  Inside a texel cell the bilinear slopes are
 // dB/dU = bx + ay bxy and dB/dV = by + ax bxy (the cell of floor(U), floor(V)); an invalid warp
 // returns +inf and zero derivatives.
 struct SampleGrad {
