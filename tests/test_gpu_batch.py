@@ -151,6 +151,37 @@ def test_batch_sweep_against_exact(uneven, cost_type, paths):
         assert torch.equal(b["depth"], c["depth"])   # depth-only run: same depth
 
 
+def test_batch_sweep_small_keyframes():
+    """Sweep storage on keyframes narrower than one band and shorter than one row block (and
+    one just past each), where the halo runs off both image edges."""
+    from probabilistic_mapping_amd import pipeline, synth
+    specs = []
+    for seed, (W, H, D) in enumerate([(40, 12, 32), (70, 17, 64), (65, 33, 40), (130, 16, 64)]):
+        sc = synth.trajectory_scene(seed + 31, width=W, height=H, n_frames=3, n_sources=2,
+                                    planes=D, fx=0.6 * W)
+        K = sc["scene"].K
+        imgs = [torch.from_numpy(im).cuda() for im in sc["images"]]
+        quads = [pipeline.make_quad(im) for im in imgs]
+        homs = [np_homography(K, *sc["poses"][0], *sc["poses"][j]) for j in sc["sources"][0]]
+        specs.append({"ref": imgs[0], "quads": [quads[j] for j in sc["sources"][0]],
+                      "homs": homs, "planes": D, "idep_min": sc["idep_min"],
+                      "dep_sample": (sc["idep_max"] - sc["idep_min"]) / (D - 1)})
+    ex = pipeline.Batch(specs, "f16", KNOBS, 8, "exact")
+    ex.run()
+    sw = pipeline.Batch(specs, "f16", KNOBS, 8, "sweep")
+    sw.run()
+    torch.cuda.synchronize()
+    bound_step = (2 + 3) * _excess_step(KNOBS) / 2
+    for i, (a, b) in enumerate(zip(ex.outputs, sw.outputs)):
+        ge = a["agg"].cpu().numpy().astype(np.float64)
+        gs = b["agg"].cpu().numpy().astype(np.float64)
+        ulp = np.spacing(np.abs(ge).astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(gs - ge) <= bound_step + 32 * ulp), i
+        st = parity.depth_parity(b["depth"].cpu().numpy(), a["depth"].cpu().numpy())
+        n = ge.shape[0] * ge.shape[1]
+        assert st["category_mismatch"] * n <= max(3, 1e-3 * n), (i, st)   # a few ties
+
+
 def test_batch_sweep_graph_replay(uneven):
     """The sweep form's ~2 x (row blocks) launches captured in one hipGraph replay to the
     eager run's bits."""
