@@ -289,6 +289,40 @@ pm_status pm_mapper_download(pm_mapper* m, const char* name, void* host, size_t 
  * initReference. */
 pm_status pm_mapper_set_cost_type(pm_mapper* m, pm_cost_type type);
 
+/* ---- keyframe / source selection driver (evaluate.cc:137-276, processImg) ---------------
+ * The reference's ROS node without ROS (C++: include/pm/frame_driver.h).  Images and poses are
+ * grabbed in arrival order (GrabImage / GrabPose, evaluate.cc:75-110); each process call is
+ * one processImg step: the newest image as the reference, its pose by exact timestamp, and
+ * agg_num measurement frames every interval-th image further back, with the poses as many
+ * places further down the pose list; then StereoMapper initReference / update /
+ * outputFusionCPU (CASE as set by pm_set_knobs). */
+typedef struct pm_driver pm_driver;
+typedef struct pm_driver_config {
+  int32_t agg_num;    /* aggNum, evaluate.cc:24 (3) */
+  int32_t interval;   /* interval, evaluate.cc:25 (6) */
+  int32_t planes;     /* DEP_CNT (64) */
+  double fx, fy, cx, cy;   /* evaluate.cc:37-40 */
+  float dep_sample;   /* evaluate.cc:347: 1 / (0.15 * 160) */
+} pm_driver_config;
+/* with_mapper = 0 only plans the steps (host only: no device memory, no GPU work). */
+pm_status pm_driver_create(const pm_driver_config* cfg, int32_t with_mapper, pm_driver** out);
+void pm_driver_destroy(pm_driver* d);
+/* GrabImage: an 8-bit grayscale width x height host image (copied). */
+pm_status pm_driver_grab_image(pm_driver* d, int64_t stamp_ns, const uint8_t* image,
+                               int32_t width, int32_t height);
+/* GrabPose: pose = (tx, ty, tz, qx, qy, qz, qw), camera -> world. */
+pm_status pm_driver_grab_pose(pm_driver* d, int64_t stamp_ns, const double pose[7]);
+/* One processImg step.  *ran = 1 if it ran; then stamp_ns, ref_image and src_images[agg_num]
+ * (grab-order indices, each may be NULL) describe it, and depth (host float [H][W], NULL for
+ * a planning driver) receives the 32FC1 result. */
+pm_status pm_driver_process(pm_driver* d, int32_t* ran, int64_t* stamp_ns, int32_t* ref_image,
+                            int32_t* src_images, float* depth, size_t depth_bytes);
+/* Steps not run because the sources' poses were not buffered yet (the reference reads past
+ * the end of its pose list there). */
+pm_status pm_driver_skipped(const pm_driver* d, int32_t* skipped);
+/* Eigen's Quaterniond(w, x, y, z).toRotationMatrix(), q = (x, y, z, w); R row-major. */
+pm_status pm_quaternion_to_rotation(const double q[4], double R[9]);
+
 /* ---- knobs (motion_stereo/parameters.cpp:10-18, plus additive ones) ------------------- */
 typedef struct pm_knobs {
   float pi1, pi2, tau_so, sgm_q1, sgm_q2, var_scale;

@@ -93,6 +93,12 @@ class DiffParams(ctypes.Structure):
                 ("recompute", ctypes.c_int32)]
 
 
+class DriverConfig(ctypes.Structure):
+    _fields_ = [("agg_num", ctypes.c_int32), ("interval", ctypes.c_int32),
+                ("planes", ctypes.c_int32), ("fx", ctypes.c_double), ("fy", ctypes.c_double),
+                ("cx", ctypes.c_double), ("cy", ctypes.c_double), ("dep_sample", ctypes.c_float)]
+
+
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -156,6 +162,15 @@ SIGNATURES = {
     "pm_diff_download": (_i32, [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]),
     "pm_se3_homography": (_i32, [_dp, _dp, _dp, _dp, _dp, _dp]),
     "pm_se3_pose_grad": (_i32, [_dp, _dp, _dp, _dp, _dp, _dp]),
+    "pm_driver_create": (_i32, [ctypes.POINTER(DriverConfig), _i32, ctypes.POINTER(_vp)]),
+    "pm_driver_destroy": (None, [_vp]),
+    "pm_driver_grab_image": (_i32, [_vp, ctypes.c_int64, _vp, _i32, _i32]),
+    "pm_driver_grab_pose": (_i32, [_vp, ctypes.c_int64, _dp]),
+    "pm_driver_process": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp,
+                                 ctypes.c_size_t]),
+    "pm_driver_skipped": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "pm_quaternion_to_rotation": (_i32, [_dp, _dp]),
 }
 
 _lib = None

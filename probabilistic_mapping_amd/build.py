@@ -20,7 +20,7 @@ SYNTH_LIB = os.path.join(HERE, "libpm_synth.so")
 
 ARCH = "gfx950"
 HIP_SOURCES = ["cost.hip", "sgm.hip", "sweep.hip", "filter.hip", "fusion.hip", "batch.hip", "diff.hip", "capi.hip", "se3.cpp",
-               "stereo_mapper.cpp", "parameters.cpp"]
+               "stereo_mapper.cpp", "parameters.cpp", "frame_driver.cpp"]
 HIP_HEADERS = ["common.hpp", "cost_tile.hpp", "sgm_scan.hpp"]
 
 # Packed-FP32 VALU forms are disabled for every device compile.
@@ -70,11 +70,25 @@ def build_hip(jobs: int = 8) -> str:
     return HIP_LIB
 
 
+EVALUATE = os.path.join(HERE, "pm_evaluate")
+
+
+def build_evaluate() -> str:
+    """The file-based driver executable, linked against the in-tree library."""
+    _run([_hipcc(), "-O2", "-std=c++17", "-I", INCLUDE, "-I", CSRC,
+          os.path.join(CSRC, "pm_evaluate.cpp"), "-L", HERE, "-lpm_hip",
+          "-Wl,-rpath,$ORIGIN", "-o", EVALUATE + ".tmp"])
+    os.replace(EVALUATE + ".tmp", EVALUATE)
+    return EVALUATE
+
+
 def build_all() -> None:
     build_synth()
     build_hip()
+    build_evaluate()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"all": build_all, "hip": build_hip, "synth": build_synth}[what]()
+    {"all": build_all, "hip": build_hip, "synth": build_synth,
+     "evaluate": build_evaluate}[what]()
