@@ -68,6 +68,10 @@ class StereoMapper {
   DeviceMat alpha0, beta0, mu0, sigma0;
   DeviceMat alpha1, beta1, mu1, sigma1;
   DeviceMat fuseDepth;                     // fused depth [H][W]
+  // state and table of outputFusionCPU's CASE >= 3 step (the reference's CPU rules keep their
+  // own filter state, stereo_mapper.cpp:6-16); allocated at its first use
+  DeviceMat cpu_alpha[2], cpu_beta[2], cpu_mu[2], cpu_sigma[2];
+  DeviceMat cpu_table;                     // int32 [H][W], (y << 16) | x
 
   int pre_index, cur_index;
   bool firstDepth;
@@ -83,6 +87,8 @@ class StereoMapper {
   void release();
   void ensure_cost();
   void run_sgm_filter();
+  void fuse_cpu_rules(Mat& result, Matrix3d R, Vector3d t, float fx, float fy, float cx,
+                      float cy);
 
   int W_, H_, D_;
   pm_cost_type cost_type_ = PM_COST_F32;
@@ -92,6 +98,7 @@ class StereoMapper {
   std::vector<pm_homography> homs_;
   DeviceMat zbuf_, med_tmp_, wta_tmp_;
   DeviceMat sgm_ws_;                       // per-path L volumes (pm_sgm_filter_paths)
+  DeviceMat cpu_work_, cpu_result_;        // outputFusionCPU's CASE >= 3 step
   bool cost_ready_ = false;
   bool depth_ready_ = false;
   bool have_reference_ = false;

@@ -185,6 +185,23 @@ pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam, const floa
                         const float t_p2c[3], float* depth, uint64_t* zbuf, int32_t* table,
                         pm_fusion_state pre, pm_fusion_state cur, pm_stream stream);
 
+/* The fusion step of outputFusionCPU with CASE >= 3 (stereo_mapper.cpp:719-800): the
+ * reference's CPU rules (depthFuseCPU, :246-587) on the GPU -- propagation gate 0.5, a 7x7
+ * hole fill, tau^2 = 64, initial variance 256, outlier density 1/16, beta + 1 on a missing
+ * measurement, output gated on the posterior inlier ratio > 0.51 -- with the measurement
+ * clamps (> 100: dropped and masked, > 10: dropped) and output masks (< 0.1 -> 0).  The
+ * raster-order z-buffer and hole fill of the reference are reproduced exactly.
+ *   depth  : [H][W] filtered depth in, the result out
+ *   table  : optional [H][W] int32 output of the filled table, (y << 16) | x, 0 = empty
+ *   work   : device scratch of pm_depth_fuse_cpu_rules_workspace_size bytes
+ *   r_p2c, t_p2c : previous->current camera transform (:778-779), row-major, double */
+pm_status pm_depth_fuse_cpu_rules_workspace_size(int32_t width, int32_t height,
+                                                 uint64_t* bytes);
+pm_status pm_depth_fuse_cpu_rules(int32_t width, int32_t height, pm_camera cam,
+                                  const double r_p2c[9], const double t_p2c[3], float* depth,
+                                  int32_t* table, void* work, uint64_t work_bytes,
+                                  pm_fusion_state pre, pm_fusion_state cur, pm_stream stream);
+
 /* Fills n floats with value (used for the state resets of outputFusion,
  * stereo_mapper.cpp:143-144). */
 pm_status pm_fill_f32(float* dst, size_t n, float value, pm_stream stream);
@@ -280,9 +297,17 @@ pm_status pm_mapper_output_fusion(pm_mapper* m, float* result, const double R[9]
 /* StereoMapper::outputFusionCPU CASE modes 0-2, stereo_mapper.cpp:625-719 (WTA on the raw
  * cost, WTA on the aggregate, filtered depth); result: host float [H][W]. */
 pm_status pm_mapper_output_case(pm_mapper* m, int32_t case_mode, float* result);
+/* StereoMapper::outputFusionCPU, stereo_mapper.cpp:625-800, dispatching on the CASE knob:
+ * modes 0-2 as pm_mapper_output_case; CASE >= 3 runs the reference's CPU fusion rules
+ * (pm_depth_fuse_cpu_rules) on the mapper's own state, and result: host float [H][W] is the
+ * fused depth (zeros on the first call, which only resets). */
+pm_status pm_mapper_output_fusion_cpu(pm_mapper* m, float* result, const double R[9],
+                                      const double t[3], float fx, float fy, float cx, float cy);
 /* Copies a member buffer to host memory.  Names: "raw_cost", "sgm_cost" (float [H][W][D],
  * raw_cost converted from its storage type), "dep", "fuse_depth", "table" (int32),
  * "alpha", "beta", "mu", "sig" (the current state after the last fusion step),
+ * "cpu_alpha", "cpu_beta", "cpu_mu", "cpu_sig", "cpu_table" (the same for outputFusionCPU's
+ * CASE >= 3 step; the table is (y << 16) | x),
  * "img_l" (uint8).  bytes must equal the buffer size. */
 pm_status pm_mapper_download(pm_mapper* m, const char* name, void* host, size_t bytes);
 /* Storage type of the cost volume (default PM_COST_F32).  Takes effect at the next

@@ -161,12 +161,23 @@ def median3x3(depth, precision="f64"):
     return out
 
 
+def depth_fuse_cpu_rules(depth, pre, cur, r_p2c, t_p2c, intr, precision="f64"):
+    """The fusion step of outputFusionCPU with CASE >= 3 (stereo_mapper.cpp:719-800), in place
+    like :func:`depth_fuse`: ``depth`` is the filtered depth in and the result out.  Returns
+    the filled propagation table (int32, (y << 16) | x of the source, 0 = empty)."""
+    return _fuse(depth, pre, cur, r_p2c, t_p2c, intr, precision, "oracle_fuse_cpu_rules")
+
+
 def depth_fuse(depth, pre, cur, r_p2c, t_p2c, intr, precision="f64"):
     """One Gaussian x Beta fusion step, in place on ``depth``, ``pre`` and ``cur``.
 
     ``pre`` / ``cur`` are dicts of ``alpha, beta, mu, sig`` ``[H][W]`` arrays of the chosen
     precision.  Returns the filled propagation table (int32, 0 = empty).
     """
+    return _fuse(depth, pre, cur, r_p2c, t_p2c, intr, precision, "oracle_fuse")
+
+
+def _fuse(depth, pre, cur, r_p2c, t_p2c, intr, precision, name):
     dt, ct = _dtype(precision)
     H, W = depth.shape
     for a in [depth] + [pre[k] for k in "alpha beta mu sig".split()] + \
@@ -179,7 +190,7 @@ def depth_fuse(depth, pre, cur, r_p2c, t_p2c, intr, precision="f64"):
     intr = np.ascontiguousarray(np.asarray(intr, np.float64).reshape(4))
     r = np.ascontiguousarray(np.asarray(r_p2c, np.float64).reshape(9))
     t = np.ascontiguousarray(np.asarray(t_p2c, np.float64).reshape(3))
-    fn = lib().oracle_fuse_f64 if precision == "f64" else lib().oracle_fuse_f32
+    fn = getattr(lib(), f"{name}_{precision}")
     fn(ctypes.c_int(W), ctypes.c_int(H), _ptr(intr, ctypes.c_double),
        _ptr(r, ctypes.c_double), _ptr(t, ctypes.c_double), _ptr(depth, ct),
        _ptr(table, ctypes.c_int32), pre_arr, cur_arr)

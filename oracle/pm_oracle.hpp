@@ -498,4 +498,135 @@ void depth_fuse(int W, int H, T fx, T fy, T cx, T cy, const double* r, const dou
   }
 }
 
+// normpdfCPU, stereo_mapper.cpp:434-437: it MULTIPLIES by sqrt(2 pi sigma^2), where the GPU
+// normpdf divides (depth_fusion.cu:5).
+template <typename T>
+inline T normpdf_cpu(T x, T mu, T sigma_sq) {
+  return std::exp(-(x - mu) * (x - mu) / (T(2) * sigma_sq)) *
+         std::sqrt(T(2) * T(M_PI) * sigma_sq);
+}
+
+// The fusion step of outputFusionCPU with CASE >= 3 (stereo_mapper.cpp:719-800), i.e. the
+// reference's CPU rules: the measurement clamps (:737-757), depthFuseCPU (:565-587) =
+// transformMapCPU (:246-326), holeFillingCPU (:328-388, 7x7), depthPreditCPU (:390-431),
+// depthUpdateCPU (:439-545, tau^2 = 64, init sigma^2 = 256, outlier density 1/16, beta += 1 on
+// a missing measurement, posterior gate 0.51), then the output masks (:765-775).  Everything
+// runs in raster order exactly as the reference's loops do.
+//   depth : [N] filtered depth in, fused result out
+//   table : [N] filled propagation table, (y << 16) | x of the source, 0 = empty
+//   pre   : previous state (transformMapCPU rewrites mu and sig in place)
+//   cur   : current state (written)
+template <typename T>
+void depth_fuse_cpu_rules(int W, int H, T fx, T fy, T cx, T cy, const double* r,
+                          const double* t, T* depth, int32_t* table, State<T> pre,
+                          State<T> cur) {
+  const long N = (long)W * H;
+  // measurement clamps (:742-757): above 100 -> 0 and masked, above 10 -> 0
+  std::vector<char> mask(N, 0);
+  for (long j = 0; j < N; ++j) {
+    if (depth[j] > T(100)) { depth[j] = T(0); mask[j] = 1; }
+    if (depth[j] > T(10)) depth[j] = T(0);
+  }
+  // transformMapCPU
+  std::vector<uint32_t> tab(N, 0u);
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const long i = (long)y * W + x;
+      const T a = pre.alpha[i];
+      if (a < T(kFloatEps)) continue;                            // :275-278
+      const T b = pre.beta[i];
+      if (double(a / (a + b)) < 0.5) continue;                   // :279-282
+      const T dep = pre.mu[i];
+      const double p0 = double((T(x) - cx) / fx * dep), p1 = double((T(y) - cy) / fy * dep);
+      const double p2v = double(dep);
+      const double q0 = r[0] * p0 + r[1] * p1 + r[2] * p2v + t[0];   // :291-292
+      const double q1 = r[3] * p0 + r[4] * p1 + r[5] * p2v + t[1];
+      const double q2 = r[6] * p0 + r[7] * p1 + r[8] * p2v + t[2];
+      if (q2 < kFloatEps) continue;                               // :293-299
+      const int x2 = int(q0 / q2 * double(fx) + double(cx) + 0.5);   // :300-301, truncation
+      const int y2 = int(q1 / q2 * double(fy) + double(cy) + 0.5);
+      if (x2 < 0 || x2 >= W || y2 < 0 || y2 >= H) continue;     // :302-308
+      pre.mu[i] = T(q2);                                          // :310-311
+      pre.sig[i] = T(double(pre.sig[i]) + q2 * 0.05);
+      uint32_t& e = tab[(long)y2 * W + x2];
+      const uint32_t me = ((uint32_t)y << 16) | (uint32_t)x;
+      if (e == 0) {                                               // :312-314
+        e = me;
+      } else {                                                    // :315-325
+        const long now = (long)(e >> 16) * W + (long)(e & 0xFFFFu);
+        if (pre.mu[now] <= dep) continue;
+        pre.mu[now] = dep;
+        e = me;
+      }
+    }
+  // holeFillingCPU, expand_sz = 3: sources visited in raster order fill the empty cells of
+  // their 7x7 window that nobody filled yet
+  const int ex = 3;
+  std::vector<char> vst(N, 0);
+  for (int y = ex; y < H - ex; ++y)
+    for (int x = ex; x < W - ex; ++x) {
+      const long i = (long)y * W + x;
+      if (vst[i] || tab[i] == 0) continue;
+      for (int dy = -ex; dy <= ex; ++dy)
+        for (int dx = -ex; dx <= ex; ++dx) {
+          const long k = (long)(y + dy) * W + (x + dx);
+          if (tab[k] == 0 && !vst[k]) { tab[k] = tab[i]; vst[k] = 1; }
+        }
+    }
+  // depthPreditCPU: no source -> alpha 0 only (the rest keeps stale values)
+  for (long j = 0; j < N; ++j) {
+    table[j] = (int32_t)tab[j];
+    if (tab[j] == 0) {
+      cur.alpha[j] = T(0);
+    } else {
+      const long s = (long)(tab[j] >> 16) * W + (long)(tab[j] & 0xFFFFu);
+      cur.alpha[j] = pre.alpha[s]; cur.beta[j] = pre.beta[s];
+      cur.mu[j] = pre.mu[s]; cur.sig[j] = pre.sig[s];
+    }
+  }
+  // depthUpdateCPU with threshold 0.51 (a float, compared as double)
+  const double thr = double(0.51f);
+  const T unc = T(64);
+  for (long j = 0; j < N; ++j) {
+    const T d = depth[j];
+    if (cur.alpha[j] < T(kFloatEps)) {                           // :467-494
+      if (d < T(kFloatEps)) {
+        depth[j] = T(0);
+        cur.mu[j] = d;
+      } else {
+        depth[j] = T(0);
+        cur.alpha[j] = T(10); cur.beta[j] = T(10); cur.mu[j] = d; cur.sig[j] = T(256);
+      }
+      continue;
+    }
+    if (d < T(kFloatEps)) {                                       // :496-518
+      cur.beta[j] = cur.beta[j] + T(1);
+      depth[j] = double(cur.alpha[j] / (cur.alpha[j] + cur.beta[j])) > thr ? cur.mu[j] : T(0);
+      continue;
+    }
+    const T a = cur.alpha[j], b = cur.beta[j], miu = cur.mu[j], sq = cur.sig[j];   // :521-545
+    const T new_sq = unc * sq / (unc + sq);
+    const T new_miu = (d * sq + miu * unc) / (unc + sq);
+    T c1 = (a / (a + b)) * normpdf_cpu<T>(d, miu, unc + sq);
+    T c2 = (b / (a + b)) * T(1) / T(16);
+    const T nc = c1 + c2;
+    c1 = c1 / nc;
+    c2 = c2 / nc;
+    const T f = c1 * ((a + T(1)) / (a + b + T(1))) + c2 * (a / (a + b + T(1)));
+    const T e = c1 * (((a + T(1)) * (a + T(2))) / ((a + b + T(1)) * (a + b + T(2)))) +
+                c2 * (a * (a + T(1)) / ((a + b + T(1)) * (a + b + T(2))));
+    const T mu_p = c1 * new_miu + c2 * miu;
+    const T sig_p = c1 * (new_sq + new_miu * new_miu) + c2 * (sq + miu * miu) - mu_p * mu_p;
+    const T a_p = (e - f) / (f - e / f);
+    const T b_p = a_p * (T(1) - f) / f;
+    cur.alpha[j] = a_p; cur.beta[j] = b_p; cur.mu[j] = mu_p; cur.sig[j] = sig_p;
+    depth[j] = double(a_p / (a_p + b_p)) > thr ? mu_p : T(0);
+  }
+  // output masks (:765-775)
+  for (long j = 0; j < N; ++j) {
+    if (mask[j]) depth[j] = T(0);
+    if (depth[j] < T(0.1)) depth[j] = T(0);
+  }
+}
+
 }  // namespace pmo

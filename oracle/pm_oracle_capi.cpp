@@ -111,4 +111,20 @@ void oracle_fuse_f32(int W, int H, const double* intr, const double* r, const do
                          r, t, depth, table, p, c);
 }
 
+// The CPU-rules step of outputFusionCPU (CASE >= 3); same arguments as oracle_fuse_*.
+void oracle_fuse_cpu_rules_f64(int W, int H, const double* intr, const double* r,
+                               const double* t, double* depth, int32_t* table,
+                               double* const* pre, double* const* cur) {
+  pmo::State<double> p{pre[0], pre[1], pre[2], pre[3]}, c{cur[0], cur[1], cur[2], cur[3]};
+  pmo::depth_fuse_cpu_rules<double>(W, H, intr[0], intr[1], intr[2], intr[3], r, t, depth,
+                                    table, p, c);
+}
+void oracle_fuse_cpu_rules_f32(int W, int H, const double* intr, const double* r,
+                               const double* t, float* depth, int32_t* table,
+                               float* const* pre, float* const* cur) {
+  pmo::State<float> p{pre[0], pre[1], pre[2], pre[3]}, c{cur[0], cur[1], cur[2], cur[3]};
+  pmo::depth_fuse_cpu_rules<float>(W, H, float(intr[0]), float(intr[1]), float(intr[2]),
+                                   float(intr[3]), r, t, depth, table, p, c);
+}
+
 }  // extern "C"

@@ -148,6 +148,8 @@ SIGNATURES = {
     "pm_mapper_output": (_i32, [_vp, _vp]),
     "pm_mapper_output_fusion": (_i32, [_vp, _vp, _dp, _dp, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, ctypes.c_float]),
+    "pm_mapper_output_fusion_cpu": (_i32, [_vp, _vp, _dp, _dp, ctypes.c_float,
+                                           ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     "pm_mapper_output_case": (_i32, [_vp, _i32, _vp]),
     "pm_mapper_download": (_i32, [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]),
     "pm_mapper_set_cost_type": (_i32, [_vp, _i32]),
@@ -162,6 +164,10 @@ SIGNATURES = {
     "pm_diff_download": (_i32, [_vp, ctypes.c_char_p, _vp, ctypes.c_size_t]),
     "pm_se3_homography": (_i32, [_dp, _dp, _dp, _dp, _dp, _dp]),
     "pm_se3_pose_grad": (_i32, [_dp, _dp, _dp, _dp, _dp, _dp]),
+    "pm_depth_fuse_cpu_rules_workspace_size": (_i32, [_i32, _i32,
+                                                      ctypes.POINTER(ctypes.c_uint64)]),
+    "pm_depth_fuse_cpu_rules": (_i32, [_i32, _i32, Camera, _dp, _dp, _vp, _vp, _vp,
+                                       ctypes.c_uint64, FusionState, FusionState, _vp]),
     "pm_driver_create": (_i32, [ctypes.POINTER(DriverConfig), _i32, ctypes.POINTER(_vp)]),
     "pm_driver_destroy": (None, [_vp]),
     "pm_driver_grab_image": (_i32, [_vp, ctypes.c_int64, _vp, _i32, _i32]),

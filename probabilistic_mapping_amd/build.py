@@ -19,7 +19,7 @@ HIP_LIB = os.path.join(HERE, "libpm_hip.so")
 SYNTH_LIB = os.path.join(HERE, "libpm_synth.so")
 
 ARCH = "gfx950"
-HIP_SOURCES = ["cost.hip", "sgm.hip", "sweep.hip", "filter.hip", "fusion.hip", "batch.hip", "diff.hip", "capi.hip", "se3.cpp",
+HIP_SOURCES = ["cost.hip", "sgm.hip", "sweep.hip", "filter.hip", "fusion.hip", "fusion_cpu.hip", "batch.hip", "diff.hip", "capi.hip", "se3.cpp",
                "stereo_mapper.cpp", "parameters.cpp", "frame_driver.cpp"]
 HIP_HEADERS = ["common.hpp", "cost_tile.hpp", "sgm_scan.hpp"]
 

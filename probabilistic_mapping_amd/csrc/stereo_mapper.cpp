@@ -132,8 +132,12 @@ void StereoMapper::alloc() {
 void StereoMapper::release() {
   DeviceMat* all[] = {&img_l, &raw_cost, &sgm_cost, &dep, &fuseDepth, &propogate_table,
                       &alpha0, &beta0, &mu0, &sigma0, &alpha1, &beta1, &mu1, &sigma1,
-                      &zbuf_, &med_tmp_, &wta_tmp_, &sgm_ws_};
+                      &zbuf_, &med_tmp_, &wta_tmp_, &sgm_ws_, &cpu_table, &cpu_work_,
+                      &cpu_result_};
   for (DeviceMat* m : all) dfree(*m);
+  for (int h = 0; h < 2; ++h) {
+    dfree(cpu_alpha[h]); dfree(cpu_beta[h]); dfree(cpu_mu[h]); dfree(cpu_sigma[h]);
+  }
   for (DeviceMat& q : quads_) dfree(q);
   quads_.clear();
 }
@@ -323,7 +327,67 @@ void StereoMapper::wta(int case_mode, Mat& result) {
   wta_tmp_.download(result);
 }
 
-// stereo_mapper.cpp:625-719
+// stereo_mapper.cpp:719-800, the CASE >= 3 branch of outputFusionCPU: the CPU rules of
+// depthFuseCPU (:246-587), run on the GPU by pm_depth_fuse_cpu_rules.  The first keyframe
+// only resets (:725-741); the state flags and pose (firstDepth, pre_index, cur_index,
+// R_b_2_w, t_b_2_w) are the members outputFusion uses as well, as in the reference.
+void StereoMapper::fuse_cpu_rules(Mat& result, Matrix3d Rw, Vector3d tw, float fx, float fy,
+                                  float cx, float cy) {
+  run_sgm_filter();
+  hipStream_t st = (hipStream_t)stream_;
+  const size_t N = (size_t)W_ * H_;
+  if (!cpu_table.data) {
+    for (int h = 0; h < 2; ++h) {
+      cpu_alpha[h] = dalloc(H_, W_, CV_32F); cpu_beta[h] = dalloc(H_, W_, CV_32F);
+      cpu_mu[h] = dalloc(H_, W_, CV_32F); cpu_sigma[h] = dalloc(H_, W_, CV_32F);
+    }
+    cpu_table = dalloc(H_, W_, CV_32S);
+    cpu_result_ = dalloc(H_, W_, CV_32F);
+    uint64_t bytes = 0;
+    check(pm_depth_fuse_cpu_rules_workspace_size(W_, H_, &bytes));
+    if (bytes > (uint64_t)INT32_MAX)
+      throw Error(PM_ERR_INVALID_ARGUMENT, "outputFusionCPU: image too large for the fusion scratch");
+    cpu_work_ = dalloc(1, (int)bytes, CV_8U);
+    hip_check(hipMemsetAsync(cpu_table.data, 0, cpu_table.bytes(), st), "hipMemsetAsync");
+  }
+  if (firstDepth) {   // :725-741
+    firstDepth = false;
+    result.create(H_, W_, CV_32F);
+    result.setTo<float>(0.0f);
+    R_b_2_w = Rw;
+    t_b_2_w = tw;
+    pre_index = 0;
+    cur_index = 1;
+    for (int h = 0; h < 2; ++h)
+      for (DeviceMat* m : {&cpu_alpha[h], &cpu_beta[h], &cpu_mu[h], &cpu_sigma[h]})
+        check(pm_fill_f32(reinterpret_cast<float*>(m->data), N, 0.0f, stream_));
+    synchronize();
+    return;
+  }
+  hip_check(hipMemcpyAsync(cpu_result_.data, dep.data, dep.bytes(), hipMemcpyDeviceToDevice, st),
+            "copy depth");
+  const Matrix3d R_p_2_c = Rw.transpose() * R_b_2_w;   // :778-779, double
+  const Vector3d t_p_2_c = Rw.transpose() * (t_b_2_w - tw);
+  auto state = [&](int h) {
+    return pm_fusion_state{reinterpret_cast<float*>(cpu_alpha[h].data),
+                           reinterpret_cast<float*>(cpu_beta[h].data),
+                           reinterpret_cast<float*>(cpu_mu[h].data),
+                           reinterpret_cast<float*>(cpu_sigma[h].data)};
+  };
+  check(pm_depth_fuse_cpu_rules(W_, H_, pm_camera{fx, fy, cx, cy}, R_p_2_c.m, t_p_2_c.v,
+                                reinterpret_cast<float*>(cpu_result_.data),
+                                reinterpret_cast<int32_t*>(cpu_table.data), cpu_work_.data,
+                                cpu_work_.bytes(), state(pre_index), state(pre_index ^ 1),
+                                stream_));
+  synchronize();
+  cpu_result_.download(result);
+  R_b_2_w = Rw;           // :794-798
+  t_b_2_w = tw;
+  pre_index = cur_index;
+  cur_index ^= 1;
+}
+
+// stereo_mapper.cpp:625-800
 void StereoMapper::outputFusionCPU(Mat& result, Matrix3d Rw, Vector3d tw, float fx, float fy,
                                    float cx, float cy) {
   if (CASE == 0 || CASE == 1) {
@@ -334,7 +398,7 @@ void StereoMapper::outputFusionCPU(Mat& result, Matrix3d Rw, Vector3d tw, float 
     output(result);
     return;
   }
-  outputFusion(result, Rw, tw, fx, fy, cx, cy);
+  fuse_cpu_rules(result, Rw, tw, fx, fy, cx, cy);
 }
 
 }  // namespace pm
@@ -468,6 +532,20 @@ extern "C" pm_status pm_mapper_output_fusion(pm_mapper* m, float* result, const 
   });
 }
 
+extern "C" pm_status pm_mapper_output_fusion_cpu(pm_mapper* m, float* result, const double R[9],
+                                                 const double t[3], float fx, float fy,
+                                                 float cx, float cy) {
+  PM_MAPPER_CHECK(m);
+  if (!result || !R || !t) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null argument");
+  return guard([&] {
+    pm::Matrix3d Rm;
+    for (int k = 0; k < 9; ++k) Rm.m[k] = R[k];
+    pm::Vector3d tv(t[0], t[1], t[2]);
+    pm::Mat r(m->impl->height(), m->impl->width(), pm::CV_32F, result);
+    m->impl->outputFusionCPU(r, Rm, tv, fx, fy, cx, cy);
+  });
+}
+
 extern "C" pm_status pm_mapper_output_case(pm_mapper* m, int32_t case_mode, float* result) {
   PM_MAPPER_CHECK(m);
   if (!result) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null result");
@@ -512,6 +590,11 @@ extern "C" pm_status pm_mapper_download(pm_mapper* m, const char* name, void* ho
     else if (n == "beta1") src = &s.beta1;
     else if (n == "mu1") src = &s.mu1;
     else if (n == "sigma1") src = &s.sigma1;
+    else if (n == "cpu_alpha") src = &s.cpu_alpha[s.pre_index];
+    else if (n == "cpu_beta") src = &s.cpu_beta[s.pre_index];
+    else if (n == "cpu_mu") src = &s.cpu_mu[s.pre_index];
+    else if (n == "cpu_sig") src = &s.cpu_sigma[s.pre_index];
+    else if (n == "cpu_table") src = &s.cpu_table;
     else throw pm::Error(PM_ERR_INVALID_ARGUMENT, "unknown buffer name: " + n);
     s.synchronize();
     const size_t count = (size_t)src->rows * src->cols;

@@ -98,18 +98,20 @@ class StereoMapper:
         return r
 
     def outputFusionCPU(self, R, t, fx, fy, cx, cy) -> np.ndarray:
-        mode = get_knobs()["case_mode"]
-        if mode in (0, 1, 2):
-            r = self._result()
-            C.check(self._lib.pm_mapper_output_case(self._h, mode,
-                                                    r.ctypes.data_as(ctypes.c_void_p)))
-            return r
-        return self.outputFusion(R, t, fx, fy, cx, cy)
+        """Dispatches on the CASE knob (``case_mode``): 0-2 give WTA / filtered depths, 3 and
+        above the fused depth of the reference's CPU fusion rules."""
+        r = self._result()
+        (Ra, Rp), (ta, tp) = _d(R, 9), _d(t, 3)
+        C.check(self._lib.pm_mapper_output_fusion_cpu(self._h, r.ctypes.data_as(ctypes.c_void_p),
+                                                      Rp, tp, float(fx), float(fy), float(cx),
+                                                      float(cy)))
+        return r
 
     def download(self, name: str) -> np.ndarray:
         H, W, D = self.height, self.width, self.dep_cnt
         shapes = {"raw_cost": ((H, W, D), np.float32), "sgm_cost": ((H, W, D), np.float32),
-                  "table": ((H, W), np.int32), "img_l": ((H, W), np.uint8)}
+                  "table": ((H, W), np.int32), "cpu_table": ((H, W), np.int32),
+                  "img_l": ((H, W), np.uint8)}
         shape, dt = shapes.get(name, ((H, W), np.float32))
         out = np.empty(shape, dt)
         C.check(self._lib.pm_mapper_download(self._h, name.encode(),

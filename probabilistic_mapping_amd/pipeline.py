@@ -218,6 +218,26 @@ def depth_fuse(depth: torch.Tensor, zbuf: torch.Tensor, pre: dict, cur: dict, r_
                                    _ptr(zbuf), tab, _state(pre), _state(cur), _stream()))
 
 
+def depth_fuse_cpu_rules(depth: torch.Tensor, pre: dict, cur: dict, r_p2c, t_p2c, cam,
+                         table: torch.Tensor | None = None,
+                         work: torch.Tensor | None = None) -> torch.Tensor:
+    """outputFusionCPU's CASE >= 3 step (the reference's CPU rules), in place on ``depth``
+    (filtered depth in, result out), ``pre`` and ``cur``; r_p2c, t_p2c in double.  Returns the
+    scratch it used (pass it back to reuse it)."""
+    H, W = depth.shape
+    need = ctypes.c_uint64(0)
+    C.check(C.load().pm_depth_fuse_cpu_rules_workspace_size(W, H, ctypes.byref(need)))
+    if work is None or work.numel() < need.value:
+        work = torch.empty(need.value, dtype=torch.uint8, device=depth.device)
+    r = np.ascontiguousarray(np.asarray(r_p2c, np.float64).reshape(9))
+    t = np.ascontiguousarray(np.asarray(t_p2c, np.float64).reshape(3))
+    tab = _ptr(table) if table is not None else ctypes.c_void_p(0)
+    C.check(C.load().pm_depth_fuse_cpu_rules(
+        W, H, C.Camera(*[float(c) for c in cam]), C.dptr(r), C.dptr(t), _ptr(depth), tab,
+        _ptr(work), work.numel(), _state(pre), _state(cur), _stream()))
+    return work
+
+
 def pose_p2c(R, t, R_prev, t_prev):
     """R_p2c = R^T R_prev, t_p2c = R^T (t_prev - t) in FP64, cast to float
     (stereo_mapper.cpp:155-156, 167-170)."""

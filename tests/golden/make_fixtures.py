@@ -19,6 +19,11 @@ the mathematics fixes, so the oracle is checked against it rather than against i
       p(z, pi | d) ~ [pi N(d; z, tau^2) + (1 - pi) / 50] N(z; mu, s2) Beta(pi; a, b)
   computed by brute-force 2-D quadrature.  The Beta(a', b') matching E[pi], E[pi^2] and
   (mu', s2') = (E[z], Var[z]) are what depthUpdate must produce (depth_fusion.cu:195-216).
+* fusion_posterior_cpu.json -- the same quadrature for the CPU rules of outputFusionCPU
+  (stereo_mapper.cpp:439-545): tau^2 = 64, outlier density 1/16, and an inlier weight that
+  normpdfCPU MULTIPLIES by sqrt(2 pi (tau^2 + s2)) instead of dividing, i.e. the inlier
+  integral N(d; mu, tau^2 + s2) scaled by 2 pi (tau^2 + s2).  The moments are matched the same
+  way; the output gate is on the posterior inlier ratio (> 0.51).
 * hole_fill.json -- a hand-made 6x5 propagation table and its filled form under the
   deterministic rule (empty pixels take the entry of their lowest-linear-index non-empty
   interior 3x3 neighbour, read from the pre-fill table), applied by hand here.
@@ -126,6 +131,42 @@ def fusion_posterior():
     return out
 
 
+def fusion_posterior_cpu():
+    out = []
+    cases = [(10.0, 10.0, 2.0, 256.0, 2.5), (10.0, 10.0, 2.0, 4.0, 2.05), (25.0, 4.0, 3.1, 0.5, 3.0),
+             (9.0, 6.0, 1.2, 30.0, 7.5), (40.0, 2.0, 5.0, 1.0, 9.9)]
+    p = np.linspace(0, 1, 200001)[1:-1]
+    dp = p[1] - p[0]
+    for a, b, mu, s2, d in cases:
+        tau2 = 64.0
+        span = 12 * np.sqrt(max(s2, tau2))
+        z = np.linspace(mu - span, mu + span, 400001)
+        dz = z[1] - z[0]
+        gz = np.exp(-(z - mu) ** 2 / (2 * s2)) / np.sqrt(2 * np.pi * s2)
+        like = np.exp(-(d - z) ** 2 / (2 * tau2)) / np.sqrt(2 * np.pi * tau2)
+        kappa = 2 * np.pi * (tau2 + s2)           # normpdfCPU's sqrt factor, twice
+        A0 = kappa * np.sum(like * gz) * dz
+        A1 = kappa * np.sum(z * like * gz) * dz
+        A2 = kappa * np.sum(z * z * like * gz) * dz
+        B0 = np.sum(gz) * dz / 16.0
+        B1 = np.sum(z * gz) * dz / 16.0
+        B2 = np.sum(z * z * gz) * dz / 16.0
+        logbeta = (a - 1) * np.log(p) + (b - 1) * np.log(1 - p)
+        w = np.exp(logbeta - logbeta.max())
+        w /= np.sum(w) * dp
+        E = lambda f: np.sum(f * w) * dp  # noqa: E731
+        Z = E(p) * A0 + E(1 - p) * B0
+        Ez = (E(p) * A1 + E(1 - p) * B1) / Z
+        Ez2 = (E(p) * A2 + E(1 - p) * B2) / Z
+        m = (E(p * p) * A0 + E(p * (1 - p)) * B0) / Z
+        m2 = (E(p ** 3) * A0 + E(p * p * (1 - p)) * B0) / Z
+        ab = (m - m2) / (m2 - m * m)
+        out.append({"prior": [a, b, mu, s2], "measurement": d,
+                    "posterior": {"mu": Ez, "sig": Ez2 - Ez * Ez, "alpha": m * ab,
+                                  "beta": (1 - m) * ab, "inlier": m}})
+    return out
+
+
 def hole_fill():
     """Sources at their own pixel (what an identity pose produces), so the pre-fill table holds
     t[y, x] = y*W + x at the chosen pixels; (5, 0) is on the border and cannot scatter, and
@@ -154,7 +195,8 @@ def hole_fill():
 
 def main():
     fixtures = {"cost_identity.json": cost_identity(), "filter_parabola.json": filter_parabola(),
-                "fusion_posterior.json": fusion_posterior(), "hole_fill.json": hole_fill()}
+                "fusion_posterior.json": fusion_posterior(),
+                "fusion_posterior_cpu.json": fusion_posterior_cpu(), "hole_fill.json": hole_fill()}
     for name, obj in fixtures.items():
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(obj, f, indent=1)

@@ -128,6 +128,62 @@ def test_case_modes(small_scene):
     assert np.array_equal(res[2], dep)
 
 
+def test_case3_cpu_rules_sequence():
+    """outputFusionCPU CASE 3 over 6 keyframes (stereo_mapper.cpp:719-800): the first call
+    resets and returns zeros, each later one fuses the mapper's own filtered depth with the
+    reference's CPU rules; result and state follow the oracle fed the same filtered depths
+    and the double-precision pose composition, from the oracle's own state."""
+    from probabilistic_mapping_amd import synth
+    from probabilistic_mapping_amd.mapper import StereoMapper, set_knobs
+    import oracle
+    D = 32
+    s = synth.trajectory_scene(7, width=160, height=120, n_frames=6, n_sources=2, planes=D,
+                               fx=110.0)
+    K = s["scene"].K
+    fx, fy, cx, cy = (float(K[0, 0]), float(K[1, 1]), float(K[0, 2]), float(K[1, 2]))
+    H, W = s["images"][0].shape
+    old = set_knobs(idep_min=s["idep_min"], case_mode=3, **s["knobs"])
+    ds = (s["idep_max"] - s["idep_min"]) / (D - 1)
+    ost = [{k: np.zeros((H, W)) for k in ("alpha", "beta", "mu", "sig")} for _ in range(2)]
+    pre = 0
+    prev = None
+    try:
+        m = StereoMapper(W, H, D)
+        m.initIntrinsic(K, 0.0, ds)
+        for i in range(len(s["images"])):
+            R, T = (np.asarray(a, np.float64) for a in s["poses"][i])
+            T = T.reshape(3)
+            m.initReference(s["images"][i])
+            for j in s["sources"][i]:
+                m.update(s["images"][j], R, T, *s["poses"][j])
+            got = m.outputFusionCPU(R, T, fx, fy, cx, cy)
+            if prev is None:
+                assert not got.any()
+                prev = (R, T)
+                continue
+            od = m.download("dep").astype(np.float64)
+            r = R.T @ prev[0]
+            t = R.T @ (prev[1] - T)
+            otab = oracle.depth_fuse_cpu_rules(od, ost[pre], ost[pre ^ 1], r, t,
+                                               (fx, fy, cx, cy), "f64")
+            gstate = {k: m.download("cpu_" + k)
+                      for k in ("alpha", "beta", "mu", "sig")}
+            st = parity.state_parity(gstate, ost[pre ^ 1])
+            st["table_mismatch"] = float((m.download("cpu_table") != otab).mean())
+            st["fused"] = parity.depth_parity(got, od)
+            _report(f"CASE 3 keyframe {i}", st)
+            assert st["table_mismatch"] <= 1e-3
+            assert st["init_mismatch"] <= 1e-3
+            for k in ("mu", "sig", "inlier"):
+                assert st[k]["p999"] <= 1e-4, k
+            assert st["fused"]["category_mismatch"] <= 1e-3
+            prev = (R, T)
+            pre ^= 1
+        m.close()
+    finally:
+        set_knobs(**old)
+
+
 @pytest.mark.parametrize("cost_type", ["f32"])
 def test_trajectory_sequence(trajectory_scene, cost_type):
     """752x480, 64 planes, 5 sources, 8 paths: the first 5 keyframes (4 fusion steps)."""

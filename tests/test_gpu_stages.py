@@ -220,6 +220,53 @@ def test_fusion_same_depths(dev, trajectory_scene):
         pre ^= 1
 
 
+def test_fusion_cpu_rules_same_depths(dev, trajectory_scene):
+    """outputFusionCPU's CASE >= 3 step (the reference's CPU rules) over 7 keyframes with
+    identical measurements on both sides: its raster-order z-buffer and 7x7 hole fill give
+    the oracle's table, and the state and result follow it."""
+    s = trajectory_scene
+    K = s["scene"].K
+    cam = (K[0, 0], K[1, 1], K[0, 2], K[1, 2])
+    H, W = s["images"][0].shape
+    n = 8
+    rng = np.random.default_rng(12)
+    meas = []
+    for i in range(n):
+        g = s["gt_depths"][i].astype(np.float64) * (1 + 0.01 * rng.standard_normal((H, W)))
+        g[rng.random((H, W)) < 0.15] = 0.0
+        g[rng.random((H, W)) < 0.01] = 1000.0
+        g[rng.random((H, W)) < 0.01] = 12.0       # dropped by the > 10 clamp
+        meas.append(g.astype(np.float32))
+    ost = [{k: np.zeros((H, W)) for k in ("alpha", "beta", "mu", "sig")} for _ in range(2)]
+    gst = [dev.new_state(H, W, "cuda") for _ in range(2)]
+    table = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    work = None
+    pre = 0
+    for i in range(1, n):
+        Rc, tc = s["poses"][i]
+        Rp, tp = s["poses"][i - 1]
+        Rc = np.asarray(Rc, np.float64); tc = np.asarray(tc, np.float64).reshape(3)
+        r = Rc.T @ np.asarray(Rp, np.float64)
+        t = Rc.T @ (np.asarray(tp, np.float64).reshape(3) - tc)
+        od = meas[i].astype(np.float64)
+        otab = oracle.depth_fuse_cpu_rules(od, ost[pre], ost[pre ^ 1], r, t, cam, "f64")
+        gd = torch.from_numpy(meas[i].copy()).cuda()
+        work = dev.depth_fuse_cpu_rules(gd, gst[pre], gst[pre ^ 1], r, t, cam, table=table,
+                                        work=work)
+        gtab = table.cpu().numpy()
+        gstate = {k: v.cpu().numpy() for k, v in gst[pre ^ 1].items()}
+        st = parity.state_parity(gstate, ost[pre ^ 1])
+        st["table_mismatch"] = float((gtab != otab).mean())
+        st["fused"] = parity.depth_parity(gd.cpu().numpy(), od)
+        _report(f"fusion (CPU rules) step {i}", st)
+        assert st["table_mismatch"] <= 1e-3
+        assert st["init_mismatch"] <= 1e-3
+        for k in ("mu", "sig", "inlier"):
+            assert st[k]["p999"] <= 1e-4, k
+        assert st["fused"]["category_mismatch"] <= 1e-3
+        pre ^= 1
+
+
 @pytest.mark.parametrize("paths", [4, 8])
 @pytest.mark.parametrize("planes", [32, 64, 96, 128])
 def test_sgm_filter_equals_sgm_then_filter(dev, small_scene, paths, planes):

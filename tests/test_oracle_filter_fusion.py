@@ -153,3 +153,129 @@ def test_gates():
     # nothing predicted: every pixel initialises to (10, 10, d, 2500) and outputs d
     assert np.all(cur["alpha"] == 10) and np.all(cur["beta"] == 10)
     assert np.all(cur["mu"] == 1.5) and np.all(cur["sig"] == 2500) and np.all(depth == 1.5)
+
+
+# ---- the CPU rules of outputFusionCPU (CASE >= 3), stereo_mapper.cpp:246-587, 719-800 ------
+
+def _enc(y, x):
+    return (y << 16) | x
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_cpu_rules_update_matches_posterior_moments(case):
+    """depthUpdateCPU's moment matching against brute-force quadrature with the reference's own
+    weights (tau^2 = 64, outlier density 1/16, normpdfCPU's sqrt factor); output gated on the
+    POSTERIOR inlier ratio > 0.51."""
+    c = _load("fusion_posterior_cpu.json")[case]
+    a, b, mu, s2 = c["prior"]
+    H, W = 9, 9
+    pre = _state(H, W, a, b, mu, s2 - 0.05 * mu)    # identity pose adds 0.05 mu
+    cur = _state(H, W)
+    depth = np.full((H, W), c["measurement"])
+    oracle.depth_fuse_cpu_rules(depth, pre, cur, *EYE, CAM, "f64")
+    p = c["posterior"]
+    y, x = 4, 4
+    assert cur["mu"][y, x] == pytest.approx(p["mu"], rel=1e-7)
+    assert cur["sig"][y, x] == pytest.approx(p["sig"], rel=1e-6)
+    assert cur["alpha"][y, x] == pytest.approx(p["alpha"], rel=1e-6)
+    assert cur["beta"][y, x] == pytest.approx(p["beta"], rel=1e-6)
+    expect = p["mu"] if p["inlier"] > 0.51 else 0.0
+    assert depth[y, x] == pytest.approx(expect, rel=1e-7)
+
+
+def test_cpu_rules_identity_pose_table():
+    """Every eligible pixel maps to itself, encoded (y << 16) | x; pixel (0, 0) encodes to the
+    empty sentinel and is filled from the first interior source of its 7x7 window, (3, 3)."""
+    H, W = 9, 11
+    rng = np.random.default_rng(1)
+    mu = rng.uniform(1, 4, size=(H, W))
+    pre = {"alpha": np.full((H, W), 10.0), "beta": np.full((H, W), 5.0), "mu": mu.copy(),
+           "sig": np.full((H, W), 0.2)}
+    cur = _state(H, W)
+    depth = np.zeros((H, W))
+    table = oracle.depth_fuse_cpu_rules(depth, pre, cur, *EYE, CAM, "f64")
+    ys, xs = np.mgrid[0:H, 0:W]
+    want = (ys << 16) | xs
+    want[0, 0] = _enc(3, 3)
+    assert np.array_equal(table, want)
+    np.testing.assert_allclose(pre["sig"], 0.2 + 0.05 * mu, rtol=1e-15)
+    # no measurement: beta + 1, output the predicted mean if 10 / 16 > 0.51, then >= 0.1 kept
+    assert np.all(cur["beta"].ravel()[1:] == 6.0)
+    np.testing.assert_allclose(depth.ravel()[1:], mu.ravel()[1:], rtol=0)
+
+
+def test_cpu_rules_zbuffer_compares_transformed_with_original_depth():
+    """transformMapCPU keeps the first arrival unless its TRANSFORMED depth exceeds the
+    newcomer's ORIGINAL depth (stereo_mapper.cpp:315-325).  A forward translation t_z = 1 at
+    depth 1 halves offsets from the principal point: x = 4 and x = 5 (cx = 3) both land on
+    x = 4; the first (x = 4) has transformed depth 2 > 1, so x = 5 takes the cell, where the
+    GPU rules (nearest transformed depth, then lowest index) keep x = 4."""
+    H, W = 9, 9
+    cam = (1.0, 1.0, 3.0, 4.0)
+    pre = _state(H, W)
+    for x in (4, 5):
+        pre["alpha"][4, x], pre["beta"][4, x], pre["mu"][4, x] = 10, 1, 1.0
+    t = np.array([0.0, 0.0, 1.0], np.float32)
+    table = oracle.depth_fuse_cpu_rules(np.zeros((H, W)), pre, _state(H, W),
+                                        np.eye(3, dtype=np.float32), t, cam, "f64")
+    assert table[4, 4] == _enc(4, 5)
+    assert pre["mu"][4, 4] == 1.0      # the displaced source's mean is overwritten (:322)
+    pre2 = _state(H, W)
+    for x in (4, 5):
+        pre2["alpha"][4, x], pre2["beta"][4, x], pre2["mu"][4, x] = 10, 1, 1.0
+    gpu = oracle.depth_fuse(np.zeros((H, W)), pre2, _state(H, W),
+                            np.eye(3, dtype=np.float32), t, cam, "f64")
+    assert gpu[4, 4] == 4 * W + 4
+
+
+def test_cpu_rules_hole_fill_first_source_in_raster_order():
+    """holeFillingCPU (7x7, in place with visited marks): an empty cell takes the entry of the
+    first non-empty cell, in raster order, among the interior sources (3 <= y < H-3,
+    3 <= x < W-3) of its 7x7 window; the rule is checked here by direct construction."""
+    H, W = 12, 14
+    src = {(3, 4): 2.0, (5, 8): 3.0, (8, 3): 4.0, (9, 10): 5.0, (1, 1): 6.0}   # (1,1): not interior
+    pre = _state(H, W)
+    for (y, x), z in src.items():
+        pre["alpha"][y, x], pre["beta"][y, x], pre["mu"][y, x], pre["sig"][y, x] = 10, 1, z, 1
+    table = oracle.depth_fuse_cpu_rules(np.zeros((H, W)), pre, _state(H, W), *EYE,
+                                        (100.0, 100.0, 6.0, 5.0), "f64")
+    orig = {(y, x) for (y, x) in src}
+    want = np.zeros((H, W), np.int64)
+    for (y, x) in orig:
+        want[y, x] = _enc(y, x)
+    for y in range(H):
+        for x in range(W):
+            if (y, x) in orig:
+                continue
+            for sy in range(y - 3, y + 4):
+                hit = [(sy, sx) for sx in range(x - 3, x + 4)
+                       if (sy, sx) in orig and 3 <= sy < H - 3 and 3 <= sx < W - 3]
+                if hit:
+                    want[y, x] = _enc(*hit[0])
+                    break
+    assert np.array_equal(table, want)
+
+
+def test_cpu_rules_clamps_gates_and_init():
+    """Measurements above 100 are dropped and their output masked, above 10 dropped; pixels
+    with an inlier ratio below 0.5 never propagate; a new pixel starts at (10, 10, d, 256)
+    with output 0; outputs below 0.1 become 0."""
+    H, W = 9, 9
+    pre = _state(H, W, 10.0, 10.5, 2.0, 1.0)      # ratio < 0.5: nothing propagates
+    cur = _state(H, W)
+    depth = np.full((H, W), 1.5)
+    depth[2, 2], depth[3, 3], depth[4, 4] = 150.0, 20.0, 0.05
+    table = oracle.depth_fuse_cpu_rules(depth, pre, cur, *EYE, CAM, "f64")
+    assert np.all(table == 0)
+    assert cur["alpha"][0, 0] == 10 and cur["beta"][0, 0] == 10
+    assert cur["mu"][0, 0] == 1.5 and cur["sig"][0, 0] == 256
+    assert np.all(depth == 0)                      # a new pixel outputs 0
+    assert cur["alpha"][2, 2] == 0 and cur["mu"][2, 2] == 0   # dropped measurement: mu = 0 only
+    assert cur["alpha"][4, 4] == 10 and cur["mu"][4, 4] == 0.05
+    # second step: identity pose, every pixel now propagates (ratio 0.5)
+    depth2 = np.full((H, W), 1.5)
+    depth2[2, 2] = 150.0
+    pre2, cur2 = cur, _state(H, W)
+    oracle.depth_fuse_cpu_rules(depth2, pre2, cur2, *EYE, CAM, "f64")
+    assert depth2[2, 2] == 0.0                     # masked
+    assert depth2[5, 5] > 0.1
