@@ -12,6 +12,9 @@ Forms: "plain" runs one keyframe at a time (FP32 cost, exact SGM storage by defa
 runs every keyframe's cost, SGM and filter as one autotuned, hipGraph-captured batch, then the
 fusion steps (FP16 cost, compact SGM storage by default).  The default, "both", reports the
 plain FP32 form as ``value`` and the FP16 batched form beside it, with their ratio.
+``--form diff`` times the differentiable readout's forward + backward; ``--form driver`` times
+the reference node's processImg loop (host images in, host depth out), with and without the
+upload-once frame ring.
 
 Prints one JSON line.  Extra objects: "roofline" (the SGM pass, HBM-bound), "stages" (per-stage
 ms and roofline fractions) and "cpu_baseline" (the float/OpenMP oracle on a bounded sample of
@@ -50,7 +53,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cost-type", default="f32", choices=["f32", "f16", "bf16"],
                     help="cost storage of the plain / batched form (both: of the plain form)")
-    ap.add_argument("--form", default="both", choices=["both", "plain", "batched", "diff"])
+    ap.add_argument("--form", default="both",
+                    choices=["both", "plain", "batched", "diff", "driver"])
     ap.add_argument("--sgm-storage", default=None, choices=["exact", "compact", "sweep"],
                     help="SGM per-path storage (default: exact for f32 cost, compact otherwise; "
                          "sweep exists in the batched form only)")
@@ -366,6 +370,69 @@ def bench_diff(args, torch):
     print(json.dumps(out), flush=True)
 
 
+def bench_driver(args, torch):
+    """The reference node's loop (evaluate.cc processImg through pm::FrameDriver): a 640x480
+    stream of 60 frames, 64 planes, aggNum 3 at interval 6, CASE 3 (CPU fusion rules), host
+    images in and host depths out.  Timed per processImg step that runs (after the first,
+    which allocates the mapper), with the frames reaching the mapper through the upload-once
+    frame ring and, beside it, uploaded at every use as the reference does."""
+    from scipy.spatial.transform import Rotation
+    from probabilistic_mapping_amd import driver, mapper, synth
+    n_frames, planes = 60, 64
+    s = synth.trajectory_scene(args.seed, width=640, height=480, n_frames=n_frames, n_sources=3,
+                               planes=planes, fx=481.2017)
+    K = s["scene"].K
+    cfg = dict(agg_num=3, interval=6, planes=planes, fx=float(K[0, 0]), fy=float(K[1, 1]),
+               cx=float(K[0, 2]), cy=float(K[1, 2]),
+               dep_sample=(s["idep_max"] - s["idep_min"]) / (planes - 1))
+    poses = []
+    for R, T in s["poses"]:
+        q = Rotation.from_matrix(np.asarray(R, np.float64)).as_quat()
+        poses.append((np.asarray(T, np.float64).reshape(3), q))
+    old = mapper.set_knobs(case_mode=3, idep_min=s["idep_min"], sgm_paths=4, **s["knobs"])
+    forms = {}
+    try:
+        for name, ring in (("frame_ring", True), ("per_use_upload", False)):
+            runs = []
+            for _ in range(max(args.warmup, 0) + 1):
+                d = driver.FrameDriver(frame_ring=ring, **cfg)
+                times = []
+                for f in range(n_frames):
+                    d.grab_pose(1000 + f, *poses[f])
+                    d.grab_image(1000 + f, s["images"][f])
+                    t0 = time.perf_counter()
+                    r = d.process()
+                    if r is not None:
+                        times.append(time.perf_counter() - t0)
+                runs.append((times, d.ring_puts))
+                d.close()
+            times, puts = runs[-1]
+            steady = times[1:]
+            forms[name] = {"steps": len(times), "ms_per_step": float(np.mean(steady)) * 1e3,
+                           "ms_per_step_p50": float(np.median(steady)) * 1e3,
+                           "ms_steps": [round(t * 1e3, 3) for t in times],
+                           "steps_per_s": 1.0 / float(np.mean(steady)),
+                           "frame_uploads": puts if ring else len(times) * (cfg["agg_num"] + 1)}
+    finally:
+        mapper.set_knobs(**old)
+    fr, pu = forms["frame_ring"], forms["per_use_upload"]
+    out = {
+        "metric": "processImg steps/s (host images in, host depth out) @640x480x64 planes, "
+                  "aggNum 3, interval 6, CASE 3",
+        "value": fr["steps_per_s"], "unit": "steps/s",
+        "n_gpus": 1, "steps": fr["steps"] - 1, "warmup": args.warmup,
+        "ms_per_step": fr["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded textured-plane scene, 60-frame trajectory)",
+        "config": {"workload": "640x480 frames, 64 planes, 4-path SGM, aggNum 3 at interval 6, "
+                               "CPU fusion rules, host-side driver loop", "form": "driver"},
+        "forms": forms,
+        "ring_speedup": pu["ms_per_step"] / fr["ms_per_step"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -379,6 +446,8 @@ def main():
     torch.cuda.set_device(0)
     if args.form == "diff":
         return bench_diff(args, torch)
+    if args.form == "driver":
+        return bench_driver(args, torch)
     s = synth.trajectory_scene(args.seed)
     copy = attainable_copy(torch, args.copy_mib)
     storage = default_storage(args.cost_type, args.sgm_storage)

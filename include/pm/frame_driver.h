@@ -36,6 +36,9 @@ struct DriverConfig {
   double fx = 481.2017, fy = -480.0002, cx = 319.5, cy = 239.5;
   float dep_sample = 1.0f / (0.15f * 160.0f);
   int planes = 64;                        // DEP_CNT
+  // frames reach the mapper through an upload-once FrameRing (each used frame is uploaded and
+  // converted once instead of at every use); the depths are identical either way
+  bool frame_ring = true;
 };
 
 // One processImg step's choice: the reference frame and its sources, as indices into the
@@ -68,6 +71,7 @@ class FrameDriver {
 
   int skipped() const { return skipped_; }   // steps refused for lack of source poses
   StereoMapper* mapper() { return mapper_.get(); }
+  const FrameRing* ring() const { return ring_.get(); }
 
  private:
   struct Image {
@@ -83,6 +87,7 @@ class FrameDriver {
   DriverConfig cfg_;
   bool with_mapper_;
   std::unique_ptr<StereoMapper> mapper_;
+  std::unique_ptr<FrameRing> ring_;
   std::list<Image> images_;   // newest first
   std::list<Pose> poses_;     // newest first
   int last_size_;

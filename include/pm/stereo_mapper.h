@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "pm/frame_ring.h"
 #include "pm/parameters.h"
 #include "pm/types.h"
 #include "pm_capi.h"
@@ -42,12 +43,18 @@ class StereoMapper {
   void output(Mat& result);
   void outputFusion(Mat& result, Matrix3d R, Vector3d t, float fx, float fy, float cx,
                     float cy);
-  // CASE 0 / 1 / 2 ablation outputs (stereo_mapper.cpp:625-719).  For CASE >= 3 this runs
-  // the GPU fusion of outputFusion (the reference's CPU-constant fusion is not provided).
+  // CASE 0 / 1 / 2 ablation outputs (stereo_mapper.cpp:625-719); CASE >= 3 runs the
+  // reference's CPU fusion rules (:719-800) on the GPU, with their own state.
   void outputFusionCPU(Mat& result, Matrix3d R, Vector3d t, float fx, float fy, float cx,
                        float cy);
 
   // extensions
+  // initReference / update reading an ingested frame of a FrameRing instead of a host image
+  // (no upload, no conversion; the homography is set up as in update).  The frame must stay
+  // in the ring until the next output call returns.
+  void initReference(const FrameRing& ring, int64_t frame);
+  void update(const FrameRing& ring, int64_t frame, const Mat& R_l, const Mat& T_l,
+              const Mat& R_r, const Mat& T_r);
   void set_cost_type(pm_cost_type t);
   pm_cost_type cost_type() const { return cost_type_; }
   int width() const { return W_; }
@@ -87,6 +94,8 @@ class StereoMapper {
   void release();
   void ensure_cost();
   void run_sgm_filter();
+  pm_homography stage_homography(const Mat& R_l, const Mat& T_l, const Mat& R_r,
+                                 const Mat& T_r);
   void fuse_cpu_rules(Mat& result, Matrix3d R, Vector3d t, float fx, float fy, float cx,
                       float cy);
 
@@ -95,6 +104,11 @@ class StereoMapper {
   pm_cost_type raw_cost_type_ = PM_COST_F32;
   void* stream_ = nullptr;
   std::vector<DeviceMat> quads_;           // per staged source, 8-byte texels [H][W]
+  struct RingFrame {
+    const FrameRing* ring;                 // null: the source's texture is quads_[slot]
+    int64_t frame;
+  };
+  std::vector<RingFrame> staged_;          // per staged source
   std::vector<pm_homography> homs_;
   DeviceMat zbuf_, med_tmp_, wta_tmp_;
   DeviceMat sgm_ws_;                       // per-path L volumes (pm_sgm_filter_paths)

@@ -274,6 +274,27 @@ pm_status pm_batch_capture(pm_batch* b, pm_stream stream);
 /* ... and replays it. */
 pm_status pm_batch_launch(pm_batch* b, pm_stream stream);
 
+/* ---- Upload-once frame ring -----------------------------------------------------------
+ * Replaces the per-use upload and u8 -> float conversion of StereoMapper::update
+ * (motion_stereo/stereo_mapper.cpp:73-77), which the ROS node repeats for every use of a
+ * frame (evaluate.cc:209-233).  A ring holds the newest `capacity` frames on the device, each
+ * uploaded once through pinned staging and converted once into its quad texture (as
+ * pm_make_quad).  Frame ids are >= 0; frame f occupies slot f % capacity until another frame
+ * is put there.  put enqueues on `stream` and records an event that readers wait for. */
+typedef struct pm_frame_ring pm_frame_ring;
+pm_status pm_frame_ring_create(int32_t width, int32_t height, int32_t capacity,
+                               pm_frame_ring** out);
+void pm_frame_ring_destroy(pm_frame_ring* r);
+/* image: host u8 [height][width]; reusable as soon as put returns. */
+pm_status pm_frame_ring_put(pm_frame_ring* r, int64_t frame, const uint8_t* image,
+                            pm_stream stream);
+/* Device pointers of a held frame (either may be NULL); PM_ERR_INVALID_STATE if the frame
+ * was never put or has been evicted. */
+pm_status pm_frame_ring_frame(const pm_frame_ring* r, int64_t frame, const uint8_t** image,
+                              const uint64_t** quad);
+/* Frames uploaded so far. */
+pm_status pm_frame_ring_puts(const pm_frame_ring* r, int64_t* puts);
+
 /* ---- StereoMapper handle (motion_stereo/stereo_mapper.h:26-63) ------------------------
  * The C++ class lives in include/pm/stereo_mapper.h; these wrappers expose it to C and
  * Python.  Images and results are HOST arrays; the mapper owns its device buffers. */
@@ -288,6 +309,14 @@ pm_status pm_mapper_init_reference(pm_mapper* m, const uint8_t* image);
 /* StereoMapper::update, stereo_mapper.cpp:73-91 */
 pm_status pm_mapper_update(pm_mapper* m, const uint8_t* image, const double R_l[9],
                            const double T_l[3], const double R_r[9], const double T_r[3]);
+/* initReference / update reading frame `frame` of a ring (same results as passing its host
+ * image; no upload).  A frame staged by update must stay in the ring until the next output
+ * call returns, else that call fails with PM_ERR_INVALID_STATE. */
+pm_status pm_mapper_init_reference_frame(pm_mapper* m, const pm_frame_ring* ring,
+                                         int64_t frame);
+pm_status pm_mapper_update_frame(pm_mapper* m, const pm_frame_ring* ring, int64_t frame,
+                                 const double R_l[9], const double T_l[3],
+                                 const double R_r[9], const double T_r[3]);
 /* StereoMapper::output, stereo_mapper.cpp:93-109; result: host float [H][W] */
 pm_status pm_mapper_output(pm_mapper* m, float* result);
 /* StereoMapper::outputFusion, stereo_mapper.cpp:111-244; result: host float [H][W] holds the
@@ -328,6 +357,9 @@ typedef struct pm_driver_config {
   int32_t planes;     /* DEP_CNT (64) */
   double fx, fy, cx, cy;   /* evaluate.cc:37-40 */
   float dep_sample;   /* evaluate.cc:347: 1 / (0.15 * 160) */
+  int32_t frame_ring; /* 1: frames reach the mapper through an upload-once frame ring of
+                         agg_num * interval + 1 frames; 0: uploaded at every use, as the
+                         reference does.  The depths are identical. */
 } pm_driver_config;
 /* with_mapper = 0 only plans the steps (host only: no device memory, no GPU work). */
 pm_status pm_driver_create(const pm_driver_config* cfg, int32_t with_mapper, pm_driver** out);
@@ -345,6 +377,8 @@ pm_status pm_driver_process(pm_driver* d, int32_t* ran, int64_t* stamp_ns, int32
 /* Steps not run because the sources' poses were not buffered yet (the reference reads past
  * the end of its pose list there). */
 pm_status pm_driver_skipped(const pm_driver* d, int32_t* skipped);
+/* Frames uploaded through the driver's frame ring so far (0 without a ring). */
+pm_status pm_driver_ring_puts(const pm_driver* d, int64_t* puts);
 /* Eigen's Quaterniond(w, x, y, z).toRotationMatrix(), q = (x, y, z, w); R row-major. */
 pm_status pm_quaternion_to_rotation(const double q[4], double R[9]);
 

@@ -96,7 +96,8 @@ class DiffParams(ctypes.Structure):
 class DriverConfig(ctypes.Structure):
     _fields_ = [("agg_num", ctypes.c_int32), ("interval", ctypes.c_int32),
                 ("planes", ctypes.c_int32), ("fx", ctypes.c_double), ("fy", ctypes.c_double),
-                ("cx", ctypes.c_double), ("cy", ctypes.c_double), ("dep_sample", ctypes.c_float)]
+                ("cx", ctypes.c_double), ("cy", ctypes.c_double), ("dep_sample", ctypes.c_float),
+                ("frame_ring", ctypes.c_int32)]
 
 
 _vp = ctypes.c_void_p
@@ -145,6 +146,14 @@ SIGNATURES = {
     "pm_mapper_init_intrinsic": (_i32, [_vp, _dp, ctypes.c_float, ctypes.c_float]),
     "pm_mapper_init_reference": (_i32, [_vp, _vp]),
     "pm_mapper_update": (_i32, [_vp, _vp, _dp, _dp, _dp, _dp]),
+    "pm_mapper_init_reference_frame": (_i32, [_vp, _vp, ctypes.c_int64]),
+    "pm_mapper_update_frame": (_i32, [_vp, _vp, ctypes.c_int64, _dp, _dp, _dp, _dp]),
+    "pm_frame_ring_create": (_i32, [_i32, _i32, _i32, ctypes.POINTER(_vp)]),
+    "pm_frame_ring_destroy": (None, [_vp]),
+    "pm_frame_ring_put": (_i32, [_vp, ctypes.c_int64, _vp, _vp]),
+    "pm_frame_ring_frame": (_i32, [_vp, ctypes.c_int64, ctypes.POINTER(_vp),
+                                   ctypes.POINTER(_vp)]),
+    "pm_frame_ring_puts": (_i32, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "pm_mapper_output": (_i32, [_vp, _vp]),
     "pm_mapper_output_fusion": (_i32, [_vp, _vp, _dp, _dp, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, ctypes.c_float]),
@@ -176,6 +185,7 @@ SIGNATURES = {
                                  ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp,
                                  ctypes.c_size_t]),
     "pm_driver_skipped": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "pm_driver_ring_puts": (_i32, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "pm_quaternion_to_rotation": (_i32, [_dp, _dp]),
 }
 

@@ -20,7 +20,7 @@ SYNTH_LIB = os.path.join(HERE, "libpm_synth.so")
 
 ARCH = "gfx950"
 HIP_SOURCES = ["cost.hip", "sgm.hip", "sweep.hip", "filter.hip", "fusion.hip", "fusion_cpu.hip", "batch.hip", "diff.hip", "capi.hip", "se3.cpp",
-               "stereo_mapper.cpp", "parameters.cpp", "frame_driver.cpp"]
+               "stereo_mapper.cpp", "parameters.cpp", "frame_driver.cpp", "frame_ring.cpp"]
 HIP_HEADERS = ["common.hpp", "cost_tile.hpp", "sgm_scan.hpp"]
 
 # Packed-FP32 VALU forms are disabled for every device compile.

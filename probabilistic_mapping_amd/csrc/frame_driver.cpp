@@ -86,9 +86,11 @@ bool FrameDriver::process(DriverJob& job, Mat* depth) {
   job.ref_image = img->index;
   job.ref_pose = pose->index;
   const Mat ref = img->image;   // shares the pixels; the lists may drop the entry below
+  const int ref_index = img->index;
   const Matrix3d R_ref = quaternion_to_rotation(pose->qx, pose->qy, pose->qz, pose->qw);
   const Vector3d t_ref(pose->tx, pose->ty, pose->tz);
   std::vector<Mat> src;
+  std::vector<int> src_index;
   std::vector<Matrix3d> R_src;
   std::vector<Vector3d> t_src;
   // :209-233: every interval-th older image, and the pose as many places further on
@@ -100,6 +102,7 @@ bool FrameDriver::process(DriverJob& job, Mat* depth) {
     job.src_images.push_back(si->index);
     job.src_poses.push_back(sp->index);
     src.push_back(si->image);
+    src_index.push_back(si->index);
     R_src.push_back(quaternion_to_rotation(sp->qx, sp->qy, sp->qz, sp->qw));
     t_src.push_back(Vector3d(sp->tx, sp->ty, sp->tz));
   }
@@ -125,9 +128,25 @@ bool FrameDriver::process(DriverJob& job, Mat* depth) {
   if (ref.rows != mapper_->height() || ref.cols != mapper_->width())
     throw Error(PM_ERR_INVALID_ARGUMENT, "FrameDriver: every image must have the first one's size");
   const Mat R_l = mat33(R_ref), T_l = mat31(t_ref(0), t_ref(1), t_ref(2));
-  mapper_->initReference(ref);
-  for (int i = 0; i < cfg_.agg_num; ++i)
-    mapper_->update(src[i], R_l, T_l, mat33(R_src[i]), mat31(t_src[i](0), t_src[i](1), t_src[i](2)));
+  if (cfg_.frame_ring) {
+    // the reference and its sources are among the newest agg_num * interval + 1 images
+    // grabbed, so a ring of that many slots never evicts a frame a step still needs; a frame
+    // is uploaded at its first use
+    if (!ring_)
+      ring_.reset(new FrameRing(ref.cols, ref.rows, cfg_.agg_num * cfg_.interval + 1));
+    if (!ring_->holds(ref_index)) ring_->put(ref_index, ref, nullptr);
+    for (int i = 0; i < cfg_.agg_num; ++i)
+      if (!ring_->holds(src_index[i])) ring_->put(src_index[i], src[i], nullptr);
+    mapper_->initReference(*ring_, ref_index);
+    for (int i = 0; i < cfg_.agg_num; ++i)
+      mapper_->update(*ring_, src_index[i], R_l, T_l, mat33(R_src[i]),
+                      mat31(t_src[i](0), t_src[i](1), t_src[i](2)));
+  } else {
+    mapper_->initReference(ref);
+    for (int i = 0; i < cfg_.agg_num; ++i)
+      mapper_->update(src[i], R_l, T_l, mat33(R_src[i]),
+                      mat31(t_src[i](0), t_src[i](1), t_src[i](2)));
+  }
   Mat result;
   mapper_->outputFusionCPU(result, R_ref, t_ref, (float)cfg_.fx, (float)cfg_.fy, (float)cfg_.cx,
                            (float)cfg_.cy);
@@ -174,6 +193,7 @@ extern "C" pm_status pm_driver_create(const pm_driver_config* cfg, int32_t with_
     c.planes = cfg->planes;
     c.fx = cfg->fx; c.fy = cfg->fy; c.cx = cfg->cx; c.cy = cfg->cy;
     c.dep_sample = cfg->dep_sample;
+    c.frame_ring = cfg->frame_ring != 0;
     auto* d = new pm_driver{nullptr, c.agg_num};
     try {
       d->impl = new pm::FrameDriver(c, with_mapper != 0);
@@ -236,6 +256,14 @@ extern "C" pm_status pm_driver_skipped(const pm_driver* d, int32_t* skipped) {
   if (!d || !d->impl || !skipped)
     return pm::fail(PM_ERR_INVALID_ARGUMENT, "pm_driver_skipped: bad argument");
   *skipped = d->impl->skipped();
+  return PM_OK;
+}
+
+extern "C" pm_status pm_driver_ring_puts(const pm_driver* d, int64_t* puts) {
+  if (!d || !d->impl || !puts)
+    return pm::fail(PM_ERR_INVALID_ARGUMENT, "pm_driver_ring_puts: bad argument");
+  const pm::FrameRing* r = d->impl->ring();
+  *puts = r ? r->puts() : 0;
   return PM_OK;
 }
 

@@ -12,9 +12,10 @@
 // order; both are reproduced exactly:
 //   - z-buffer (:312-325): the first arrival at a cell keeps it unless its transformed depth
 //     exceeds a later arrival's ORIGINAL depth; the displaced arrival's mean is then set to
-//     that original depth.  Here every arrival appends its source index to its cell
-//     (cpu_transform_kernel); cpu_zbuffer_kernel sorts a cell's arrivals by raster index
-//     (at most kSlots; a cell with more scans all sources in order) and folds them.
+//     that original depth.  Here the arrivals are grouped by cell with a counting sort
+//     (cpu_transform_kernel counts them, two scan kernels turn the counts into offsets,
+//     cpu_scatter_kernel writes each source index into its cell's segment), and
+//     cpu_zbuffer_kernel puts a cell's segment in raster order and folds it.
 //   - hole filling (:361-388): sources are visited in raster order and fill the still-empty
 //     cells of their 7x7 windows, so an empty cell ends with the entry of the first interior
 //     source, in raster order, inside its own 7x7 window (cpu_update_kernel reads it from the
@@ -27,8 +28,9 @@ namespace pm {
 
 namespace {
 
-constexpr int kSlots = 8;     // arrivals a cell records; more fall back to a full scan
-constexpr int kExpand = 3;    // holeFillingCPU's expand_sz
+constexpr int kExpand = 3;        // holeFillingCPU's expand_sz
+constexpr int kScanBlock = 1024;  // cells per block of the offset scan (256 threads x 4)
+constexpr int kSmallCell = 32;    // arrivals a cell orders by repeated minimum; more: heap sort
 
 struct CpuFuseArgs {
   int W, H;
@@ -38,27 +40,35 @@ struct CpuFuseArgs {
 
 // scratch layout (all [N] unless noted)
 struct CpuFuseWork {
-  int32_t* cnt;      // arrivals per cell
-  int32_t* slots;    // [N][kSlots] source indices
+  int32_t* cnt;      // arrivals per cell (zeroed)
+  int32_t* cur;      // scatter cursor per cell (zeroed; follows cnt)
+  int32_t* off;      // exclusive prefix of cnt within its scan block
+  int32_t* bsum;     // [blocks] exclusive prefix of the scan blocks' totals
+  int32_t* list;     // source indices grouped by target cell
   int32_t* tgt;      // target cell of each source, -1 if it does not propagate
   float* zt;         // transformed depth of each propagating source
   float* dorig;      // its depth before the transform
   uint32_t* raw;     // pre-fill table
 };
 
-__host__ __device__ inline size_t cpu_fuse_work_bytes(long N) {
-  return (size_t)N * (4 + 4 * kSlots + 4 + 4 + 4 + 4);
+inline long scan_blocks(long N) { return (N + kScanBlock - 1) / kScanBlock; }
+
+inline size_t cpu_fuse_work_bytes(long N) {
+  return (size_t)N * (4 * 8 + 4) + (size_t)scan_blocks(N) * 4;
 }
 
 CpuFuseWork carve(void* base, long N) {
   char* p = static_cast<char*>(base);
   CpuFuseWork w;
   w.cnt = reinterpret_cast<int32_t*>(p); p += (size_t)N * 4;
-  w.slots = reinterpret_cast<int32_t*>(p); p += (size_t)N * 4 * kSlots;
+  w.cur = reinterpret_cast<int32_t*>(p); p += (size_t)N * 4;
+  w.off = reinterpret_cast<int32_t*>(p); p += (size_t)N * 4;
+  w.list = reinterpret_cast<int32_t*>(p); p += (size_t)N * 4;
   w.tgt = reinterpret_cast<int32_t*>(p); p += (size_t)N * 4;
   w.zt = reinterpret_cast<float*>(p); p += (size_t)N * 4;
   w.dorig = reinterpret_cast<float*>(p); p += (size_t)N * 4;
-  w.raw = reinterpret_cast<uint32_t*>(p);
+  w.raw = reinterpret_cast<uint32_t*>(p); p += (size_t)N * 4;
+  w.bsum = reinterpret_cast<int32_t*>(p);
   return w;
 }
 
@@ -102,8 +112,86 @@ __global__ __launch_bounds__(256) void cpu_transform_kernel(const CpuFuseArgs a,
   w.zt[i] = z2;
   w.dorig[i] = depth;
   w.tgt[i] = (int32_t)cell;
-  const int slot = atomicAdd(&w.cnt[cell], 1);
-  if (slot < kSlots) w.slots[cell * kSlots + slot] = (int32_t)i;
+  atomicAdd(&w.cnt[cell], 1);
+}
+
+// exclusive scan of cnt within blocks of kScanBlock cells; the block totals go to bsum
+__global__ __launch_bounds__(256) void cpu_scan_blocks_kernel(long N, CpuFuseWork w) {
+  __shared__ int32_t part[256];
+  const long base = (long)blockIdx.x * kScanBlock + threadIdx.x * 4;
+  int32_t v[4];
+  int32_t sum = 0;
+  for (int k = 0; k < 4; ++k) {
+    v[k] = base + k < N ? w.cnt[base + k] : 0;
+    sum += v[k];
+  }
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {   // inclusive Hillis-Steele scan of the thread sums
+    const int32_t t = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int32_t run = part[threadIdx.x] - sum;
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < N) w.off[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) w.bsum[blockIdx.x] = part[255];
+}
+
+// exclusive scan of the block totals, in place, by one block looping over chunks of 256
+__global__ __launch_bounds__(256) void cpu_scan_sums_kernel(long blocks, CpuFuseWork w) {
+  __shared__ int32_t part[256];
+  int32_t carry = 0;
+  for (long c0 = 0; c0 < blocks; c0 += 256) {
+    const long i = c0 + threadIdx.x;
+    const int32_t v = i < blocks ? w.bsum[i] : 0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+      const int32_t t = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+      __syncthreads();
+      part[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < blocks) w.bsum[i] = carry + part[threadIdx.x] - v;
+    carry += part[255];
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ long cell_base(const CpuFuseWork& w, long c) {
+  return (long)w.off[c] + (long)w.bsum[c / kScanBlock];
+}
+
+// each propagating source writes its index into its target cell's segment (in any order)
+__global__ __launch_bounds__(256) void cpu_scatter_kernel(long N, CpuFuseWork w) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int32_t c = w.tgt[i];
+  if (c < 0) return;
+  const int32_t k = atomicAdd(&w.cur[c], 1);
+  w.list[cell_base(w, c) + k] = (int32_t)i;
+}
+
+// in-place ascending heap sort of a cell's segment (cells with many arrivals)
+__device__ void heap_sort(int32_t* a, long n) {
+  auto sift = [&](long root, long end) {
+    while (2 * root + 1 < end) {
+      long child = 2 * root + 1;
+      if (child + 1 < end && a[child] < a[child + 1]) ++child;
+      if (!(a[root] < a[child])) return;
+      const int32_t t = a[root]; a[root] = a[child]; a[child] = t;
+      root = child;
+    }
+  };
+  for (long s = n / 2 - 1; s >= 0; --s) sift(s, n);
+  for (long end = n - 1; end > 0; --end) {
+    const int32_t t = a[0]; a[0] = a[end]; a[end] = t;
+    sift(0, end);
+  }
 }
 
 // the z-buffer of transformMapCPU (:312-325), folded over a cell's arrivals in raster order
@@ -114,6 +202,7 @@ __global__ __launch_bounds__(256) void cpu_zbuffer_kernel(const CpuFuseArgs a,
   const long N = (long)a.W * a.H;
   if (c >= N) return;
   const int n = w.cnt[c];
+  const long base = n > 0 ? cell_base(w, c) : 0;
   uint32_t e = 0;
   long occ = -1;
   auto arrive = [&](long s) {
@@ -126,19 +215,21 @@ __global__ __launch_bounds__(256) void cpu_zbuffer_kernel(const CpuFuseArgs a,
       occ = s;
     }
   };
-  if (n <= kSlots) {
-    int32_t s[kSlots];
-    for (int k = 0; k < n; ++k) s[k] = w.slots[c * kSlots + k];
-    for (int k = 1; k < n; ++k) {                   // arrival order = raster order
-      const int32_t v = s[k];
-      int m = k - 1;
-      while (m >= 0 && s[m] > v) { s[m + 1] = s[m]; --m; }
-      s[m + 1] = v;
+  // arrival order = raster order of the sources
+  if (n <= kSmallCell) {
+    int32_t prev = -1;
+    for (int k = 0; k < n; ++k) {                   // the next larger index, n times
+      int32_t next = 0x7fffffff;
+      for (int m = 0; m < n; ++m) {
+        const int32_t v = w.list[base + m];
+        if (v > prev && v < next) next = v;
+      }
+      arrive(next);
+      prev = next;
     }
-    for (int k = 0; k < n; ++k) arrive(s[k]);
   } else {
-    for (long s = 0; s < N; ++s)
-      if (w.tgt[s] == (int32_t)c) arrive(s);
+    heap_sort(w.list + base, n);
+    for (int k = 0; k < n; ++k) arrive(w.list[base + k]);
   }
   w.raw[c] = e;
 }
@@ -239,6 +330,8 @@ extern "C" pm_status pm_depth_fuse_cpu_rules_workspace_size(int32_t width, int32
   clear_error();
   if (!bytes || width <= 0 || height <= 0 || width > 65535 || height > 65535)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse_cpu_rules_workspace_size: bad argument");
+  if ((long)width * height > 0x7fffffffL)
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse_cpu_rules_workspace_size: image too large");
   *bytes = cpu_fuse_work_bytes((long)width * height);
   return PM_OK;
 }
@@ -255,6 +348,8 @@ extern "C" pm_status pm_depth_fuse_cpu_rules(int32_t width, int32_t height, pm_c
   if (width <= 0 || height <= 0 || width > 65535 || height > 65535)
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse_cpu_rules: bad image size");
   const long N = (long)width * height;
+  if (N > 0x7fffffffL)   // cells and sources are int32 indices
+    return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse_cpu_rules: image too large");
   if (work_bytes < cpu_fuse_work_bytes(N))
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse_cpu_rules: workspace too small");
   if (pre.alpha == cur.alpha || pre.beta == cur.beta || pre.mu == cur.mu || pre.sig == cur.sig)
@@ -276,12 +371,22 @@ extern "C" pm_status pm_depth_fuse_cpu_rules(int32_t width, int32_t height, pm_c
   hipStream_t st = (hipStream_t)stream;
   const CpuFuseWork w = carve(work, N);
   const unsigned lin = (unsigned)((N + 255) / 256);
-  zero_i32_kernel<<<lin, 256, 0, st>>>(w.cnt, N);
+  const long blocks = scan_blocks(N);
+  zero_i32_kernel<<<(unsigned)((2 * N + 255) / 256), 256, 0, st>>>(w.cnt, 2 * N);   // cnt, cur
   pm_status s = check_launch("zero_i32_kernel");
   if (s != PM_OK) return s;
   const dim3 grid((width + 255) / 256, height);
   cpu_transform_kernel<<<grid, 256, 0, st>>>(a, pre.alpha, pre.beta, pre.mu, pre.sig, w);
   s = check_launch("cpu_transform_kernel");
+  if (s != PM_OK) return s;
+  cpu_scan_blocks_kernel<<<(unsigned)blocks, 256, 0, st>>>(N, w);
+  s = check_launch("cpu_scan_blocks_kernel");
+  if (s != PM_OK) return s;
+  cpu_scan_sums_kernel<<<1, 256, 0, st>>>(blocks, w);
+  s = check_launch("cpu_scan_sums_kernel");
+  if (s != PM_OK) return s;
+  cpu_scatter_kernel<<<lin, 256, 0, st>>>(N, w);
+  s = check_launch("cpu_scatter_kernel");
   if (s != PM_OK) return s;
   cpu_zbuffer_kernel<<<lin, 256, 0, st>>>(a, pre.mu, w);
   s = check_launch("cpu_zbuffer_kernel");

@@ -2,6 +2,7 @@
 //
 //   pm_evaluate IMAGES POSES OUT_DIR [CASE] [--fx F] [--fy F] [--cx F] [--cy F]
 //               [--agg-num N] [--interval N] [--planes N] [--dep-sample F] [--plan-only]
+//               [--no-frame-ring]
 //
 // IMAGES: lines "timestamp path" (TUM rgb.txt layout; '#' starts a comment; a relative path is
 // taken from IMAGES' directory), each an 8-bit binary PGM (P5).  POSES: lines
@@ -12,7 +13,8 @@
 // evaluate.cc:262-264) is written to OUT_DIR/<timestamp>.npy (float32 [H][W], little-endian)
 // and listed in OUT_DIR/depths.txt as "timestamp file reference-image source-images...".
 // CASE is the reference's command-line mode (evaluate.cc:313-319).  --plan-only prints the
-// steps without running the mapper (no GPU needed).
+// steps without running the mapper (no GPU needed).  --no-frame-ring uploads every frame at
+// every use, as the reference does, instead of once through the frame ring (same depths).
 #include <algorithm>
 #include <cctype>
 #include <stdexcept>
@@ -122,6 +124,7 @@ int main(int argc, char** argv) {
       else if (a == "--planes") cfg.planes = std::stoi(val());
       else if (a == "--dep-sample") cfg.dep_sample = std::stof(val());
       else if (a == "--plan-only") plan_only = true;
+      else if (a == "--no-frame-ring") cfg.frame_ring = false;
       else pos.push_back(a);
     } catch (const std::exception& e) {
       std::cerr << "pm_evaluate: " << e.what() << "\n";
@@ -131,7 +134,7 @@ int main(int argc, char** argv) {
   if (pos.size() < 3 || pos.size() > 4) {
     std::cerr << "usage: pm_evaluate IMAGES POSES OUT_DIR [CASE] [--fx F] [--fy F] [--cx F] "
                  "[--cy F] [--agg-num N] [--interval N] [--planes N] [--dep-sample F] "
-                 "[--plan-only]\n";
+                 "[--plan-only] [--no-frame-ring]\n";
     return 2;
   }
   if (pos.size() == 4) CASE = std::atoi(pos[3].c_str());

@@ -51,9 +51,15 @@ void upload_image(const Mat& img, const DeviceMat& dst, hipStream_t st, const ch
                                              std::to_string(dst.rows));
   if (img.type != CV_8U)
     throw Error(PM_ERR_INVALID_ARGUMENT, std::string(who) + ": image must be 8-bit grayscale");
-  hip_check(hipMemcpy2DAsync(dst.data, dst.step, img.data, img.step, (size_t)img.cols,
-                             (size_t)img.rows, hipMemcpyHostToDevice, st),
-            "upload");
+  // a dense image goes as one copy (a strided pageable copy moves row by row)
+  if (img.step == (size_t)img.cols && dst.step == (size_t)dst.cols)
+    hip_check(hipMemcpyAsync(dst.data, img.data, (size_t)img.cols * img.rows,
+                             hipMemcpyHostToDevice, st),
+              "upload");
+  else
+    hip_check(hipMemcpy2DAsync(dst.data, dst.step, img.data, img.step, (size_t)img.cols,
+                               (size_t)img.rows, hipMemcpyHostToDevice, st),
+              "upload");
 }
 
 void mat_to_array(const Mat& m, int rows, int cols, double* out, const char* what) {
@@ -172,14 +178,39 @@ void StereoMapper::initReference(const Mat& _img_l) {
     raw_cost_type_ = cost_type_;
   }
   homs_.clear();
+  staged_.clear();
   have_reference_ = true;
   cost_ready_ = false;
   depth_ready_ = false;
 }
 
-// stereo_mapper.cpp:73-91: stage a measurement frame (image + homography)
-void StereoMapper::update(const Mat& _img_r, const Mat& R_l, const Mat& T_l, const Mat& R_r,
-                          const Mat& T_r) {
+// initReference from an ingested frame: a device copy into img_l, ordered after the frame's
+// upload by its event
+void StereoMapper::initReference(const FrameRing& ring, int64_t frame) {
+  if (ring.width() != W_ || ring.height() != H_)
+    throw Error(PM_ERR_INVALID_ARGUMENT, "initReference: the ring's frames must be " +
+                                             std::to_string(W_) + "x" + std::to_string(H_));
+  hipStream_t st = (hipStream_t)stream_;
+  hip_check(hipStreamWaitEvent(st, (hipEvent_t)ring.ready_event(frame), 0), "hipStreamWaitEvent");
+  hip_check(hipMemcpyAsync(img_l.data, ring.image(frame), img_l.bytes(), hipMemcpyDeviceToDevice,
+                           st),
+            "copy reference");
+  if (raw_cost_type_ != cost_type_) {
+    synchronize();
+    dfree(raw_cost);
+    raw_cost = dalloc(1, (int)((long)W_ * H_ * D_), storage_type(cost_type_));
+    raw_cost_type_ = cost_type_;
+  }
+  homs_.clear();
+  staged_.clear();
+  have_reference_ = true;
+  cost_ready_ = false;
+  depth_ready_ = false;
+}
+
+// the homography of update (stereo_mapper.cpp:79-90), also kept in R, T as the reference does
+pm_homography StereoMapper::stage_homography(const Mat& R_l, const Mat& T_l, const Mat& R_r,
+                                             const Mat& T_r) {
   if (!have_reference_)
     throw Error(PM_ERR_INVALID_STATE, "update: initReference has not been called");
   if (nK1.empty()) throw Error(PM_ERR_INVALID_STATE, "update: initIntrinsic has not been called");
@@ -193,8 +224,23 @@ void StereoMapper::update(const Mat& _img_r, const Mat& R_l, const Mat& T_l, con
   mat_to_array(T_r, 3, 1, tr, "update: T_r");
   if (!homography_fp64(K, rl, tl, rr, tr, Hm, t))
     throw Error(PM_ERR_INVALID_ARGUMENT, "update: non-finite pose or singular K");
+  pm_homography h;
+  for (int k = 0; k < 9; ++k) h.h[k] = (float)Hm[k];
+  for (int k = 0; k < 3; ++k) h.t[k] = (float)t[k];
+  R.create(3, 3, CV_64F);
+  T.create(3, 1, CV_64F);
+  for (int k = 0; k < 9; ++k) R.at<double>(k / 3, k % 3) = Hm[k];
+  for (int k = 0; k < 3; ++k) T.at<double>(k, 0) = t[k];
+  return h;
+}
+
+// stereo_mapper.cpp:73-91: stage a measurement frame (image + homography)
+void StereoMapper::update(const Mat& _img_r, const Mat& R_l, const Mat& T_l, const Mat& R_r,
+                          const Mat& T_r) {
+  const pm_homography h = stage_homography(R_l, T_l, R_r, T_r);
   const size_t slot = homs_.size();
-  if (quads_.size() <= slot) quads_.push_back(dalloc(H_, 2 * W_, CV_32S));   // 8 B texels
+  if (quads_.size() <= slot) quads_.resize(slot + 1);   // ring-staged slots stay unallocated
+  if (!quads_[slot].data) quads_[slot] = dalloc(H_, 2 * W_, CV_32S);   // 8-byte texels
   hipStream_t st = (hipStream_t)stream_;
   // the u8 frame goes through the filtered-depth buffer's memory, which is free until output
   DeviceMat staging = wta_tmp_;
@@ -203,17 +249,28 @@ void StereoMapper::update(const Mat& _img_r, const Mat& R_l, const Mat& T_l, con
   upload_image(_img_r, staging, st, "update");
   check(pm_make_quad(staging.data, reinterpret_cast<uint64_t*>(quads_[slot].data), W_, H_,
                      stream_));
-  pm_homography h;
-  for (int k = 0; k < 9; ++k) h.h[k] = (float)Hm[k];
-  for (int k = 0; k < 3; ++k) h.t[k] = (float)t[k];
   homs_.push_back(h);
-  R.create(3, 3, CV_64F);
-  T.create(3, 1, CV_64F);
-  for (int k = 0; k < 9; ++k) R.at<double>(k / 3, k % 3) = Hm[k];
-  for (int k = 0; k < 3; ++k) T.at<double>(k, 0) = t[k];
+  staged_.push_back(RingFrame{nullptr, -1});
   cost_ready_ = false;
   depth_ready_ = false;
   synchronize();   // the caller may reuse its image buffer after update returns
+}
+
+// update from an ingested frame: only the homography is set up here; the cost pass reads
+// the frame's texture from the ring
+void StereoMapper::update(const FrameRing& ring, int64_t frame, const Mat& R_l, const Mat& T_l,
+                          const Mat& R_r, const Mat& T_r) {
+  if (ring.width() != W_ || ring.height() != H_)
+    throw Error(PM_ERR_INVALID_ARGUMENT, "update: the ring's frames must be " +
+                                             std::to_string(W_) + "x" + std::to_string(H_));
+  if (!ring.holds(frame))
+    throw Error(PM_ERR_INVALID_STATE, "update: frame " + std::to_string(frame) +
+                                          " is not in the ring");
+  const pm_homography h = stage_homography(R_l, T_l, R_r, T_r);
+  homs_.push_back(h);
+  staged_.push_back(RingFrame{&ring, frame});
+  cost_ready_ = false;
+  depth_ready_ = false;
 }
 
 void StereoMapper::ensure_cost() {
@@ -221,7 +278,20 @@ void StereoMapper::ensure_cost() {
   if (!have_reference_)
     throw Error(PM_ERR_INVALID_STATE, "output: initReference has not been called");
   std::vector<const uint64_t*> q(homs_.size());
-  for (size_t s = 0; s < homs_.size(); ++s) q[s] = reinterpret_cast<const uint64_t*>(quads_[s].data);
+  hipStream_t st = (hipStream_t)stream_;
+  for (size_t s = 0; s < homs_.size(); ++s) {
+    const RingFrame& f = staged_[s];
+    if (!f.ring) {
+      q[s] = reinterpret_cast<const uint64_t*>(quads_[s].data);
+      continue;
+    }
+    if (!f.ring->holds(f.frame))
+      throw Error(PM_ERR_INVALID_STATE, "output: staged frame " + std::to_string(f.frame) +
+                                            " was evicted from its ring before the output call");
+    hip_check(hipStreamWaitEvent(st, (hipEvent_t)f.ring->ready_event(f.frame), 0),
+              "hipStreamWaitEvent");
+    q[s] = f.ring->quad(f.frame);
+  }
   check(pm_calc_cost(pm_shape{W_, H_, D_}, img_l.data, (int32_t)homs_.size(), q.data(),
                      homs_.data(), IDEP_MIN, DEP_SAMPLE, raw_cost.data, raw_cost_type_,
                      stream_));
@@ -506,6 +576,28 @@ extern "C" pm_status pm_mapper_update(pm_mapper* m, const uint8_t* image, const 
   return guard([&] {
     m->impl->update(wrap_image(image, m->impl->height(), m->impl->width()), wrap_d(R_l, 3, 3),
                     wrap_d(T_l, 3, 1), wrap_d(R_r, 3, 3), wrap_d(T_r, 3, 1));
+  });
+}
+
+extern "C" pm_status pm_mapper_init_reference_frame(pm_mapper* m, const pm_frame_ring* ring,
+                                                    int64_t frame) {
+  PM_MAPPER_CHECK(m);
+  const pm::FrameRing* r = pm::ring_of(ring);
+  if (!r) return pm::fail(PM_ERR_INVALID_ARGUMENT, "null ring");
+  return guard([&] { m->impl->initReference(*r, frame); });
+}
+
+extern "C" pm_status pm_mapper_update_frame(pm_mapper* m, const pm_frame_ring* ring,
+                                            int64_t frame, const double R_l[9],
+                                            const double T_l[3], const double R_r[9],
+                                            const double T_r[3]) {
+  PM_MAPPER_CHECK(m);
+  const pm::FrameRing* r = pm::ring_of(ring);
+  if (!r || !R_l || !T_l || !R_r || !T_r)
+    return pm::fail(PM_ERR_INVALID_ARGUMENT, "null argument");
+  return guard([&] {
+    m->impl->update(*r, frame, wrap_d(R_l, 3, 3), wrap_d(T_l, 3, 1), wrap_d(R_r, 3, 3),
+                    wrap_d(T_r, 3, 1));
   });
 }
 

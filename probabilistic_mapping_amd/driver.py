@@ -31,10 +31,11 @@ class FrameDriver:
     """``with_mapper=False`` plans the steps only (host code, no GPU)."""
 
     def __init__(self, agg_num=3, interval=6, planes=64, fx=481.2017, fy=-480.0002, cx=319.5,
-                 cy=239.5, dep_sample=1.0 / (0.15 * 160.0), with_mapper=True):
+                 cy=239.5, dep_sample=1.0 / (0.15 * 160.0), with_mapper=True, frame_ring=True):
         self.lib = C.load()
         self.agg_num = agg_num
-        cfg = C.DriverConfig(agg_num, interval, planes, fx, fy, cx, cy, dep_sample)
+        cfg = C.DriverConfig(agg_num, interval, planes, fx, fy, cx, cy, dep_sample,
+                             1 if frame_ring else 0)
         h = ctypes.c_void_p()
         C.check(self.lib.pm_driver_create(ctypes.byref(cfg), 1 if with_mapper else 0,
                                           ctypes.byref(h)))
@@ -79,6 +80,13 @@ class FrameDriver:
         return stamp.value, ref.value, list(srcs)[:self.agg_num], depth
 
     @property
+    def ring_puts(self) -> int:
+        """Frames uploaded through the frame ring so far (0 without one)."""
+        n = ctypes.c_int64(0)
+        C.check(self.lib.pm_driver_ring_puts(self._h, ctypes.byref(n)))
+        return n.value
+
+    @property
     def skipped(self) -> int:
         n = ctypes.c_int32(0)
         C.check(self.lib.pm_driver_skipped(self._h, ctypes.byref(n)))
@@ -115,7 +123,7 @@ def write_dataset(root: str, stamps, images, poses) -> tuple[str, str]:
 
 
 def run_evaluate(img_list: str, pose_list: str, out_dir: str, case=None, plan_only=False,
-                 timeout=600, **cfg) -> list:
+                 frame_ring=True, timeout=600, **cfg) -> list:
     """Runs the executable; returns its steps as (stamp text, ref, [srcs], depth or None)."""
     args = [EVALUATE, img_list, pose_list, out_dir]
     if case is not None:
@@ -124,6 +132,8 @@ def run_evaluate(img_list: str, pose_list: str, out_dir: str, case=None, plan_on
         args += ["--" + k.replace("_", "-"), repr(v)]
     if plan_only:
         args.append("--plan-only")
+    if not frame_ring:
+        args.append("--no-frame-ring")
     r = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"pm_evaluate failed ({r.returncode}): {r.stderr}")

@@ -40,6 +40,54 @@ def set_knobs(**kw) -> dict:
     return old
 
 
+class FrameRing:
+    """Upload-once frame ring (``pm_frame_ring_*``): frame ``f`` is uploaded and converted
+    once and stays on the device in slot ``f % capacity`` until another frame takes it."""
+
+    def __init__(self, width: int, height: int, capacity: int):
+        self._lib = C.load()
+        h = ctypes.c_void_p()
+        C.check(self._lib.pm_frame_ring_create(width, height, capacity, ctypes.byref(h)))
+        self._h = h
+        self.width, self.height, self.capacity = width, height, capacity
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pm_frame_ring_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def put(self, frame: int, image, stream=None):
+        img = np.ascontiguousarray(image)
+        if img.dtype != np.uint8 or img.shape != (self.height, self.width):
+            raise C.PmError(C.PM_ERR_INVALID_ARGUMENT,
+                            f"image must be uint8 {self.height}x{self.width}")
+        C.check(self._lib.pm_frame_ring_put(self._h, int(frame),
+                                            img.ctypes.data_as(ctypes.c_void_p), stream))
+
+    def device_pointers(self, frame: int) -> tuple[int, int]:
+        """(u8 image, quad texture) device addresses of a held frame."""
+        img, quad = ctypes.c_void_p(), ctypes.c_void_p()
+        C.check(self._lib.pm_frame_ring_frame(self._h, int(frame), ctypes.byref(img),
+                                              ctypes.byref(quad)))
+        return img.value, quad.value
+
+    def holds(self, frame: int) -> bool:
+        try:
+            self.device_pointers(frame)
+            return True
+        except C.PmError:
+            return False
+
+    @property
+    def puts(self) -> int:
+        n = ctypes.c_int64(0)
+        C.check(self._lib.pm_frame_ring_puts(self._h, ctypes.byref(n)))
+        return n.value
+
+
 class StereoMapper:
     def __init__(self, width: int = WIDTH, height: int = HEIGHT, dep_cnt: int = DEP_CNT):
         self._lib = C.load()
@@ -80,6 +128,17 @@ class StereoMapper:
         a = [_d(R_l, 9), _d(T_l, 3), _d(R_r, 9), _d(T_r, 3)]
         C.check(self._lib.pm_mapper_update(self._h, img.ctypes.data_as(ctypes.c_void_p),
                                            *[p for _, p in a]))
+
+    def initReferenceFrame(self, ring: FrameRing, frame: int):
+        """initReference with frame ``frame`` of ``ring`` (no upload)."""
+        C.check(self._lib.pm_mapper_init_reference_frame(self._h, ring._h, int(frame)))
+
+    def updateFrame(self, ring: FrameRing, frame: int, R_l, T_l, R_r, T_r):
+        """update with frame ``frame`` of ``ring``; it must stay in the ring until the next
+        output call returns."""
+        a = [_d(R_l, 9), _d(T_l, 3), _d(R_r, 9), _d(T_r, 3)]
+        C.check(self._lib.pm_mapper_update_frame(self._h, ring._h, int(frame),
+                                                 *[p for _, p in a]))
 
     def _result(self):
         return np.empty((self.height, self.width), np.float32)

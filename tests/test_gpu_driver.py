@@ -96,3 +96,11 @@ def test_evaluate_executable_matches_driver(seq, tmp_path):
     for (st, ref, srcs, depth), (t, ref2, srcs2, depth2) in zip(got, steps):
         assert st == driver.stamp_text(t) and ref == ref2 and srcs == srcs2
         assert np.array_equal(depth, depth2), ref
+    # uploading every frame at every use (--no-frame-ring) gives the same files
+    out2 = tmp_path / "out_per_use"
+    out2.mkdir()
+    got2 = driver.run_evaluate(img_list, pose_list, str(out2), case=3, frame_ring=False,
+                               **seq["cfg"])
+    assert len(got2) == len(got)
+    for a, b in zip(got, got2):
+        assert a[:3] == b[:3] and np.array_equal(a[3], b[3])

@@ -267,6 +267,49 @@ def test_fusion_cpu_rules_same_depths(dev, trajectory_scene):
         pre ^= 1
 
 
+@pytest.mark.parametrize("back", [0.5, 5.0, 10.0, 40.0])
+def test_fusion_cpu_rules_crowded_cells(dev, back):
+    """The CPU rules' z-buffer when the camera backs away along its axis by ``back`` (depths
+    1-2): up to ~6, ~30, ~60 and ~800 sources share a cell near the centre, so cells take the
+    short ordering path and the heap sort; the raster-order fold must still give the oracle's
+    table and state."""
+    H, W = 64, 96
+    cam = (70.0, 70.0, 47.5, 31.5)
+    rng = np.random.default_rng(int(back * 10))
+    pre = {"alpha": rng.uniform(2.0, 15.0, (H, W)), "beta": rng.uniform(0.5, 3.0, (H, W)),
+           "mu": rng.uniform(1.0, 2.0, (H, W)), "sig": rng.uniform(0.01, 1.0, (H, W))}
+    pre["alpha"][rng.random((H, W)) < 0.1] = 0.0
+    pre = {k: v.astype(np.float32).astype(np.float64) for k, v in pre.items()}
+    meas = rng.uniform(1.0, 8.0, (H, W)).astype(np.float32)
+    meas[rng.random((H, W)) < 0.2] = 0.0
+    r = np.eye(3)
+    t = np.array([0.0, 0.0, back])
+    ost = [dict((k, v.copy()) for k, v in pre.items()),
+           {k: np.zeros((H, W)) for k in ("alpha", "beta", "mu", "sig")}]
+    od = meas.astype(np.float64)
+    otab = oracle.depth_fuse_cpu_rules(od, ost[0], ost[1], r, t, cam, "f64")
+    gst = [dev.new_state(H, W, "cuda") for _ in range(2)]
+    for k in ("alpha", "beta", "mu", "sig"):
+        gst[0][k].copy_(torch.from_numpy(pre[k].astype(np.float32)))
+    table = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    gd = torch.from_numpy(meas.copy()).cuda()
+    dev.depth_fuse_cpu_rules(gd, gst[0], gst[1], r, t, cam, table=table)
+    gtab = table.cpu().numpy()
+    st = parity.state_parity({k: v.cpu().numpy() for k, v in gst[1].items()}, ost[1])
+    st["table_mismatch"] = float((gtab != otab).mean())
+    st["fused"] = parity.depth_parity(gd.cpu().numpy(), od)
+    # the displaced occupants' means rewritten in the previous state
+    st["pre_mu_mismatch"] = float((np.abs(gst[0]["mu"].cpu().numpy() - ost[0]["mu"]) >
+                                   1e-5 * np.abs(ost[0]["mu"])).mean())
+    _report(f"fusion (CPU rules) back {back}", st)
+    assert st["table_mismatch"] == 0.0
+    assert st["pre_mu_mismatch"] == 0.0
+    assert st["init_mismatch"] <= 1e-3
+    for k in ("mu", "sig", "inlier"):
+        assert st[k]["p999"] <= 1e-4, k
+    assert st["fused"]["category_mismatch"] <= 1e-3
+
+
 @pytest.mark.parametrize("paths", [4, 8])
 @pytest.mark.parametrize("planes", [32, 64, 96, 128])
 def test_sgm_filter_equals_sgm_then_filter(dev, small_scene, paths, planes):
