@@ -271,14 +271,21 @@ __device__ __forceinline__ float wave_shl1z(float v) {
                                                                0xF, 0xF, true));
 }
 
-// Minimum over the 64 lanes of a non-negative float, in every lane.  Inside each row of 16
-// lanes: four DPP stages (quad_perm [1,0,3,2], [2,3,0,1], row_ror:4, row_ror:8) that fold into
-// v_min_u32_dpp (every source lane exists, so bound_ctrl changes nothing); across rows: the
-// gfx950 permlane32 / permlane16 swaps exchange the row minima.  Ten VALU instructions, no
-// readlane.
+// Minimum over the 64 lanes of a non-negative float, wave-uniform (a scalar register).
+// Inside each row of 16 lanes: four DPP stages (quad_perm [1,0,3,2], [2,3,0,1], row_ror:4,
+// row_ror:8) that fold into v_min_u32_dpp (every source lane exists, so bound_ctrl changes
+// nothing).  Across rows: row_bcast:15 (rows 1 and 3 take the previous row's minimum) and
+// row_bcast:31 (rows 2 and 3 take row 1's), leaving the minimum of all 64 lanes in lane 63,
+// which a readlane moves to a scalar register.  The broadcasts' DPP "old" value is 0xFFFFFFFF,
+// the identity of the unsigned minimum, so they too fold into v_min_u32_dpp: seven VALU
+// instructions.
 template <int CTRL>
 __device__ __forceinline__ uint32_t umin_dpp(uint32_t u) {
   return min(u, (uint32_t)__builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, true));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t umin_bcast(uint32_t u) {
+  return min(u, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)u, CTRL, ROWS, 0xF, false));
 }
 __device__ __forceinline__ float wave_min_nonneg(float v) {
   uint32_t u = __float_as_uint(v);
@@ -286,10 +293,9 @@ __device__ __forceinline__ float wave_min_nonneg(float v) {
   u = umin_dpp<0x4E>(u);
   u = umin_dpp<0x124>(u);
   u = umin_dpp<0x128>(u);
-  const auto h = __builtin_amdgcn_permlane32_swap(u, u, false, false);
-  u = min(h[0], h[1]);
-  const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
-  return __uint_as_float(min(r[0], r[1]));
+  u = umin_bcast<0x142, 0xA>(u);   // row_bcast:15 into rows 1 and 3
+  u = umin_bcast<0x143, 0xC>(u);   // row_bcast:31 into rows 2 and 3
+  return __uint_as_float(__builtin_amdgcn_readlane(u, 63));
 }
 
 // min(lane i-1's `lo`, lane i+1's `hi`) for the neighbouring planes of the SGM recurrence; a

@@ -11,6 +11,44 @@ namespace pm {
 __device__ __forceinline__ uint32_t ref_bits(uint8_t v) { return v; }
 __device__ __forceinline__ uint32_t ref_bits(float v) { return __float_as_uint(v); }
 
+// A cost element as the scan holds it between its load and its use.  The recurrence adds it
+// as fma(c, w, x) with a wave-uniform w: 1 at a valid pixel (c * 1 + x, one rounding, the
+// value of c + x) and 0 at an invalid one (x, the reference's COST_INF = 0 in place of c;
+// costs are finite).  w comes from a scalar register the compiler cannot see into, so the
+// FMA stays an FMA: for FP16 costs, held as raw 16-bit values, it becomes v_fma_mix and the
+// conversion disappears; for every type the per-lane select of the cost disappears.
+template <typename CT>
+struct CostSlot {
+  using T = float;
+  static __device__ __forceinline__ T load(const Buf& b, uint32_t voff, uint32_t soff) {
+    return buf_cost<CT>(b, voff, soff);
+  }
+  static __device__ __forceinline__ bool negative(T c) { return c < 0.0f; }
+  static __device__ __forceinline__ float value(T c) { return c; }
+  static __device__ __forceinline__ float add(T c, float x, float w) { return fmaf(c, w, x); }
+};
+template <>
+struct CostSlot<__half> {
+  using T = _Float16;
+  static __device__ __forceinline__ T load(const Buf& b, uint32_t voff, uint32_t soff) {
+    const unsigned short u = (unsigned short)b.u16(voff, soff);
+    T h;
+    __builtin_memcpy(&h, &u, 2);
+    return h;
+  }
+  static __device__ __forceinline__ float value(T c) { return (float)c; }
+  static __device__ __forceinline__ bool negative(T c) { return c < (T)0; }
+  static __device__ __forceinline__ float add(T c, float x, float w) {
+    return fmaf((float)c, w, x);
+  }
+};
+
+__device__ __forceinline__ float opaque_one() {
+  float one;
+  asm("s_mov_b32 %0, 1.0" : "=s"(one));
+  return one;
+}
+
 __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, int& y) {
   if (a.dx != 0) {
     const int xs = a.dx > 0 ? 0 : a.W - 1;
@@ -71,9 +109,13 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
 
   // two register buffers of B steps each, used ping-pong: while one block computes, the
   // loads of the next are in flight (addresses past the end re-read the last pixel)
-  float c0[B][NV], g0[B][NV], c1[B][NV], g1[B][NV];
+  using CS = CostSlot<CT>;
+  using CV = typename CS::T;
+  const float one = opaque_one();
+  CV c0[B][NV], c1[B][NV];
+  float g0[B][NV], g1[B][NV];
   uint32_t r0, r1;
-  auto load_block = [&](int base, float (&c)[B][NV], float (&g)[B][NV], uint32_t& r) {
+  auto load_block = [&](int base, CV (&c)[B][NV], float (&g)[B][NV], uint32_t& r) {
     // lane i fetches the reference intensity of step base+i; issued first so that reading it
     // does not wait for the block's cost and aggregate loads
     r = ref_bits(ref[p0 + min(base + lane, n - 1) * pstep]);
@@ -83,7 +125,7 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       const uint32_t p = (uint32_t)(p0 + min(base + i, n - 1) * pstep) * (uint32_t)D;
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
-        c[i][j] = buf_cost<CT>(bcost, lb(j, sizeof(CT)), p * (uint32_t)sizeof(CT));
+        c[i][j] = CS::load(bcost, lb(j, sizeof(CT)), p * (uint32_t)sizeof(CT));
         if (!FIRST && AGG) g[i][j] = bagg.f32(lb(j, 4), p * 4u);
       }
     }
@@ -103,12 +145,15 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       return (uint64_t)__ballot(abs((int)r - (int)prev) < a.tau_i);
   };
   // one step k >= 1 from slot i of a block buffer
-  auto step = [&](int k, int i, const float (&c)[B][NV], const float (&g)[B][NV]) {
+  auto step = [&](int k, int i, const CV (&c)[B][NV], const float (&g)[B][NV]) {
     const uint32_t p = (uint32_t)(p0 + k * pstep) * (uint32_t)D;
     bool neg = false;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) neg = neg || c[i][j] < 0.0f;
+    for (int j = 0; j < NV; ++j) neg = neg || CS::negative(c[i][j]);
     const bool invalid = __ballot(neg) != 0;
+    static_assert(kCostInf == 0.0f, "an invalid pixel adds 0 in place of its cost");
+    // wave-uniform selects written on integer bits, which the compiler keeps scalar
+    const float w = __uint_as_float(invalid ? 0u : __float_as_uint(one));
     // m = min_d L(q, d) (sgm.cu:55-66)
     float lm = L[0];
 #pragma unroll
@@ -116,8 +161,8 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
     const float m = wave_min_nonneg(lm);
     // gradient-adaptive penalties (sgm.cu:78-82), scalar
     const bool sm = (smooth >> i) & 1ull;
-    const float P1 = sm ? a.P1s : a.P1n;
-    const float P2 = sm ? a.P2s : a.P2n;
+    const float P1 = __uint_as_float(sm ? __float_as_uint(a.P1s) : __float_as_uint(a.P1n));
+    const float P2 = __uint_as_float(sm ? __float_as_uint(a.P2s) : __float_as_uint(a.P2n));
     // L(q, d0-1) and L(q, d0+NV) from the neighbouring lanes (0xFFFFFFFF where none)
     const uint32_t lo = shr1_max(L[NV - 1]);
     const uint32_t hi = shl1_max(L[0]);
@@ -129,8 +174,7 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       const uint32_t right = j + 1 < NV ? __float_as_uint(L[j + 1]) : hi;
       // min(L(q,d), L(q,d-1)+P1, L(q,d+1)+P1, m+P2) (sgm.cu:85-93)
       const float val = umin(umin(L[j], mP2), __uint_as_float(min(left, right)) + P1);
-      const float cc = invalid ? kCostInf : c[i][j];
-      Ln[j] = act[j] ? (cc + val) - m : INF;       // sgm.cu:98
+      Ln[j] = act[j] ? CS::add(c[i][j], val, w) - m : INF;   // sgm.cu:98
       const float out = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);
       if (AGG && act[j]) bagg.put_f32(out, lb(j, 4), p * 4u);   // sgm.cu:101-108
       if (LOUT && act[j]) blout.put_f32((LMASK && invalid) ? kCostInf : Ln[j], lb(j, 4), p * 4u);
@@ -145,12 +189,12 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
     for (int j = 0; j < NV; ++j) L[j] = Ln[j];
   };
   // a block whose B steps all exist: no exits, so the loads of the other buffer stay in flight
-  auto run_full = [&](int base, const float (&c)[B][NV], const float (&g)[B][NV], uint32_t r) {
+  auto run_full = [&](int base, const CV (&c)[B][NV], const float (&g)[B][NV], uint32_t r) {
     smooth = smooth_mask(r);
 #pragma unroll
     for (int i = 0; i < B; ++i) step(base + i, i, c, g);
   };
-  auto run_checked = [&](int base, const float (&c)[B][NV], const float (&g)[B][NV],
+  auto run_checked = [&](int base, const CV (&c)[B][NV], const float (&g)[B][NV],
                          uint32_t r) {
     smooth = smooth_mask(r);
 #pragma unroll
