@@ -3,7 +3,8 @@
 Workload (synthetic, seeded): 752x480 uint8 frames of a scene of 12 random textured planes and
 boxes at 0.5-8 m, fx=fy=460, a 20-frame sinusoidal trajectory with 3-15 cm spacing; each
 keyframe uses its 5 nearest frames as measurement frames; 64 planes over inverse depth
-0.125-2.0; 8-path SGM; WTA + subpixel + filter; Gaussian x Beta fusion over all 20 frames.
+0.125-2.0; 8-path SGM; WTA + subpixel + filter; 3x3 median (the median/consistency filter,
+--median 0 drops it); Gaussian x Beta fusion over all 20 frames.
 
 One step = one pass of the whole 20-keyframe sequence (cost over 5 sources -> SGM -> filter ->
 fusion per keyframe) with every frame already resident in HBM.  value = keyframes / second.
@@ -37,7 +38,8 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed)
 METRIC = "fused depth frames/s @752x480x64 planes, plain vs FP16-batched; % HBM peak"
 WORKLOAD = ("752x480 frames, 64 planes (idep 0.125-2.0), 5 sources, 8-path SGM, filter, "
-            "Gaussian x Beta fusion, 20-keyframe sequence per step")
+            "3x3 median (unless --median 0), Gaussian x Beta fusion, 20-keyframe sequence "
+            "per step")
 COST_BYTES = {"f32": 4, "f16": 2, "bf16": 2}
 # HBM bytes per keyframe of the SGM pass (paths launch + combine/filter launch) on this
 # workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales measured on a 1 GiB device
@@ -58,7 +60,9 @@ def parse():
     ap.add_argument("--sgm-storage", default=None, choices=["exact", "compact", "sweep"],
                     help="SGM per-path storage (default: exact for f32 cost, compact otherwise; "
                          "sweep exists in the batched form only)")
-    ap.add_argument("--median", type=int, default=0)
+    ap.add_argument("--median", type=int, default=1,
+                    help="3x3 median over the filtered depth before fusion (the workload's "
+                         "median/consistency filter; 0 disables it)")
     ap.add_argument("--cpu-baseline-seconds", type=float, default=12.0,
                     help="bound on the CPU baseline sample (0 disables it)")
     ap.add_argument("--seed", type=int, default=0)
@@ -74,7 +78,7 @@ def default_storage(cost_type: str, storage: str | None) -> str:
     return storage or ("exact" if cost_type == "f32" else "compact")
 
 
-def cpu_baseline(scene, seconds: float) -> dict | None:
+def cpu_baseline(scene, seconds: float, median: bool) -> dict | None:
     """float/OpenMP oracle over as many keyframes of the workload as fit in ~`seconds`."""
     if seconds <= 0:
         return None
@@ -85,7 +89,7 @@ def cpu_baseline(scene, seconds: float) -> dict | None:
     s = scene
     seq = OracleSequence(s["images"], s["poses"], s["sources"], s["scene"].K, s["planes"],
                          s["idep_min"], s["idep_max"], s["knobs"], s["paths"],
-                         precision="f32")
+                         median=median, precision="f32")
     done = 0
     t0 = time.perf_counter()
     while done < len(s["images"]):
@@ -463,7 +467,7 @@ def main():
         if args.form == "both":
             torch.cuda.empty_cache()
             extra = measure_batched(args, torch, s, "f16", "compact")
-    base = cpu_baseline(s, args.cpu_baseline_seconds)
+    base = cpu_baseline(s, args.cpu_baseline_seconds, bool(args.median))
     print(json.dumps(report(args, main_r, extra, copy, base)), flush=True)
 
 

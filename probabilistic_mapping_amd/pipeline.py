@@ -503,12 +503,20 @@ class BatchedSequence(KeyframeSequence):
                   "dep_sample": self.dep_sample} for i in range(len(self.images))]
         self.batch = Batch(specs, self.cost_type, self.knobs, self.paths, self.sgm_storage,
                            write_agg=self.write_agg)
+        self.dep_med = torch.empty((H, W), dtype=torch.float32, device=self.device) \
+            if self.median else None
 
     def depth_of(self, i: int) -> torch.Tensor:
+        """Keyframe i's filtered depth as the batch wrote it (before the median, if any)."""
         return self.batch.outputs[i]["depth"]
 
     def fusion_step(self, i: int) -> None:
-        self.dep = self.depth_of(i)
+        """(median ->) fusion of keyframe i, after the batch has run."""
+        d = self.depth_of(i)
+        if self.median:
+            median3x3(d, out=self.dep_med)
+            d = self.dep_med
+        self.dep = d
         self.fusion_stage(i)
 
     def run(self) -> None:
