@@ -310,8 +310,10 @@ pm_status pm_mapper_init_reference(pm_mapper* m, const uint8_t* image);
 pm_status pm_mapper_update(pm_mapper* m, const uint8_t* image, const double R_l[9],
                            const double T_l[3], const double R_r[9], const double T_r[3]);
 /* initReference / update reading frame `frame` of a ring (same results as passing its host
- * image; no upload).  A frame staged by update must stay in the ring until the next output
- * call returns, else that call fails with PM_ERR_INVALID_STATE. */
+ * image; no upload).  Both only enqueue work on the mapper's stream, so a frame passed to
+ * either must stay in the ring until the next output call returns: the output call fails with
+ * PM_ERR_INVALID_STATE if a staged source was evicted, and a put into the reference's slot
+ * on a stream not ordered after the mapper's races with its copy. */
 pm_status pm_mapper_init_reference_frame(pm_mapper* m, const pm_frame_ring* ring,
                                          int64_t frame);
 pm_status pm_mapper_update_frame(pm_mapper* m, const pm_frame_ring* ring, int64_t frame,
