@@ -68,6 +68,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the attainable-bandwidth copy (0 disables it)")
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="both/batched forms: also time this many independent 20-keyframe "
+                         "sequences in flight on as many streams, each with its own buffers "
+                         "and captured graph (reported beside the one-at-a-time figure; 1 "
+                         "disables it)")
     ap.add_argument("--tuning", default=None,
                     help="batched form: launch shapes as JSON (pm_batch_tuning fields) instead "
                          "of running the autotuner")
@@ -251,6 +256,8 @@ def measure_batched(args, torch, s, cost_type, storage):
             torch.cuda.synchronize()
             acc += [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
     ms = acc / reps / nkf
+    inflight = measure_inflight(args, torch, s, seq, cost_type, storage, tuning) \
+        if args.inflight > 1 else None
     # the SGM pass is stage 1 (paths) + stage 2 (filter; compact: combine + filter)
     stages = stage_report(seq, s, cost_type, ms[0], ms[1] + ms[2], ms[3], storage)
     stages["sgm"]["paths_ms"] = float(ms[1])
@@ -259,7 +266,55 @@ def measure_batched(args, torch, s, cost_type, storage):
             "stages": stages, "sgm_bytes": sgm_pass_bytes(seq, cost_type),
             "form": "batched (20 keyframes per hipGraph replay)", "cost_storage": cost_type,
             "sgm_storage": storage, "keyframes_per_step": nkf, "tuning": tuning,
-            "traffic": SGM_PASS_TRAFFIC.get(("batched", cost_type, storage))}
+            "traffic": SGM_PASS_TRAFFIC.get(("batched", cost_type, storage)),
+            "inflight": inflight}
+
+
+def measure_inflight(args, torch, s, first, cost_type, storage, tuning):
+    """Several independent 20-keyframe sequences in flight: sequence j of the timed steps runs
+    on stream j % n with its own batch buffers, captured graph and fusion state, so one
+    sequence's fusion steps (small, latency-bound launches) overlap the next one's batch.
+    Every step still computes a whole sequence; the figure is keyframes/s over all of them."""
+    from probabilistic_mapping_amd import pipeline as pl
+    n = args.inflight
+    nkf = len(s["images"])
+    seqs, streams = [first], [torch.cuda.Stream()]   # first: already captured
+    for _ in range(1, n):
+        st = torch.cuda.Stream()
+        with torch.cuda.stream(st):
+            q = pl.BatchedSequence(s["images"], s["poses"], s["sources"], s["scene"].K,
+                                   s["planes"], s["idep_min"], s["idep_max"], s["knobs"],
+                                   s["paths"], cost_type=cost_type, median=bool(args.median),
+                                   sgm_storage=storage, write_agg=False)
+            q.batch.tuning = tuning
+            q.batch.capture()
+        seqs.append(q)
+        streams.append(st)
+    torch.cuda.synchronize()
+
+    def step(j):
+        q, st = seqs[j % n], streams[j % n]
+        with torch.cuda.stream(st):
+            q.reset()
+            q.batch.launch()
+            for i in range(nkf):
+                q.fusion_step(i)
+
+    for j in range(max(args.warmup, 1) * n):
+        step(j)
+    torch.cuda.synchronize()
+    steps = max(args.steps, n)
+    t0 = time.perf_counter()
+    for j in range(steps):
+        step(j)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    same = all(torch.equal(q.fused, seqs[0].fused) for q in seqs[1:])
+    for q in seqs[1:]:
+        q.batch.close()
+    return {"sequences_in_flight": n, "value": steps * nkf / elapsed,
+            "ms_per_step": elapsed / steps * 1e3, "steps": steps,
+            "same_fused_result": bool(same)}
 
 
 def report(args, main: dict, extra: dict | None, copy, base) -> dict:
@@ -291,16 +346,20 @@ def report(args, main: dict, extra: dict | None, copy, base) -> dict:
     }
     if "tuning" in main:
         out["config"]["tuning"] = main["tuning"]
+    if main.get("inflight"):
+        out["inflight"] = main["inflight"]
     if extra is not None:
         out["fp16_batched"] = {k: extra[k] for k in ("value", "ms_per_step", "form",
                                                      "cost_storage", "sgm_storage", "stages",
-                                                     "tuning")}
+                                                     "tuning", "inflight")}
         out["fp16_batched"]["roofline"] = {
             "bound": "hbm", "kernel": "sgm pass (compact paths launch + combine/filter)",
             "achieved": extra["stages"]["sgm"]["achieved_gbs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": extra["stages"]["sgm"]["frac"],
             "algorithmic_bytes": extra["sgm_bytes"], "traffic": extra["traffic"]}
         out["fp16_batched_vs_plain"] = extra["value"] / main["value"]
+        if extra.get("inflight"):
+            out["fp16_batched_inflight_vs_plain"] = extra["inflight"]["value"] / main["value"]
     return out
 
 
