@@ -499,7 +499,7 @@ def bench_driver(args, torch):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+        print(json.dumps({"error": "this library runs on one GPU only; --gpus must be 1 "
                                    "(keyframe streams would shard one stream per GPU)"}))
         sys.exit(2)
     import torch

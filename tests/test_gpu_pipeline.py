@@ -1,6 +1,6 @@
 """End-to-end parity of the GPU pipeline against the FP64 oracle (needs a GPU).
 
-Bars (north-star acceptance for the plain FP32 form):
+Acceptance bars for the plain FP32 form:
   * WTA plane index identical on >= 99.9 % of pixels;
   * filtered depth category (rejected / non-overlap / value) identical on >= 99.9 % of pixels;
   * subpixel depth, fused mean, variance and inlier ratio: relative error <= 1e-4 on the
