@@ -124,6 +124,55 @@ def test_sgm_same_cost(dev, small_scene, paths, planes):
         (parity.wta_index(g) != parity.wta_index(o)).mean() <= 1e-4
 
 
+@pytest.mark.parametrize("W,H,D", [(9, 6, 16), (40, 3, 32), (63, 12, 256), (70, 17, 64),
+                                   (130, 7, 96), (1, 24, 8)])
+def test_ragged_sizes_cost_sgm_filter(dev, W, H, D):
+    """Images narrower than one cost tile or shorter than one row block, widths around the
+    62-column tile, 256 planes: cost, 8-path SGM (sequential and concurrent paths, compact
+    storage) and the filter against the FP64 oracle on the same inputs."""
+    from probabilistic_mapping_amd import synth
+    sc = synth.trajectory_scene(W * 100 + H, width=W, height=H, n_frames=3, n_sources=2,
+                                planes=D, fx=max(0.6 * W, 4.0))
+    K = sc["scene"].K
+    imgs = sc["images"]
+    homs = [np_homography(K, *sc["poses"][0], *sc["poses"][j]) for j in sc["sources"][0]]
+    ds = (sc["idep_max"] - sc["idep_min"]) / (D - 1)
+    ref = torch.from_numpy(imgs[0]).cuda()
+    quads = [dev.make_quad(torch.from_numpy(imgs[j]).cuda()) for j in sc["sources"][0]]
+    g = dev.calc_cost(ref, quads, homs, sc["idep_min"], ds, D).cpu().numpy()
+    o = oracle.calc_cost(imgs[0], [imgs[j] for j in sc["sources"][0]], homs, sc["idep_min"],
+                         ds, D, "f64")
+    st = parity.cost_parity(g, o)
+    _report(f"ragged cost {W}x{H}x{D}", st)
+    assert st["validity_mismatch"] * g.size <= max(2, 1e-4 * g.size)
+    assert st["rel"]["max"] <= 1e-3
+    knobs = sc["knobs"]
+    c32 = o.astype(np.float32)
+    oa = oracle.sgm(imgs[0], c32.astype(np.float64), knobs, 8, "f64")
+    od = oracle.filter_cost(oa.astype(np.float32).astype(np.float64), knobs["var_scale"],
+                            sc["idep_min"], ds, "f64")
+    gc = torch.from_numpy(c32).cuda()
+    ga = dev.sgm(ref, gc, knobs, 8).cpu().numpy()
+    sg = parity.summary(parity.rel_err(ga, oa, floor=1.0))
+    _report(f"ragged sgm {W}x{H}x{D}", sg)
+    assert sg["max"] <= 1e-4
+    # the concurrent-path pass sums the paths in the same order: bit-equal to sequential
+    agg_c, dep_c = dev.sgm_filter_paths(ref, gc, knobs, 8, sc["idep_min"], ds)
+    agg_s, dep_s = dev.sgm_filter(ref, gc, knobs, 8, sc["idep_min"], ds)
+    assert torch.equal(agg_c, agg_s) and torch.equal(dep_c, dep_s)
+    gd = dev.filter_cost(torch.from_numpy(oa.astype(np.float32)).cuda(), knobs["var_scale"],
+                         sc["idep_min"], ds).cpu().numpy()
+    fd = parity.depth_parity(gd, od)
+    _report(f"ragged filter {W}x{H}x{D}", fd)
+    assert fd["category_mismatch"] * gd.size <= max(2, 1e-4 * gd.size)
+    # compact storage: the aggregate within its level bound of the exact one
+    agg_x, _ = dev.sgm_filter_excess(ref, gc, knobs, 8, sc["idep_min"], ds)
+    ex = agg_s.cpu().numpy().astype(np.float64)
+    cx = agg_x.cpu().numpy().astype(np.float64)
+    ulp = np.spacing(np.abs(ex).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(cx - ex) <= 8 * _excess_step(knobs) / 2 + 4 * 8 * ulp)
+
+
 def test_sgm_zero_penalties_is_path_count_times_cost(dev, small_scene):
     """P1 = P2 = 0 makes every path's L equal the cost (L = C + min(L) - min(L))."""
     s = small_scene
