@@ -93,6 +93,56 @@ def test_depth_fuse_validation():
     assert lib.pm_depth_fuse(8, 6, cam, bad, t, FAKE, FAKE, NULL, st, st2, NULL) == 1
 
 
+def test_cpu_rules_fusion_validation():
+    """pm_depth_fuse_cpu_rules (outputFusionCPU CASE >= 3) refuses bad arguments before any
+    launch."""
+    lib = C.load()
+    n = ctypes.c_uint64(0)
+    assert lib.pm_depth_fuse_cpu_rules_workspace_size(8, 6, ctypes.byref(n)) == 0
+    assert n.value >= 8 * 6 * 36
+    assert lib.pm_depth_fuse_cpu_rules_workspace_size(0, 6, ctypes.byref(n)) == 1
+    assert lib.pm_depth_fuse_cpu_rules_workspace_size(65535, 65535, ctypes.byref(n)) == 1
+    r = (ctypes.c_double * 9)(1, 0, 0, 0, 1, 0, 0, 0, 1)
+    t = (ctypes.c_double * 3)(0, 0, 0)
+    cam = C.Camera(100, 100, 3, 2)
+    a = C.FusionState(0x1000, 0x2000, 0x3000, 0x4000)
+    b = C.FusionState(0x5000, 0x6000, 0x7000, 0x8000)
+    ws = 8 * 6 * 64
+
+    def call(w=8, h=6, cam=cam, r=r, t=t, depth=FAKE, work=FAKE, nbytes=ws, pre=a, cur=b):
+        return lib.pm_depth_fuse_cpu_rules(w, h, cam, r, t, depth, NULL, work, nbytes, pre,
+                                           cur, NULL)
+    assert call(pre=a, cur=a) == 1                                   # pre and cur alias
+    assert call(depth=NULL) == 1
+    assert call(work=NULL) == 1
+    assert call(nbytes=16) == 1                                      # workspace too small
+    assert call(cam=C.Camera(100, 0, 3, 2)) == 1
+    assert call(r=(ctypes.c_double * 9)(1, 0, 0, 0, float("nan"), 0, 0, 0, 1)) == 1
+    assert call(t=(ctypes.c_double * 3)(0, float("inf"), 0)) == 1
+    assert call(w=0) == 1
+    assert C.last_error() != ""
+
+
+def test_frame_ring_and_mapper_validation():
+    """The frame ring and the mapper's frame-id entry points refuse bad arguments before
+    touching the device."""
+    lib = C.load()
+    h = ctypes.c_void_p()
+    assert lib.pm_frame_ring_create(0, 6, 4, ctypes.byref(h)) == 1 and not h.value
+    assert lib.pm_frame_ring_create(8, 6, 0, ctypes.byref(h)) == 1 and not h.value
+    assert lib.pm_frame_ring_create(8, 6, 5000, ctypes.byref(h)) == 1 and not h.value
+    assert lib.pm_frame_ring_put(NULL, 0, FAKE, NULL) == 1
+    assert lib.pm_frame_ring_frame(NULL, 0, None, None) == 1
+    n = ctypes.c_int64(0)
+    assert lib.pm_frame_ring_puts(NULL, ctypes.byref(n)) == 1
+    R = (ctypes.c_double * 9)(1, 0, 0, 0, 1, 0, 0, 0, 1)
+    T = (ctypes.c_double * 3)(0, 0, 0)
+    assert lib.pm_mapper_init_reference_frame(NULL, NULL, 0) == 1
+    assert lib.pm_mapper_update_frame(NULL, NULL, 0, R, T, R, T) == 1
+    assert lib.pm_mapper_output_fusion_cpu(NULL, FAKE, R, T, 1.0, 1.0, 0.0, 0.0) == 1
+    assert C.last_error() != ""
+
+
 def test_homography_against_explicit_projection():
     """h p + idep t must be proportional to the projection of the back-projected point
     (stereo_mapper.cpp:79-80 with camera->world poses)."""
