@@ -15,7 +15,10 @@
 //     result is the node's 32FC1 depth message.
 // Differences: a step whose source pose would lie past the end of the pose list (the
 // reference dereferences past the end there) is not run and counts as skipped; timestamps
-// are integer nanoseconds (ros::Time compares seconds and nanoseconds exactly).
+// are integer nanoseconds (ros::Time compares seconds and nanoseconds exactly).  The lists
+// grow by about one message per step, as the reference's do, but an image more than
+// agg_num * interval places from the front can never be a reference or a source again (its
+// position only grows), so the driver frees its pixels and keeps only its list entry.
 #ifndef PM_FRAME_DRIVER_H
 #define PM_FRAME_DRIVER_H
 
@@ -70,6 +73,10 @@ class FrameDriver {
   bool process(DriverJob& job, Mat* depth);
 
   int skipped() const { return skipped_; }   // steps refused for lack of source poses
+  // list lengths, and how many buffered images still hold their pixels
+  int buffered_images() const { return (int)images_.size(); }
+  int buffered_poses() const { return (int)poses_.size(); }
+  int images_with_pixels() const;
   StereoMapper* mapper() { return mapper_.get(); }
   const FrameRing* ring() const { return ring_.get(); }
 

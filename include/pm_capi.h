@@ -379,6 +379,11 @@ pm_status pm_driver_process(pm_driver* d, int32_t* ran, int64_t* stamp_ns, int32
 /* Steps not run because the sources' poses were not buffered yet (the reference reads past
  * the end of its pose list there). */
 pm_status pm_driver_skipped(const pm_driver* d, int32_t* skipped);
+/* Buffered message counts (the lists grow by about one message per step, as the
+ * reference's do) and how many buffered images still hold their pixels (at most
+ * agg_num * interval + 1: older ones can never be read again).  Each pointer may be NULL. */
+pm_status pm_driver_buffered(const pm_driver* d, int32_t* images, int32_t* poses,
+                             int32_t* images_with_pixels);
 /* Frames uploaded through the driver's frame ring so far (0 without a ring). */
 pm_status pm_driver_ring_puts(const pm_driver* d, int64_t* puts);
 /* Eigen's Quaterniond(w, x, y, z).toRotationMatrix(), q = (x, y, z, w); R row-major. */

@@ -186,6 +186,8 @@ SIGNATURES = {
                                  ctypes.c_size_t]),
     "pm_driver_skipped": (_i32, [_vp, ctypes.POINTER(_i32)]),
     "pm_driver_ring_puts": (_i32, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "pm_driver_buffered": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                  ctypes.POINTER(_i32)]),
     "pm_quaternion_to_rotation": (_i32, [_dp, _dp]),
 }
 

@@ -54,6 +54,15 @@ void FrameDriver::grab_image(int64_t stamp_ns, const Mat& image) {
   if (image.type != CV_8U || image.empty())
     throw Error(PM_ERR_INVALID_ARGUMENT, "FrameDriver: images are 8-bit grayscale");
   images_.push_front(Image{stamp_ns, next_image_++, image.clone()});
+  // positions only grow, so the image that just moved past the window is never read again
+  const size_t window = (size_t)cfg_.agg_num * (size_t)cfg_.interval;
+  if (images_.size() > window + 1) std::next(images_.begin(), (long)window + 1)->image = Mat();
+}
+
+int FrameDriver::images_with_pixels() const {
+  int n = 0;
+  for (const Image& im : images_) n += im.image.empty() ? 0 : 1;
+  return n;
 }
 
 void FrameDriver::grab_pose(int64_t stamp_ns, double tx, double ty, double tz, double qx,
@@ -256,6 +265,16 @@ extern "C" pm_status pm_driver_skipped(const pm_driver* d, int32_t* skipped) {
   if (!d || !d->impl || !skipped)
     return pm::fail(PM_ERR_INVALID_ARGUMENT, "pm_driver_skipped: bad argument");
   *skipped = d->impl->skipped();
+  return PM_OK;
+}
+
+extern "C" pm_status pm_driver_buffered(const pm_driver* d, int32_t* images, int32_t* poses,
+                                        int32_t* images_with_pixels) {
+  if (!d || !d->impl)
+    return pm::fail(PM_ERR_INVALID_ARGUMENT, "pm_driver_buffered: bad argument");
+  if (images) *images = d->impl->buffered_images();
+  if (poses) *poses = d->impl->buffered_poses();
+  if (images_with_pixels) *images_with_pixels = d->impl->images_with_pixels();
   return PM_OK;
 }
 

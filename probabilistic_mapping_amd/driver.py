@@ -79,6 +79,13 @@ class FrameDriver:
             return None
         return stamp.value, ref.value, list(srcs)[:self.agg_num], depth
 
+    def buffered(self) -> tuple[int, int, int]:
+        """(images, poses, images still holding pixels) in the message lists."""
+        a, b, c = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        C.check(self.lib.pm_driver_buffered(self._h, ctypes.byref(a), ctypes.byref(b),
+                                            ctypes.byref(c)))
+        return a.value, b.value, c.value
+
     @property
     def ring_puts(self) -> int:
         """Frames uploaded through the frame ring so far (0 without one)."""

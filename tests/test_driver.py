@@ -113,6 +113,26 @@ def test_plan_matches_oracle(drv, name, agg, interval):
     assert len(got) > 0
 
 
+@pytest.mark.parametrize("agg,interval", [(3, 6), (2, 3), (1, 1), (0, 4)])
+def test_driver_frees_pixels_outside_the_window(drv, agg, interval):
+    """The message lists grow as the reference's do (one more than they shrink per step), but
+    only the newest agg_num * interval + 1 images keep their pixels; the plan is unchanged."""
+    lib = _Lib(drv, agg, interval)
+    o = DriverOracle(agg, interval)
+    stream = _scenarios()["aligned"] * 1
+    got, want = [], []
+    for kind, t in stream:
+        for plan, out in ((lib, got), (o, want)):
+            (plan.grab_pose if kind == "p" else plan.grab_image)(t)
+            r = plan.process()
+            if r is not None:
+                out.append(r if plan is lib else (r[0], r[1], r[3]))
+        n_img, n_pose, with_px = lib.d.buffered()
+        assert n_img == len(o.images) and n_pose == len(o.poses)
+        assert with_px == min(n_img, agg * interval + 1)
+    assert got == want and len(got) > 0
+
+
 def test_plan_skips_without_source_poses(drv):
     """Images buffered, but the sources' poses not yet: no step (the reference would read past
     its pose list), counted as skipped, and the oracle agrees."""
