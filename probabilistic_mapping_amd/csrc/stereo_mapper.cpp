@@ -99,6 +99,10 @@ StereoMapper::StereoMapper(int width, int height, int dep_cnt)
     throw Error(PM_ERR_INVALID_ARGUMENT, "StereoMapper: bad image size");
   if (D_ < 1 || D_ > PM_MAX_PLANES)
     throw Error(PM_ERR_INVALID_ARGUMENT, "StereoMapper: dep_cnt must be in 1..256");
+  // the stages address a volume through 32-bit buffer offsets (pm_sgm and friends)
+  if ((uint64_t)W_ * (uint64_t)H_ * (uint64_t)D_ * 4u >= (1ull << 32))
+    throw Error(PM_ERR_INVALID_ARGUMENT,
+                "StereoMapper: width * height * dep_cnt * 4 bytes must stay below 4 GiB");
   hipStream_t st = nullptr;
   hip_check(hipStreamCreate(&st), "hipStreamCreate");
   stream_ = st;

@@ -123,6 +123,17 @@ def test_cpu_rules_fusion_validation():
     assert C.last_error() != ""
 
 
+def test_mapper_create_rejects_oversized_volumes():
+    """A mapper whose cost volume would pass 4 GiB is refused before any device call."""
+    lib = C.load()
+    h = ctypes.c_void_p()
+    assert lib.pm_mapper_create(65535, 65535, 256, ctypes.byref(h)) == 1 and not h.value
+    assert "4 GiB" in C.last_error()
+    assert lib.pm_mapper_create(8192, 8192, 16, ctypes.byref(h)) == 1 and not h.value
+    assert lib.pm_mapper_create(640, 480, 0, ctypes.byref(h)) == 1 and not h.value
+    assert lib.pm_mapper_create(0, 480, 64, ctypes.byref(h)) == 1 and not h.value
+
+
 def test_frame_ring_and_mapper_validation():
     """The frame ring and the mapper's frame-id entry points refuse bad arguments before
     touching the device."""

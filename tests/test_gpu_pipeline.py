@@ -128,6 +128,37 @@ def test_case_modes(small_scene):
     assert np.array_equal(res[2], dep)
 
 
+def test_mapper_call_order_errors():
+    """update before initReference or initIntrinsic, output before initReference and a
+    wrong-size image raise PM_ERR_INVALID_STATE / PM_ERR_INVALID_ARGUMENT instead of exiting
+    (the reference would dereference uninitialised members); the mapper keeps working."""
+    from probabilistic_mapping_amd import _capi as C
+    from probabilistic_mapping_amd.mapper import StereoMapper
+    rng = np.random.default_rng(4)
+    img = rng.integers(0, 256, (48, 64), dtype=np.uint8)
+    I3, z = np.eye(3), np.zeros(3)
+    m = StereoMapper(64, 48, 16)
+    with pytest.raises(C.PmError) as e:
+        m.output()
+    assert e.value.status == C.PM_ERR_INVALID_STATE
+    with pytest.raises(C.PmError) as e:
+        m.update(img, I3, z, I3, z)
+    assert e.value.status == C.PM_ERR_INVALID_STATE
+    m.initReference(img)
+    with pytest.raises(C.PmError) as e:
+        m.update(img, I3, z, I3, z)                      # initIntrinsic not called
+    assert e.value.status == C.PM_ERR_INVALID_STATE
+    with pytest.raises(C.PmError) as e:
+        m.initReference(np.zeros((48, 65), np.uint8))
+    assert e.value.status == C.PM_ERR_INVALID_ARGUMENT
+    K = np.array([[60.0, 0, 32], [0, 60.0, 24], [0, 0, 1]])
+    m.initIntrinsic(K, 0.0, 1.0 / 15)
+    m.update(img, I3, z, I3, z + np.array([0.05, 0, 0]))
+    d = m.output()
+    assert d.shape == (48, 64) and np.isfinite(d).all()
+    m.close()
+
+
 def test_case3_cpu_rules_sequence():
     """outputFusionCPU CASE 3 over 6 keyframes (stereo_mapper.cpp:719-800): the first call
     resets and returns zeros, each later one fuses the mapper's own filtered depth with the
