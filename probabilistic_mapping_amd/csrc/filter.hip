@@ -90,9 +90,13 @@ __global__ __launch_bounds__(256) void combine_filter_kernel(long N, int D, int 
   auto load = [&](long pc, bool own, int d0, float (&v)[4]) {
     const long e = pc * D + d0;
     if (vec4 && d0 + 3 < D) {
-      float4 t = *reinterpret_cast<const float4*>(lbuf + e);
+      // the path volumes are read exactly once: non-temporal loads
+      typedef float vf4 __attribute__((ext_vector_type(4)));
+      const vf4 t0 = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(lbuf + e));
+      float4 t = make_float4(t0.x, t0.y, t0.z, t0.w);
       for (int r = 1; r < paths; ++r) {
-        const float4 u = *reinterpret_cast<const float4*>(lbuf + (long)r * V + e);
+        const vf4 u =
+            __builtin_nontemporal_load(reinterpret_cast<const vf4*>(lbuf + (long)r * V + e));
         t.x += u.x; t.y += u.y; t.z += u.z; t.w += u.w;
       }
       if (own && agg) *reinterpret_cast<float4*>(agg + e) = t;
@@ -173,8 +177,11 @@ __device__ __forceinline__ bool excess_agg4(const CT* __restrict__ cost,
     float c[4];
     load_cost4<CT>(cost + e, c);
     uint32_t s[4] = {0u, 0u, 0u, 0u};
+    // the excess volumes are read exactly once: non-temporal loads
+    typedef uint32_t vu2 __attribute__((ext_vector_type(2)));
     for (int r = 0; r < paths; ++r) {
-      const uint2 u = *reinterpret_cast<const uint2*>(ebuf + (long)r * V + e);
+      const vu2 u =
+          __builtin_nontemporal_load(reinterpret_cast<const vu2*>(ebuf + (long)r * V + e));
       const uint32_t q[4] = {u.x & 0xFFFFu, u.x >> 16, u.y & 0xFFFFu, u.y >> 16};
 #pragma unroll
       for (int j = 0; j < 4; ++j) s[j] += q[j];
