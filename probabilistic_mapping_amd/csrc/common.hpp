@@ -177,8 +177,15 @@ struct Buf {
   __device__ __forceinline__ void put_f32(float v, uint32_t voff, uint32_t soff) const {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
   }
+  // non-temporal store (cache policy "nt"), for data that is read once, much later
+  __device__ __forceinline__ void put_f32_nt(float v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 2);
+  }
   __device__ __forceinline__ void put_u16(uint32_t v, uint32_t voff, uint32_t soff) const {
     __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)voff, (int)soff, 0);
+  }
+  __device__ __forceinline__ void put_u16_nt(uint32_t v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)voff, (int)soff, 2);
   }
 };
 // A cost element through a Buf (byte offsets).

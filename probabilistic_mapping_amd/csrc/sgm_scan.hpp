@@ -70,7 +70,8 @@ __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, 
 // into agg (else agg is untouched).  LOUT: also write this path's L to a.lout; with LMASK the
 // stored value is 0 at invalid pixels (the path's contribution to the aggregate).  EQ: write
 // the path's excess E = L - C' to a.eout as 16-bit fixed point; L itself is computed exactly as
-// in the other forms.
+// in the other forms.  The per-path volumes (LOUT, EQ) are written once and read once by the
+// combine pass, so they go out with the non-temporal cache policy; the aggregate does not.
 template <int NV, typename CT, bool FIRST, bool FULL, int B, typename RT = uint8_t,
           bool AGG = true, bool LOUT = false, bool LMASK = false, bool EQ = false>
 __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
@@ -177,12 +178,13 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       Ln[j] = act[j] ? CS::add(c[i][j], val, w) - m : INF;   // sgm.cu:98
       const float out = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);
       if (AGG && act[j]) bagg.put_f32(out, lb(j, 4), p * 4u);   // sgm.cu:101-108
-      if (LOUT && act[j]) blout.put_f32((LMASK && invalid) ? kCostInf : Ln[j], lb(j, 4), p * 4u);
+      if (LOUT && act[j])
+        blout.put_f32_nt((LMASK && invalid) ? kCostInf : Ln[j], lb(j, 4), p * 4u);
       if (EQ && act[j]) {
         // val >= m exactly (every candidate is >= m), so E >= 0; rounding to nearest.  At an
         // invalid pixel the level is ignored (the combine finds the pixel from its cost).
         const uint32_t q = min(kExcessMaxQ, (uint32_t)fmaf(val - m, a.eq_inv, 0.5f));
-        beout.put_u16(q, lb(j, 2), p * 2u);
+        beout.put_u16_nt(q, lb(j, 2), p * 2u);
       }
     }
 #pragma unroll
@@ -223,8 +225,9 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       L[j] = act[j] ? (invalid ? kCostInf : c[j]) : INF;
       const float out = invalid ? kCostInf : (FIRST ? L[j] : g[j] + L[j]);
       if (AGG && act[j]) bagg.put_f32(out, lb(j, 4), p * 4u);
-      if (LOUT && act[j]) blout.put_f32((LMASK && invalid) ? kCostInf : L[j], lb(j, 4), p * 4u);
-      if (EQ && act[j]) beout.put_u16(0u, lb(j, 2), p * 2u);
+      if (LOUT && act[j])
+        blout.put_f32_nt((LMASK && invalid) ? kCostInf : L[j], lb(j, 4), p * 4u);
+      if (EQ && act[j]) beout.put_u16_nt(0u, lb(j, 2), p * 2u);
     }
   }
   int base = 1;
