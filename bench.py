@@ -44,7 +44,7 @@ COST_BYTES = {"f32": 4, "f16": 2, "bf16": 2}
 # HBM bytes per keyframe of the SGM pass (paths launch + combine/filter launch) on this
 # workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales measured on a 1 GiB device
 # copy), profiles/sgm_pass_752x480x64_s5_p8.md.  Keyed by (form, cost storage, SGM storage).
-SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2215e9,
+SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2038e9,
                     ("batched", "f16", "compact"): 1.1560e9}
 
 
