@@ -247,6 +247,40 @@ def test_batch_max_sources_tile_rows(uneven):
     assert best["cost_tile_rows"] != 2
 
 
+def test_batched_sequence_groups_match_one_batch(trajectory_scene):
+    """Keyframe groups (several batches, each captured and replayed) give the one-batch
+    depths and fused state bit for bit, through graph replay; the cache-sized default groups
+    the 752x480x64 FP16 sequence by five."""
+    from probabilistic_mapping_amd.pipeline import BatchedSequence, BatchGroups
+    s = trajectory_scene
+    n = 10
+    args = (s["images"][:n], s["poses"][:n], [[j for j in src if j < n] or [1]
+                                               for src in s["sources"][:n]],
+            s["scene"].K, s["planes"], s["idep_min"], s["idep_max"], s["knobs"], s["paths"])
+    one = BatchedSequence(*args, cost_type="f16", sgm_storage="compact", group=0)
+    grp = BatchedSequence(*args, cost_type="f16", sgm_storage="compact", group=3)
+    auto = BatchedSequence(*args, cost_type="f16", sgm_storage="compact")
+    assert isinstance(grp.batch, BatchGroups) and len(grp.batch.batches) == 4
+    assert [len(b.outputs) for b in auto.batch.batches] == [5, 5]
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        for q in (one, grp, auto):
+            q.batch.capture()
+        for rep in range(2):
+            for q in (one, grp, auto):
+                q.reset()
+                q.batch.launch()
+                for i in range(n):
+                    q.fusion_step(i)
+            torch.cuda.synchronize()
+            for q in (grp, auto):
+                for i in range(n):
+                    assert torch.equal(q.depth_of(i), one.depth_of(i)), (rep, i)
+                assert torch.equal(q.fused, one.fused)
+                for k in one.state:
+                    assert torch.equal(q.state[k], one.state[k]), (rep, k)
+
+
 @pytest.mark.parametrize("storage", ["exact", "compact"])
 def test_batched_sequence_with_median_matches_plain(trajectory_scene, storage):
     """The batched sequence's median -> fusion steps give the plain sequence's fused depth and
