@@ -145,7 +145,9 @@ __device__ __forceinline__ void load_cost4<float>(const float* p, float (&v)[4])
 }
 template <>
 __device__ __forceinline__ void load_cost4<__half>(const __half* p, float (&v)[4]) {
-  const uint2 t = *reinterpret_cast<const uint2*>(p);
+  // the combine is the cost volume's last reader: non-temporal
+  typedef uint32_t vu2 __attribute__((ext_vector_type(2)));
+  const vu2 t = __builtin_nontemporal_load(reinterpret_cast<const vu2*>(p));
   v[0] = __half2float(__ushort_as_half((unsigned short)(t.x & 0xFFFFu)));
   v[1] = __half2float(__ushort_as_half((unsigned short)(t.x >> 16)));
   v[2] = __half2float(__ushort_as_half((unsigned short)(t.y & 0xFFFFu)));
