@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--cost-type", default="f32", choices=["f32", "f16", "bf16"],
                     help="cost storage of the plain / batched form (both: of the plain form)")
     ap.add_argument("--form", default="both",
-                    choices=["both", "plain", "batched", "diff", "driver"])
+                    choices=["both", "plain", "batched", "uneven", "mapper", "diff", "driver"])
     ap.add_argument("--sgm-storage", default=None, choices=["exact", "compact", "sweep"],
                     help="SGM per-path storage (default: exact for f32 cost, compact otherwise; "
                          "sweep exists in the batched form only)")
@@ -84,30 +84,14 @@ def default_storage(cost_type: str, storage: str | None) -> str:
     return storage or ("exact" if cost_type == "f32" else "compact")
 
 
-def cpu_baseline(scene, seconds: float, median: bool) -> dict | None:
-    """float/OpenMP oracle over as many keyframes of the workload as fit in ~`seconds`."""
+def cpu_baseline(seed: int, seconds: float, median: bool) -> dict | None:
+    """The float/OpenMP oracle over as many keyframes of the workload as fit in ~`seconds`,
+    built with -march=native on this host and timed in a child process with OMP_NUM_THREADS =
+    the physical cores this process may use (at most 16) and OMP_PROC_BIND=close."""
     if seconds <= 0:
         return None
-    import oracle
-    from oracle.sequence import OracleSequence
-    threads = min(16, os.cpu_count() or 1)
-    oracle.set_threads(threads)
-    s = scene
-    seq = OracleSequence(s["images"], s["poses"], s["sources"], s["scene"].K, s["planes"],
-                         s["idep_min"], s["idep_max"], s["knobs"], s["paths"],
-                         median=median, precision="f32")
-    done = 0
-    t0 = time.perf_counter()
-    while done < len(s["images"]):
-        seq.run_keyframe(done)
-        done += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "fused depth frames/s", "cores": threads,
-            "kind": "port",
-            "sample": f"first {done} of 20 keyframes of the same 752x480x64, 5-source, "
-                      f"8-path workload (float instantiation of the oracle, OpenMP)"}
+    from oracle import cpu_baseline as cb
+    return cb.run(seconds, seed, median, max_threads=16)
 
 
 def attainable_copy(torch, mib: int) -> dict | None:
@@ -223,7 +207,10 @@ def measure_batched(args, torch, s, cost_type, storage):
             seq.batch.tuning = json.loads(args.tuning)
             tuning = seq.batch.tuning
         else:
-            tuning = seq.batch.autotune(3)
+            seq.batch.autotune(3)
+            tuning = seq.batch.tuning
+        # every group's own launch shapes (groups are tuned separately)
+        tunings = getattr(seq.batch, "tunings", [tuning])
         seq.batch.capture()
         torch.cuda.synchronize()
 
@@ -264,7 +251,7 @@ def measure_batched(args, torch, s, cost_type, storage):
             torch.cuda.synchronize()
             acc[3] += e0.elapsed_time(e1)
     ms = acc / reps / nkf
-    inflight = measure_inflight(args, torch, s, seq, cost_type, storage, tuning) \
+    inflight = measure_inflight(args, torch, s, seq, cost_type, storage, tunings) \
         if args.inflight > 1 else None
     # the SGM pass is stage 1 (paths) + stage 2 (filter; compact: combine + filter)
     stages = stage_report(seq, s, cost_type, ms[0], ms[1] + ms[2], ms[3], storage)
@@ -280,7 +267,7 @@ def measure_batched(args, torch, s, cost_type, storage):
             "inflight": inflight}
 
 
-def measure_inflight(args, torch, s, first, cost_type, storage, tuning):
+def measure_inflight(args, torch, s, first, cost_type, storage, tunings):
     """Several independent 20-keyframe sequences in flight: sequence j of the timed steps runs
     on stream j % n with its own batch buffers, captured graph and fusion state, so one
     sequence's fusion steps (small, latency-bound launches) overlap the next one's batch.
@@ -296,7 +283,8 @@ def measure_inflight(args, torch, s, first, cost_type, storage, tuning):
                                    s["planes"], s["idep_min"], s["idep_max"], s["knobs"],
                                    s["paths"], cost_type=cost_type, median=bool(args.median),
                                    sgm_storage=storage, write_agg=False)
-            q.batch.tuning = tuning
+            for b, t in zip(getattr(q.batch, "batches", [q.batch]), tunings):
+                b.tuning = t
             q.batch.capture()
         seqs.append(q)
         streams.append(st)
@@ -506,6 +494,194 @@ def bench_driver(args, torch):
     print(json.dumps(out), flush=True)
 
 
+def sgm_pass_design_bytes(W: int, H: int, D: int, P: int, bc: int, storage: str,
+                          write_agg: bool = False) -> int:
+    """Bytes the SGM pass (paths launch + combine/filter launch) streams when nothing is
+    re-read from a cache: each path reads the cost volume and the reference row and writes its
+    own volume (FP32 L for the exact storage, 16-bit excess for the compact one); the combine
+    reads the per-path volumes (compact: and the cost once), writes the depth and, if asked,
+    the FP32 aggregate."""
+    N = W * H
+    V = N * D
+    vol = 4 if storage == "exact" else 2
+    paths = P * (V * bc + N + V * vol)
+    combine = P * V * vol + (V * bc if storage == "compact" else 0) + N * 4
+    return paths + combine + (V * 4 if write_agg else 0)
+
+
+def bench_uneven(args, torch):
+    """16 independent keyframes of uneven resolution, source count and plane count
+    (synth.uneven_keyframes) as one FP16-cost, compact-SGM batch: autotuned, captured into one
+    hipGraph and replayed.  value = keyframes/s of the replay (cost over every source, 8-path
+    SGM, filter; independent keyframes, so no fusion).  Beside it: per-stage times of the
+    batch, and the same keyframes run one at a time through the single-keyframe entry points
+    (FP16 compact, and FP32 exact as the plain form runs them)."""
+    from probabilistic_mapping_amd import pipeline as pl, synth
+    host = synth.uneven_keyframes(synth.UNEVEN_SEED if args.seed == 0 else args.seed)
+    knobs, P = host[0]["knobs"], host[0]["paths"]
+    specs = [pl.keyframe_spec(k) for k in host]
+    nkf = len(specs)
+    shapes = [(k["width"], k["height"], k["planes"], k["num_sources"]) for k in host]
+    side = torch.cuda.Stream()
+    batch = pl.Batch(specs, "f16", knobs, P, "compact", write_agg=False)
+    with torch.cuda.stream(side):
+        if args.tuning:
+            batch.tuning = json.loads(args.tuning)
+        else:
+            batch.autotune(3)
+        tuning = batch.tuning
+        batch.capture()
+        for _ in range(args.warmup):
+            batch.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.launch()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        reps = max(1, min(args.steps, 5))
+        acc = np.zeros(3)
+        for _ in range(reps):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            ev[0].record(side)
+            for st in range(3):
+                batch.run_stage(st)
+                ev[st + 1].record(side)
+            torch.cuda.synchronize()
+            acc += [ev[i].elapsed_time(ev[i + 1]) for i in range(3)]
+        ms = acc / reps
+    batch.close()
+    torch.cuda.empty_cache()
+    design = sum(sgm_pass_design_bytes(W, H, D, P, 2, "compact") for W, H, D, _ in shapes)
+    sgm_ms = float(ms[1] + ms[2])
+
+    # one keyframe at a time through the single-keyframe entry points
+    def one_at_a_time(cost_type, storage):
+        bufs = []
+        for kf in specs:
+            H, W = kf["ref"].shape
+            D = kf["planes"]
+            cost = torch.empty((H, W, D), dtype=pl.COST_TYPES[cost_type][1], device="cuda")
+            ws = (pl.sgm_excess_workspace if storage == "compact" else pl.sgm_workspace)(
+                H, W, D, P)
+            bufs.append((cost, ws, torch.empty((H, W), dtype=torch.float32, device="cuda")))
+
+        def run():
+            for kf, (cost, ws, dep) in zip(specs, bufs):
+                pl.calc_cost(kf["ref"], kf["quads"], kf["homs"], kf["idep_min"],
+                             kf["dep_sample"], kf["planes"], cost_type, out=cost)
+                fn = pl.sgm_filter_excess if storage == "compact" else pl.sgm_filter_paths
+                fn(kf["ref"], cost, knobs, P, kf["idep_min"], kf["dep_sample"], workspace=ws,
+                   depth=dep, write_agg=False)
+        for _ in range(max(args.warmup, 1)):
+            run()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        del bufs
+        torch.cuda.empty_cache()
+        return {"value": args.steps * nkf / dt, "ms_per_step": dt / args.steps * 1e3}
+
+    seq_f16 = one_at_a_time("f16", "compact")
+    seq_f32 = one_at_a_time("f32", "exact")
+    V = sum(W * H * D for W, H, D, _ in shapes)
+    out = {
+        "metric": "uneven keyframe batch: depth keyframes/s (16 keyframes, resolutions "
+                  "320x240-1280x720, 2-8 sources, 32-128 planes, 8-path SGM)",
+        "value": args.steps * nkf / elapsed, "unit": "keyframes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16-storage/f32", "data": "synthetic (16 seeded scenes and trajectories)",
+        "config": {"workload": "16 independent keyframes, one hipGraph replay per step",
+                   "form": "uneven batch (FP16 cost, compact SGM storage)",
+                   "keyframes": [list(x) for x in shapes], "cost_volume_elements": V,
+                   "tuning": tuning},
+        "stages": {"cost_ms": float(ms[0]), "sgm_paths_ms": float(ms[1]),
+                   "combine_filter_ms": float(ms[2])},
+        "roofline": {"bound": "hbm", "kernel": "sgm pass (compact paths launch + "
+                                                "combine/filter launch), whole batch",
+                     "achieved": design / (sgm_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": design / (sgm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "algorithmic_bytes": design, "traffic": None},
+        "one_at_a_time": {"f16_compact": seq_f16, "f32_exact": seq_f32},
+        "batch_vs_one_at_a_time_f16": (args.steps * nkf / elapsed) / seq_f16["value"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_mapper(args, torch):
+    """The drop-in path timed from C++: the 20-keyframe workload through pm::StereoMapper
+    (initReference, update per source, outputFusion returning the host depth), run by the
+    pm_bench_mapper executable in a child process.  Two forms: host images at every use
+    (pinned staging, as the reference uploads them) and the upload-once frame ring.  The last
+    step's fused and filtered depth must equal the device-resident sequence's bit for bit."""
+    import struct
+    import subprocess
+    import tempfile
+    from probabilistic_mapping_amd import build as pm_build, pipeline as pl, synth
+    s = synth.trajectory_scene(args.seed)
+    H, W = s["images"][0].shape
+    D, P, S = s["planes"], s["paths"], len(s["sources"][0])
+    F = len(s["images"])
+    kn = s["knobs"]
+    exe = pm_build.BENCH_MAPPER
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} is missing: run __graft_entry__.build()")
+    seq = pl.KeyframeSequence(s["images"], s["poses"], s["sources"], s["scene"].K, D,
+                              s["idep_min"], s["idep_max"], kn, P, cost_type="f32",
+                              median=bool(args.median))
+    seq.run()
+    torch.cuda.synchronize()
+    want_fused, want_dep = seq.fused.cpu().numpy(), seq.dep.cpu().numpy()
+    del seq
+    torch.cuda.empty_cache()
+    forms = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, ring in (("host_images", 0), ("frame_ring", 1)):
+            path = os.path.join(tmp, f"workload_{ring}.bin")
+            with open(path, "wb") as f:
+                f.write(struct.pack("<10i", W, H, D, F, S, args.steps, args.warmup,
+                                    int(bool(args.median)), P, ring))
+                f.write(np.asarray(s["scene"].K, "<f8").tobytes())
+                f.write(np.array([s["idep_min"], s["idep_max"], kn["pi1"], kn["pi2"],
+                                  kn["tau_so"], kn["sgm_q1"], kn["sgm_q2"], kn["var_scale"]],
+                                 "<f8").tobytes())
+                for R, T in s["poses"]:
+                    f.write(np.concatenate([np.asarray(R, "<f8").reshape(9),
+                                            np.asarray(T, "<f8").reshape(3)]).tobytes())
+                f.write(np.asarray(s["sources"], "<i4").tobytes())
+                for im in s["images"]:
+                    f.write(np.ascontiguousarray(im, np.uint8).tobytes())
+            out = subprocess.run([exe, path, tmp], capture_output=True, text=True, check=True)
+            r = json.loads(out.stdout.strip().splitlines()[-1])
+            fused = np.fromfile(os.path.join(tmp, "fused.f32"), np.float32).reshape(H, W)
+            dep = np.fromfile(os.path.join(tmp, "dep.f32"), np.float32).reshape(H, W)
+            ms = np.asarray(r["step_ms"])
+            forms[name] = {"value": F / (ms.mean() * 1e-3), "ms_per_step": float(ms.mean()),
+                           "ms_per_step_median": float(np.median(ms)),
+                           "frame_uploads_per_step": r["ring_puts"] if ring else F * (S + 1),
+                           "same_result_as_sequence": bool(np.array_equal(fused, want_fused)
+                                                           and np.array_equal(dep, want_dep))}
+    hi = forms["host_images"]
+    out = {
+        "metric": "fused depth frames/s through the C++ StereoMapper (drop-in path) "
+                  "@752x480x64 planes, 5 sources, 8-path SGM",
+        "value": hi["value"], "unit": "fused depth frames/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": hi["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded textured-plane scene, 20-frame trajectory)",
+        "config": {"workload": WORKLOAD, "form": "mapper (host images in, host depth out; "
+                                                 "C++ driver, std::chrono)"},
+        "forms": forms,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -521,6 +697,10 @@ def main():
         return bench_diff(args, torch)
     if args.form == "driver":
         return bench_driver(args, torch)
+    if args.form == "uneven":
+        return bench_uneven(args, torch)
+    if args.form == "mapper":
+        return bench_mapper(args, torch)
     s = synth.trajectory_scene(args.seed)
     copy = attainable_copy(torch, args.copy_mib)
     storage = default_storage(args.cost_type, args.sgm_storage)
@@ -536,7 +716,7 @@ def main():
         if args.form == "both":
             torch.cuda.empty_cache()
             extra = measure_batched(args, torch, s, "f16", "compact")
-    base = cpu_baseline(s, args.cpu_baseline_seconds, bool(args.median))
+    base = cpu_baseline(args.seed, args.cpu_baseline_seconds, bool(args.median))
     print(json.dumps(report(args, main_r, extra, copy, base)), flush=True)
 
 

@@ -5,6 +5,9 @@
 //   * shapes are runtime (the default constructor uses WIDTH x HEIGHT x DEP_CNT);
 //   * update() stages the measurement frame; the cost of all staged frames is computed in one
 //     pass when an output method runs (the running mean of the reference is order-free);
+//   * initReference() and update() copy the host image into pinned staging and return
+//     without synchronising (the caller may reuse its buffer at once); output methods
+//     synchronise before they hand back a host result;
 //   * raw_cost may be stored as fp16 / bf16 (set_cost_type);
 //   * no GUI calls: outputFusion's imshow (stereo_mapper.cpp:232-235) is dropped;
 //   * errors raise pm::Error (a std::runtime_error with a pm_status) instead of exiting.
@@ -98,6 +101,7 @@ class StereoMapper {
                                  const Mat& T_r);
   void fuse_cpu_rules(Mat& result, Matrix3d R, Vector3d t, float fx, float fy, float cx,
                       float cy);
+  void stage_upload(const Mat& img, uint8_t* dst, size_t slot, const char* who);
 
   int W_, H_, D_;
   pm_cost_type cost_type_ = PM_COST_F32;
@@ -110,12 +114,18 @@ class StereoMapper {
   };
   std::vector<RingFrame> staged_;          // per staged source
   std::vector<pm_homography> homs_;
-  DeviceMat zbuf_, med_tmp_, wta_tmp_;
+  DeviceMat zbuf_, med_tmp_, wta_tmp_;   // zbuf_: two [H][W] z-buffers, zbuf_ half zpar_ empty
+  int zpar_ = 0;
   DeviceMat sgm_ws_;                       // per-path L volumes (pm_sgm_filter_paths)
   DeviceMat cpu_work_, cpu_result_;        // outputFusionCPU's CASE >= 3 step
   bool cost_ready_ = false;
   bool depth_ready_ = false;
   bool have_reference_ = false;
+  // pinned host staging of host images (slot 0: the reference, slot 1 + s: source s), each
+  // with the event recorded after its last copy; initReference and update copy the caller's
+  // image into a slot and return without waiting for the device
+  std::vector<uint8_t*> pinned_;
+  std::vector<void*> pinned_ev_;
 };
 
 }  // namespace pm

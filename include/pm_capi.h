@@ -175,15 +175,25 @@ pm_status pm_median3x3(const float* in, float* out, int32_t width, int32_t heigh
 
 /* One Gaussian x Beta fusion step.  Replaces depth_fuse (motion_stereo/depth_fusion.h:13-20,
  * depth_fusion.cu:8-280) and the convert calls around it (stereo_mapper.cpp:149-152,185-193).
- *   depth  : [H][W] filtered depth in, fused depth out (in place, like tmpDepthMap)
- *   zbuf   : [H][W] uint64 workspace (z-buffer; no initialisation needed)
- *   table  : optional [H][W] int32 output of the filled propagation table (NULL to skip)
- *   pre    : previous state; mu and sig of moved pixels are rewritten in place
- *   cur    : current state
+ *   depth     : [H][W] filtered depth in
+ *   fused     : [H][W] fused depth out; may be depth itself (in place, like tmpDepthMap)
+ *   zbuf      : [H][W] uint64 z-buffer
+ *   zbuf_next : NULL, or a second [H][W] uint64 buffer.  NULL: the call empties zbuf first
+ *               (no initialisation needed).  Non-NULL: zbuf must be empty (every entry all
+ *               bits set) on entry, and the call empties zbuf_next for the next step, which
+ *               passes it as its zbuf (two buffers alternating, both emptied once with
+ *               pm_zbuf_reset); this saves a launch per step.
+ *   table     : optional [H][W] int32 output of the filled propagation table (NULL to skip)
+ *   pre       : previous state; mu and sig of moved pixels are rewritten in place
+ *   cur       : current state
  *   r_p2c, t_p2c : previous->current camera transform (stereo_mapper.cpp:155-156), row-major */
 pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam, const float r_p2c[9],
-                        const float t_p2c[3], float* depth, uint64_t* zbuf, int32_t* table,
-                        pm_fusion_state pre, pm_fusion_state cur, pm_stream stream);
+                        const float t_p2c[3], const float* depth, float* fused, uint64_t* zbuf,
+                        uint64_t* zbuf_next, int32_t* table, pm_fusion_state pre,
+                        pm_fusion_state cur, pm_stream stream);
+/* Empties n z-buffer entries (sets every bit), for pm_depth_fuse's alternating buffers. */
+pm_status pm_zbuf_reset(uint64_t* zbuf, size_t n, pm_stream stream);
+
 
 /* The fusion step of outputFusionCPU with CASE >= 3 (stereo_mapper.cpp:719-800): the
  * reference's CPU rules (depthFuseCPU, :246-587) on the GPU -- propagation gate 0.5, a 7x7
@@ -228,13 +238,17 @@ typedef struct pm_keyframe {
 /* Launch shapes (picked by pm_batch_autotune): cost tile rows 2, 4 or 8 with an LDS staging
  * chunk of 8 or 16 planes (4 or 8 with 8 rows; 2 rows only when no keyframe has more than 10
  * sources), SGM prefetch depth 8 or 16 steps, and persistent grids of blocks_per_cu x
- * (compute units) blocks (0 = one block per work item). */
+ * (compute units) blocks (0 = one block per work item).  cost_plane_split 1 splits each cost
+ * tile's planes into groups of 32 (keyframes whose plane count is a multiple of 16), 0 keeps
+ * a tile whole.  The cost work list is divided over the chip's 8 XCDs by work (tiles x planes
+ * x sources), each XCD taking one contiguous range. */
 typedef struct pm_batch_tuning {
   int32_t cost_tile_rows;
   int32_t cost_stage_planes;
   int32_t sgm_prefetch;
   int32_t cost_blocks_per_cu;
   int32_t sgm_blocks_per_cu;
+  int32_t cost_plane_split;
 } pm_batch_tuning;
 
 typedef struct pm_batch pm_batch;

@@ -26,7 +26,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libpm_oracle.so")
+# PM_ORACLE_LIB selects another build of the same source (the -march=native one of
+# ``make native``, used by the CPU baseline's timing process)
+_LIB_PATH = os.environ.get("PM_ORACLE_LIB") or os.path.join(_HERE, "libpm_oracle.so")
 _lib = None
 
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -72,13 +74,17 @@ def _dtype(precision: str):
     raise ValueError(precision)
 
 
-def calc_cost(ref, srcs, homs, idep_min, dep_sample, planes, precision="f64"):
+def calc_cost(ref, srcs, homs, idep_min, dep_sample, planes, precision="f64", margin=False):
     """Fused-over-sources plane-sweep cost, pixel-major ``[H][W][D]``.
 
     ``srcs`` is a list of uint8 images, ``homs`` a list of ``(h[9], t[3])`` pairs holding the
-    float-rounded homography setup of each measurement frame.
+    float-rounded homography setup of each measurement frame.  ``margin=True`` (FP64 only)
+    also returns each entry's validity margin (float32 ``[H][W][D]``, pixels): the distance of
+    the nearest source's decisive warp to the image-bounds test, i.e. how far the warps must
+    move for the entry's set of contributing sources to change.
     """
-    ref = np.ascontiguousarray(ref, dtype=np.uint8)
+    fref = np.asarray(ref).dtype == np.float64   # a real-valued reference (FD checks)
+    ref = np.ascontiguousarray(ref, dtype=np.float64 if fref else np.uint8)
     H, W = ref.shape
     S = len(srcs)
     imgs = [np.ascontiguousarray(s, dtype=np.uint8) for s in srcs]
@@ -91,10 +97,21 @@ def calc_cost(ref, srcs, homs, idep_min, dep_sample, planes, precision="f64"):
                                       if S else np.zeros((0, 3)), np.float64))
     dt, ct = _dtype(precision)
     out = np.empty((H, W, planes), dt)
+    refp = _ptr(ref, ctypes.c_double if fref else ctypes.c_uint8)
+    args = [ctypes.c_int(W), ctypes.c_int(H), ctypes.c_int(planes), refp,
+            ctypes.c_int(S), arr, _ptr(h, ctypes.c_double), _ptr(t, ctypes.c_double),
+            ctypes.c_double(idep_min), ctypes.c_double(dep_sample), _ptr(out, ct)]
+    if fref:
+        assert precision == "f64" and not margin, "a float64 reference runs the FP64 oracle"
+        lib().oracle_calc_cost_fref_f64(*args)
+        return out
+    if margin:
+        assert precision == "f64", "the validity margin comes with the FP64 oracle"
+        mg = np.empty((H, W, planes), np.float32)
+        lib().oracle_calc_cost_margin_f64(*args, _ptr(mg, ctypes.c_float))
+        return out, mg
     fn = lib().oracle_calc_cost_f64 if precision == "f64" else lib().oracle_calc_cost_f32
-    fn(ctypes.c_int(W), ctypes.c_int(H), ctypes.c_int(planes), _ptr(ref, ctypes.c_uint8),
-       ctypes.c_int(S), arr, _ptr(h, ctypes.c_double), _ptr(t, ctypes.c_double),
-       ctypes.c_double(idep_min), ctypes.c_double(dep_sample), _ptr(out, ct))
+    fn(*args)
     return out
 
 
@@ -104,16 +121,23 @@ def _knob_array(knobs) -> np.ndarray:
 
 
 def sgm(ref, cost, knobs, paths, precision="f64"):
-    """Multi-path semi-global aggregation of a ``[H][W][D]`` cost volume."""
-    ref = np.ascontiguousarray(ref, dtype=np.uint8)
+    """Multi-path semi-global aggregation of a ``[H][W][D]`` cost volume.  A float64
+    ``ref`` (FP64 only) is used as real-valued intensities for the P1/P2 switch."""
+    fref = np.asarray(ref).dtype == np.float64
+    ref = np.ascontiguousarray(ref, dtype=np.float64 if fref else np.uint8)
     H, W = ref.shape
     dt, ct = _dtype(precision)
     cost = np.ascontiguousarray(cost, dtype=dt)
     D = cost.shape[2]
     agg = np.empty_like(cost)
     kn = _knob_array(knobs)
-    fn = lib().oracle_sgm_f64 if precision == "f64" else lib().oracle_sgm_f32
-    fn(ctypes.c_int(W), ctypes.c_int(H), ctypes.c_int(D), _ptr(ref, ctypes.c_uint8),
+    if fref:
+        assert precision == "f64", "a float64 reference runs the FP64 oracle"
+        fn, refp = lib().oracle_sgm_fref_f64, _ptr(ref, ctypes.c_double)
+    else:
+        fn = lib().oracle_sgm_f64 if precision == "f64" else lib().oracle_sgm_f32
+        refp = _ptr(ref, ctypes.c_uint8)
+    fn(ctypes.c_int(W), ctypes.c_int(H), ctypes.c_int(D), refp,
        _ptr(cost, ct), _ptr(kn, ctypes.c_double), ctypes.c_int(paths), _ptr(agg, ct))
     return agg
 

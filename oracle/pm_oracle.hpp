@@ -55,8 +55,8 @@ constexpr int kPathDy[8] = {0, 0, 1, -1, 1, -1, -1, 1};
 // Texel read at an integer position with border addressing (cudaAddressModeBorder,
 // calc_cost.cu:230-231): outside the image reads 0.  tex2D at (x+0.5, y+0.5) with linear
 // filtering lands exactly on texel (x, y).
-template <typename T>
-inline T texel(const uint8_t* img, int W, int H, long x, long y) {
+template <typename T, typename IT>
+inline T texel(const IT* img, int W, int H, long x, long y) {
   if (x < 0 || y < 0 || x >= W || y >= H) return T(0);
   return T(img[y * (long)W + x]);
 }
@@ -93,15 +93,26 @@ struct Source {
 //   * the result is the mean over contributing sources of (sum of 9 |diffs|)/9, or -1 if no
 //     source contributed (initReference sets -1, stereo_mapper.cpp:69).
 // idep_d = idep_min + d * dep_sample (calc_cost.cu:81 with idep_min = 0).
-template <typename T>
-void calc_cost(int W, int H, int D, const uint8_t* ref, const Source* src, int S,
-               T idep_min, T dep_sample, T* cost /* [H][W][D] */) {
+//
+// margin (optional, [H][W][D]): how close the entry is to a different set of contributing
+// sources -- the smallest, over sources, of |m_s|, where m_s is the signed pixel distance of
+// source s's decisive warp to the image-bounds test (min over the 9 warps of
+// min(U, W-1-U, V, H-1-V); the source contributes iff m_s >= 0 and every w >= 0).  A
+// computation whose warp positions carry an error e can change the entry's source set only
+// where margin < e.  Warps with w < 0 are far from flipping and do not lower the margin.
+// RT: the reference image's element type (uint8 as the reference uploads it; double for the
+// finite-difference checks of the differentiable readout's intensity gradient).
+template <typename T, typename RT = uint8_t>
+void calc_cost(int W, int H, int D, const RT* ref, const Source* src, int S,
+               T idep_min, T dep_sample, T* cost /* [H][W][D] */, float* margin = nullptr) {
 #pragma omp parallel for schedule(dynamic, 1)
   for (int v = 0; v < H; ++v) {
     std::vector<T> sum(D), cnt(D);
+    std::vector<double> mrg(D);
     for (int u = 0; u < W; ++u) {
       std::fill(sum.begin(), sum.end(), T(0));
       std::fill(cnt.begin(), cnt.end(), T(0));
+      std::fill(mrg.begin(), mrg.end(), std::numeric_limits<double>::infinity());
       // reference samples (calc_cost.cu:93-198)
       const T r[9] = {texel<T>(ref, W, H, u, v),         texel<T>(ref, W, H, u, v + 1),
                       texel<T>(ref, W, H, u, v - 1),     texel<T>(ref, W, H, u - 1, v),
@@ -142,10 +153,26 @@ void calc_cost(int W, int H, int D, const uint8_t* ref, const Source* src, int S
             sum[d] += tmp / T(9);
             cnt[d] += T(1);
           }
+          if (margin) {
+            double ms = std::numeric_limits<double>::infinity();
+            bool behind = false;
+            for (int k = 0; k < 9; ++k) {
+              const double w = double(Z[k] + t3 * idep);
+              if (!(w > 0.0)) { behind = true; break; }
+              const double U = double(X[k] + t1 * idep) / w;
+              const double V = double(Y[k] + t2 * idep) / w;
+              ms = std::min({ms, U, double(W - 1) - U, V, double(H - 1) - V});
+            }
+            if (!behind) mrg[d] = std::min(mrg[d], std::fabs(ms));
+          }
         }
       }
       T* out = cost + ((long)v * W + u) * D;
       for (int d = 0; d < D; ++d) out[d] = cnt[d] > T(0) ? sum[d] / cnt[d] : T(-1);
+      if (margin) {
+        float* mo = margin + ((long)v * W + u) * D;
+        for (int d = 0; d < D; ++d) mo[d] = (float)std::min(mrg[d], 1e30);
+      }
     }
   }
 }
@@ -178,8 +205,8 @@ inline void scanline_start(int W, int H, int dx, int dy, long i, int* x, int* y)
 //     (sgm.cu:75-82);
 //   * the aggregate is the sum of L over the paths, accumulated in path order into a
 //     buffer that starts at 0 (stereo_mapper.cpp:118).
-template <typename T>
-void sgm(int W, int H, int D, const uint8_t* ref, const T* cost, const Knobs& kn,
+template <typename T, typename RT = uint8_t>
+void sgm(int W, int H, int D, const RT* ref, const T* cost, const Knobs& kn,
          T* agg /* [H][W][D] */) {
   const long N = (long)W * H;
   std::vector<uint8_t> invalid(N);

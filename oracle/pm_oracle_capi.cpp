@@ -9,16 +9,17 @@
 
 namespace {
 
-template <typename T>
-void cost_impl(int W, int H, int D, const uint8_t* ref, int S, const uint8_t* const* imgs,
-               const double* h, const double* t, double idep_min, double dep_sample, T* cost) {
+template <typename T, typename RT = uint8_t>
+void cost_impl(int W, int H, int D, const RT* ref, int S, const uint8_t* const* imgs,
+               const double* h, const double* t, double idep_min, double dep_sample, T* cost,
+               float* margin = nullptr) {
   std::vector<pmo::Source> src(S);
   for (int s = 0; s < S; ++s) {
     src[s].img = imgs[s];
     for (int k = 0; k < 9; ++k) src[s].h[k] = h[s * 9 + k];
     for (int k = 0; k < 3; ++k) src[s].t[k] = t[s * 3 + k];
   }
-  pmo::calc_cost<T>(W, H, D, ref, src.data(), S, T(idep_min), T(dep_sample), cost);
+  pmo::calc_cost<T, RT>(W, H, D, ref, src.data(), S, T(idep_min), T(dep_sample), cost, margin);
 }
 
 pmo::Knobs knobs_from(const double* k, int paths) {
@@ -53,6 +54,24 @@ void oracle_calc_cost_f64(int W, int H, int D, const uint8_t* ref, int S,
                           const uint8_t* const* imgs, const double* h, const double* t,
                           double idep_min, double dep_sample, double* cost) {
   cost_impl<double>(W, H, D, ref, S, imgs, h, t, idep_min, dep_sample, cost);
+}
+// FP64 cost and aggregation of a real-valued reference image (the intensity finite
+// differences of the differentiable readout's checks)
+void oracle_calc_cost_fref_f64(int W, int H, int D, const double* ref, int S,
+                               const uint8_t* const* imgs, const double* h, const double* t,
+                               double idep_min, double dep_sample, double* cost) {
+  cost_impl<double, double>(W, H, D, ref, S, imgs, h, t, idep_min, dep_sample, cost);
+}
+void oracle_sgm_fref_f64(int W, int H, int D, const double* ref, const double* cost,
+                         const double* knobs, int paths, double* agg) {
+  pmo::sgm<double, double>(W, H, D, ref, cost, knobs_from(knobs, paths), agg);
+}
+// the FP64 cost with each entry's validity margin (pmo::calc_cost)
+void oracle_calc_cost_margin_f64(int W, int H, int D, const uint8_t* ref, int S,
+                                 const uint8_t* const* imgs, const double* h, const double* t,
+                                 double idep_min, double dep_sample, double* cost,
+                                 float* margin) {
+  cost_impl<double>(W, H, D, ref, S, imgs, h, t, idep_min, dep_sample, cost, margin);
 }
 void oracle_calc_cost_f32(int W, int H, int D, const uint8_t* ref, int S,
                           const uint8_t* const* imgs, const double* h, const double* t,

@@ -35,8 +35,9 @@ class OracleSequence:
     """The keyframe loop of StereoMapper::outputFusion on the CPU oracle.
 
     ``precision`` = "f64" (the oracle) or "f32" (the CPU baseline).  After each keyframe the
-    attributes ``cost``, ``agg``, ``depth`` (filtered), ``fused`` and ``state`` hold that
-    keyframe's results.
+    attributes ``cost``, ``agg``, ``depth_raw`` (filtered), ``depth`` (filtered, then the
+    median if enabled: what fusion consumes), ``fused`` and ``state`` hold that keyframe's
+    results.
     """
 
     def __init__(self, images, poses, sources, K, planes, idep_min, idep_max, knobs, paths,
@@ -72,6 +73,7 @@ class OracleSequence:
         self.agg = sgm(self.images[i], self.cost, self.knobs, self.paths, self.precision)
         d = filter_cost(self.agg, self.knobs["var_scale"], self.idep_min, self.dep_sample,
                         self.precision)
+        self.depth_raw = d
         self.depth = median3x3(d, self.precision) if self.median else d
         return self.depth
 

@@ -77,7 +77,7 @@ class Keyframe(ctypes.Structure):
 class BatchTuning(ctypes.Structure):
     _fields_ = [("cost_tile_rows", ctypes.c_int32), ("cost_stage_planes", ctypes.c_int32),
                 ("sgm_prefetch", ctypes.c_int32), ("cost_blocks_per_cu", ctypes.c_int32),
-                ("sgm_blocks_per_cu", ctypes.c_int32)]
+                ("sgm_blocks_per_cu", ctypes.c_int32), ("cost_plane_split", ctypes.c_int32)]
 
 
 class Knobs(ctypes.Structure):
@@ -128,8 +128,9 @@ SIGNATURES = {
                               _vp]),
     "pm_wta": (_i32, [Shape, _vp, _i32, ctypes.c_float, ctypes.c_float, _vp, _vp]),
     "pm_median3x3": (_i32, [_vp, _vp, _i32, _i32, _vp]),
-    "pm_depth_fuse": (_i32, [_i32, _i32, Camera, _fp, _fp, _vp, _vp, _vp, FusionState,
-                             FusionState, _vp]),
+    "pm_depth_fuse": (_i32, [_i32, _i32, Camera, _fp, _fp, _vp, _vp, _vp, _vp, _vp,
+                             FusionState, FusionState, _vp]),
+    "pm_zbuf_reset": (_i32, [_vp, ctypes.c_size_t, _vp]),
     "pm_fill_f32": (_i32, [_vp, ctypes.c_size_t, ctypes.c_float, _vp]),
     "pm_batch_create": (_i32, [ctypes.POINTER(Keyframe), _i32, _i32, SgmParams, ctypes.c_float,
                                _i32, ctypes.POINTER(_vp)]),

@@ -82,13 +82,26 @@ def build_evaluate() -> str:
     return EVALUATE
 
 
+BENCH_MAPPER = os.path.join(HERE, "pm_bench_mapper")
+
+
+def build_bench_mapper() -> str:
+    """The C++ timing driver of the StereoMapper sequence (bench.py --form mapper)."""
+    _run([_hipcc(), "-O2", "-std=c++17", "-I", INCLUDE, "-I", CSRC,
+          os.path.join(CSRC, "pm_bench_mapper.cpp"), "-L", HERE, "-lpm_hip",
+          "-Wl,-rpath,$ORIGIN", "-o", BENCH_MAPPER + ".tmp"])
+    os.replace(BENCH_MAPPER + ".tmp", BENCH_MAPPER)
+    return BENCH_MAPPER
+
+
 def build_all() -> None:
     build_synth()
     build_hip()
     build_evaluate()
+    build_bench_mapper()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     {"all": build_all, "hip": build_hip, "synth": build_synth,
-     "evaluate": build_evaluate}[what]()
+     "evaluate": build_evaluate, "bench_mapper": build_bench_mapper}[what]()

@@ -38,6 +38,13 @@ struct SgmClass {
   int sw_first_off = 0, sw_total = 0, sw_blocks = 0;
 };
 
+// The cost work list for one (tile rows, plane split) choice: item offsets and plane groups
+// per keyframe, and the 8 XCD ranges [xr[x], xr[8 + x]) of about equal work.
+struct CostList {
+  int first_off = 0, npg_off = 0, xr_off = 0;
+  int total = 0, max_range = 0;
+};
+
 struct FilterClass {
   int lp = 16;
   std::vector<int> kf;
@@ -60,11 +67,10 @@ struct pm_batch {
   std::vector<pm::KfDesc> host;
   pm::KfDesc* d_kfs = nullptr;
   int* d_pool = nullptr;
-  int cost_first_off[3] = {0, 0, 0};   // tile rows 2, 4, 8
-  int cost_total[3] = {0, 0, 0};
+  pm::CostList cost[3][2];              // [tile rows 2, 4, 8][plane split off, on]
   std::vector<pm::SgmClass> sgm;
   std::vector<pm::FilterClass> filt;
-  pm_batch_tuning tune{4, 8, 16, 0, 0};
+  pm_batch_tuning tune{4, 8, 16, 0, 0, 0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -94,16 +100,31 @@ bool valid_tuning(const pm_batch_tuning& t, int max_sources) {
   return cost_shape_built(t.cost_tile_rows, t.cost_stage_planes) &&
          cost_rows_fit(t.cost_tile_rows, max_sources) &&
          (t.sgm_prefetch == 8 || t.sgm_prefetch == 16) && t.cost_blocks_per_cu >= 0 &&
-         t.sgm_blocks_per_cu >= 0 && t.cost_blocks_per_cu <= 64 && t.sgm_blocks_per_cu <= 64;
+         t.sgm_blocks_per_cu >= 0 && t.cost_blocks_per_cu <= 64 && t.sgm_blocks_per_cu <= 64 &&
+         (t.cost_plane_split == 0 || t.cost_plane_split == 1);
+}
+
+// Plane groups of a keyframe's cost tiles: groups of 32 planes (16 when D is a multiple of 16
+// but not of 32), so that each wave's share stays a multiple of 4 planes; else one group.
+int cost_plane_groups(int D, bool split) {
+  if (!split) return 1;
+  if (D % 32 == 0) return D / 32;
+  if (D % 16 == 0) return D / 16;
+  return 1;
 }
 
 pm_status run_cost(pm_batch* b, hipStream_t st) {
   const int ti = b->tune.cost_tile_rows == 2 ? 0 : (b->tune.cost_tile_rows == 4 ? 1 : 2);
-  const int total = b->cost_total[ti];
-  if (total == 0) return PM_OK;
+  const CostList& cl = b->cost[ti][b->tune.cost_plane_split ? 1 : 0];
+  if (cl.total == 0) return PM_OK;
+  // 0 blocks per CU: one block per item of the longest XCD range; else a persistent grid
+  const int grid = b->tune.cost_blocks_per_cu <= 0
+                       ? 8 * cl.max_range
+                       : std::min(8 * cl.max_range,
+                                  round_up(std::max(8, b->tune.cost_blocks_per_cu * b->cus), 8));
   if (!launch_cost_batch(b->ct, b->tune.cost_tile_rows, b->tune.cost_stage_planes, b->d_kfs,
-                         b->d_pool + b->cost_first_off[ti], b->nkf, total,
-                         grid_for(total, b->tune.cost_blocks_per_cu, b->cus), st))
+                         b->d_pool + cl.first_off, b->d_pool + cl.npg_off, b->nkf,
+                         b->d_pool + cl.xr_off, grid, st))
     return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch: unsupported cost tuning");
   return check_launch("cost_batch_kernel");
 }
@@ -270,16 +291,45 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     pool.insert(pool.end(), v.begin(), v.end());
     return off;
   };
-  for (int ti = 0; ti < 3; ++ti) {
-    std::vector<int> first(count);
-    int acc = 0;
-    for (int k = 0; k < count; ++k) {
-      first[k] = acc;
-      acc += cost_items(b->host[k].c.W, b->host[k].c.H, 2 << ti);
+  for (int ti = 0; ti < 3; ++ti)
+    for (int sp = 0; sp < 2; ++sp) {
+      std::vector<int> first(count), npg(count);
+      std::vector<double> kw(count);   // work of one item of keyframe k
+      int acc = 0;
+      double work = 0.0;
+      for (int k = 0; k < count; ++k) {
+        const CostArgs& c = b->host[k].c;
+        first[k] = acc;
+        npg[k] = cost_plane_groups(c.D, sp == 1);
+        const int items = cost_items(c.W, c.H, 2 << ti) * npg[k];
+        // per (plane, source) passes plus the per-plane record and staging work
+        kw[k] = (double)((c.D + npg[k] - 1) / npg[k]) * (c.S + 1);
+        acc += items;
+        work += kw[k] * items;
+      }
+      // XCD x takes the items whose work midpoint falls in [x, x + 1) / 8 of the total
+      std::vector<int> xr(16, acc);
+      int x = 0;
+      double cum = 0.0;
+      xr[0] = 0;
+      for (int k = 0; k < count && x < 7; ++k) {
+        const int items = (k + 1 < count ? first[k + 1] : acc) - first[k];
+        for (int j = 0; j < items && x < 7; ++j) {
+          const double mid = cum + 0.5 * kw[k];
+          while (x < 7 && mid >= work * (x + 1) / 8.0) xr[++x] = first[k] + j;
+          cum += kw[k];
+        }
+      }
+      while (x < 7) xr[++x] = acc;
+      for (int i = 0; i < 8; ++i) xr[8 + i] = i < 7 ? xr[i + 1] : acc;
+      CostList& cl = b->cost[ti][sp];
+      cl.first_off = put(first);
+      cl.npg_off = put(npg);
+      cl.xr_off = put(xr);
+      cl.total = acc;
+      cl.max_range = 0;
+      for (int i = 0; i < 8; ++i) cl.max_range = std::max(cl.max_range, xr[8 + i] - xr[i]);
     }
-    b->cost_first_off[ti] = put(first);
-    b->cost_total[ti] = acc;
-  }
   for (int k = 0; k < count; ++k) {
     const int D = b->host[k].c.D;
     const int nv = D <= 64 ? 1 : (D <= 128 ? 2 : 4);
@@ -514,22 +564,25 @@ extern "C" pm_status pm_batch_autotune(pm_batch* b, pm_stream stream, int32_t re
   const int rows[3] = {2, 4, 8}, planes[3] = {4, 8, 16}, bpc[4] = {0, 4, 8, 16};
   for (int r : rows)
     for (int p : planes)
-      for (int g : bpc) {
-        if (status != PM_OK) break;
-        if (!cost_shape_built(r, p) || !cost_rows_fit(r, b->max_sources)) continue;
-        b->tune = cur;
-        b->tune.cost_tile_rows = r;
-        b->tune.cost_stage_planes = p;
-        b->tune.cost_blocks_per_cu = g;
-        float ms = 0;
-        status = time_stage(run_cost, &ms);
-        if (status == PM_OK && ms < best_ms) {
-          best_ms = ms;
-          win.cost_tile_rows = r;
-          win.cost_stage_planes = p;
-          win.cost_blocks_per_cu = g;
+      for (int g : bpc)
+        for (int sp = 0; sp < 2; ++sp) {
+          if (status != PM_OK) break;
+          if (!cost_shape_built(r, p) || !cost_rows_fit(r, b->max_sources)) continue;
+          b->tune = cur;
+          b->tune.cost_tile_rows = r;
+          b->tune.cost_stage_planes = p;
+          b->tune.cost_blocks_per_cu = g;
+          b->tune.cost_plane_split = sp;
+          float ms = 0;
+          status = time_stage(run_cost, &ms);
+          if (status == PM_OK && ms < best_ms) {
+            best_ms = ms;
+            win.cost_tile_rows = r;
+            win.cost_stage_planes = p;
+            win.cost_blocks_per_cu = g;
+            win.cost_plane_split = sp;
+          }
         }
-      }
   best_ms = 1e30f;
   const int pref[2] = {8, 16};
   for (int f : pref)

@@ -108,7 +108,8 @@ __device__ __forceinline__ int owner_of(const int* __restrict__ first, int nkf, 
 // ---- batched launches (defined with their kernels; used by the batch host code) --------
 int cost_items(int W, int H, int tile_rows);
 bool launch_cost_batch(pm_cost_type ct, int tile_rows, int stage_planes, const KfDesc* kfs,
-                       const int* first, int nkf, int total, int grid, hipStream_t st);
+                       const int* first, const int* npg, int nkf, const int* xr, int grid,
+                       hipStream_t st);
 bool launch_sgm_batch_path(pm_cost_type ct, int prefetch, int nv, bool full, const KfDesc* kfs,
                            const int* kf, const int* first, int n, int total, int grid,
                            bool first_path, int dx, int dy, hipStream_t st);

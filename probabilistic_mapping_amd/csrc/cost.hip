@@ -60,21 +60,25 @@ __global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, CT* __
                         nullptr, npg);
 }
 
-// Batched form: the tiles of every keyframe of a batch form one work list (keyframe k owns
-// items first[k] .. first[k+1]-1).  Persistent blocks stride through it; with a grid that is a
-// multiple of 8, block i always runs on XCD i % 8, and xcd_block gives each XCD a contiguous
-// range of the list.
+// Batched form: the work items of every keyframe of a batch form one list (keyframe k owns
+// items first[k] .. first[k+1]-1, each a tile or, with npg[k] > 1, one plane group of a tile).
+// The list is cut into 8 contiguous ranges of about equal work, [xr[x], xr[8 + x]) for XCD x:
+// workgroups are dispatched round-robin over the XCDs (block i runs on XCD i % 8, the grid is
+// a multiple of 8), so XCD x's blocks stride through range x, and neighbouring tiles share
+// that XCD's L2.
 template <typename CT, int TY, int DC>
 __global__ __launch_bounds__(256) void cost_batch_kernel(const KfDesc* __restrict__ kfs,
-                                                         const int* __restrict__ first, int nkf,
-                                                         int total) {
+                                                         const int* __restrict__ first,
+                                                         const int* __restrict__ npg, int nkf,
+                                                         const int* __restrict__ xr) {
   __shared__ float stage[4][TY * DC * 64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int i = blockIdx.x; i < total; i += gridDim.x) {
-    const int item = xcd_block(i, total);
-    const int k = owner_of(first, nkf, item);
-    cost_tile<CT, TY, DC>(kfs[k].c, item - first[k], static_cast<CT*>(kfs[k].cost),
-                          stage[wave]);
+  const int x = blockIdx.x & 7;
+  const int end = xr[8 + x];
+  for (int j = xr[x] + (int)(blockIdx.x >> 3); j < end; j += (int)(gridDim.x >> 3)) {
+    const int k = owner_of(first, nkf, j);
+    cost_tile<CT, TY, DC>(kfs[k].c, j - first[k], static_cast<CT*>(kfs[k].cost), stage[wave],
+                          nullptr, nullptr, npg[k]);
   }
 }
 
@@ -85,34 +89,38 @@ int cost_items(int W, int H, int tile_rows) {
 
 template <typename CT>
 bool launch_cost_batch_t(int tile_rows, int stage_planes, const KfDesc* kfs, const int* first,
-                         int nkf, int total, int grid, hipStream_t st) {
+                         const int* npg, int nkf, const int* xr, int grid, hipStream_t st) {
   if (tile_rows == 2 && stage_planes == 8)
-    cost_batch_kernel<CT, 2, 8><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+    cost_batch_kernel<CT, 2, 8><<<grid, 256, 0, st>>>(kfs, first, npg, nkf, xr);
   else if (tile_rows == 2 && stage_planes == 16)
-    cost_batch_kernel<CT, 2, 16><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+    cost_batch_kernel<CT, 2, 16><<<grid, 256, 0, st>>>(kfs, first, npg, nkf, xr);
   else if (tile_rows == 4 && stage_planes == 8)
-    cost_batch_kernel<CT, 4, 8><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+    cost_batch_kernel<CT, 4, 8><<<grid, 256, 0, st>>>(kfs, first, npg, nkf, xr);
   else if (tile_rows == 4 && stage_planes == 16)
-    cost_batch_kernel<CT, 4, 16><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+    cost_batch_kernel<CT, 4, 16><<<grid, 256, 0, st>>>(kfs, first, npg, nkf, xr);
   else if (tile_rows == 8 && stage_planes == 8)   // 64 KB of LDS per block
-    cost_batch_kernel<CT, 8, 8><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+    cost_batch_kernel<CT, 8, 8><<<grid, 256, 0, st>>>(kfs, first, npg, nkf, xr);
   else if (tile_rows == 8 && stage_planes == 4)
-    cost_batch_kernel<CT, 8, 4><<<grid, 256, 0, st>>>(kfs, first, nkf, total);
+    cost_batch_kernel<CT, 8, 4><<<grid, 256, 0, st>>>(kfs, first, npg, nkf, xr);
   else
     return false;
   return true;
 }
 
 bool launch_cost_batch(pm_cost_type ct, int tile_rows, int stage_planes, const KfDesc* kfs,
-                       const int* first, int nkf, int total, int grid, hipStream_t st) {
+                       const int* first, const int* npg, int nkf, const int* xr, int grid,
+                       hipStream_t st) {
+  if (grid <= 0 || grid % 8 != 0) return false;
   switch (ct) {
     case PM_COST_F32:
-      return launch_cost_batch_t<float>(tile_rows, stage_planes, kfs, first, nkf, total, grid, st);
+      return launch_cost_batch_t<float>(tile_rows, stage_planes, kfs, first, npg, nkf, xr, grid,
+                                        st);
     case PM_COST_F16:
-      return launch_cost_batch_t<__half>(tile_rows, stage_planes, kfs, first, nkf, total, grid, st);
+      return launch_cost_batch_t<__half>(tile_rows, stage_planes, kfs, first, npg, nkf, xr, grid,
+                                         st);
     case PM_COST_BF16:
-      return launch_cost_batch_t<__hip_bfloat16>(tile_rows, stage_planes, kfs, first, nkf, total,
-                                                 grid, st);
+      return launch_cost_batch_t<__hip_bfloat16>(tile_rows, stage_planes, kfs, first, npg, nkf,
+                                                 xr, grid, st);
     default:
       return false;
   }

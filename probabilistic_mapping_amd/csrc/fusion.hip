@@ -14,8 +14,16 @@
 //       table's empty sentinel and never inserts (depth_fusion.cu:54, 61, 93, 128).
 //   fuse_update_kernel : resolves the table entry, fills a hole from the pre-fill z-buffer
 //       (lowest-linear-index interior 3x3 neighbour with an entry), gathers the predicted
-//       state and runs the update, writing the fused depth in place.
+//       state and runs the update, writing the fused depth.
 // Both rules are admissible outcomes of the reference's races.
+//
+// The z-buffer must be empty (all bits set) when the transform starts.  The update kernel
+// cannot empty its own entries behind it, since the hole fill reads the neighbours' entries
+// from other threads.  So a caller that runs steps back to back gives each step a second
+// buffer, zbuf_next, which the transform kernel empties pixel by pixel for the next step (the
+// two buffers alternate); without one, the step empties its z-buffer first (one more launch).
+#include <string>
+
 #include "common.hpp"
 
 namespace pm {
@@ -43,11 +51,13 @@ __global__ __launch_bounds__(256) void fuse_transform_kernel(const FuseArgs a,
                                                              const float* __restrict__ beta,
                                                              float* __restrict__ mu,
                                                              float* __restrict__ sig,
-                                                             uint64_t* __restrict__ zbuf) {
+                                                             uint64_t* __restrict__ zbuf,
+                                                             uint64_t* __restrict__ znext) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= a.W) return;
   const long i = (long)y * a.W + x;
+  if (znext) znext[i] = kEmpty;   // the next step's z-buffer, emptied for it
   const float al = alpha[i];
   if (al < kFloatEps) return;                                   // :26
   const float be = beta[i];
@@ -85,7 +95,9 @@ __global__ __launch_bounds__(256) void fuse_update_kernel(
     const FuseArgs a, const uint64_t* __restrict__ zbuf, int32_t* __restrict__ table,
     const float* __restrict__ pa, const float* __restrict__ pb, const float* __restrict__ pm_,
     const float* __restrict__ ps, float* __restrict__ ca, float* __restrict__ cb,
-    float* __restrict__ cm, float* __restrict__ cs, float* __restrict__ depth) {
+    float* __restrict__ cm, float* __restrict__ cs, const float* depth, float* fused) {
+  // depth and fused may be the same buffer (in place, as tmpDepthMap): each thread reads its
+  // own pixel of depth before writing that pixel of fused
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   const int W = a.W, H = a.H;
@@ -116,15 +128,16 @@ __global__ __launch_bounds__(256) void fuse_update_kernel(
   const float unc = (float)((double)(dest * dest) * 0.01);
   if (al < kFloatEps) {
     if (dest < kFloatEps) {
-      depth[j] = 0.0f;
+      fused[j] = 0.0f;
     } else {
       ca[j] = 10.0f; cb[j] = 10.0f; cm[j] = dest; cs[j] = 2500.0f;
+      fused[j] = dest;   // the init branch leaves the measurement as the output (:161-174)
     }
     return;
   }
   const float aa = al, bb = bv;
   if (dest < kFloatEps) {
-    depth[j] = ((double)(aa / (aa + bb)) > 0.6) ? mv : 0.0f;
+    fused[j] = ((double)(aa / (aa + bb)) > 0.6) ? mv : 0.0f;
     return;
   }
   const float miu = mv, sq = sv;
@@ -143,7 +156,7 @@ __global__ __launch_bounds__(256) void fuse_update_kernel(
   const float a_p = (e - f) / (f - e / f);
   const float b_p = a_p * (1.0f - f) / f;
   ca[j] = a_p; cb[j] = b_p; cm[j] = mu_p; cs[j] = sig_p;
-  depth[j] = ((double)(aa / (aa + bb)) > 0.6) ? mu_p : 0.0f;
+  fused[j] = ((double)(aa / (aa + bb)) > 0.6) ? mu_p : 0.0f;
 }
 
 }  // namespace pm
@@ -159,18 +172,22 @@ extern "C" pm_status pm_fill_f32(float* dst, size_t n, float value, pm_stream st
   return check_launch("fill_f32_kernel");
 }
 
-extern "C" pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam,
-                                   const float r_p2c[9], const float t_p2c[3], float* depth,
-                                   uint64_t* zbuf, int32_t* table, pm_fusion_state pre,
-                                   pm_fusion_state cur, pm_stream stream) {
-  clear_error();
-  if (!r_p2c || !t_p2c || !depth || !zbuf || !pre.alpha || !pre.beta || !pre.mu || !pre.sig ||
-      !cur.alpha || !cur.beta || !cur.mu || !cur.sig)
-    return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse: null pointer");
+// The checks and launches of one fusion step; zbuf_next == nullptr empties zbuf first.
+static pm_status depth_fuse_impl(const char* who, int32_t width, int32_t height, pm_camera cam,
+                                 const float* r_p2c, const float* t_p2c, const float* depth,
+                                 float* fused, uint64_t* zbuf, uint64_t* zbuf_next,
+                                 int32_t* table, pm_fusion_state pre, pm_fusion_state cur,
+                                 hipStream_t st) {
+  const std::string w(who);
+  if (!r_p2c || !t_p2c || !depth || !fused || !zbuf || !pre.alpha || !pre.beta || !pre.mu ||
+      !pre.sig || !cur.alpha || !cur.beta || !cur.mu || !cur.sig)
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": null pointer");
   if (width <= 0 || height <= 0 || width > 65535 || height > 65535)
-    return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse: bad image size");
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": bad image size");
   if (pre.alpha == cur.alpha || pre.beta == cur.beta || pre.mu == cur.mu || pre.sig == cur.sig)
-    return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse: pre and cur state must not alias");
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": pre and cur state must not alias");
+  if (zbuf_next == zbuf)
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": zbuf_next must be another buffer than zbuf");
   FuseArgs a{};
   a.W = width; a.H = height;
   a.fx = cam.fx; a.fy = cam.fy; a.cx = cam.cx; a.cy = cam.cy;
@@ -178,22 +195,44 @@ extern "C" pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam,
   for (int k = 0; k < 3; ++k) a.t[k] = t_p2c[k];
   if (!(std::isfinite(a.fx) && std::isfinite(a.fy) && std::isfinite(a.cx) &&
         std::isfinite(a.cy) && a.fx != 0.0f && a.fy != 0.0f))
-    return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse: bad intrinsics");
+    return fail(PM_ERR_INVALID_ARGUMENT, w + ": bad intrinsics");
   for (int k = 0; k < 9; ++k)
-    if (!std::isfinite(a.r[k])) return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse: bad pose");
+    if (!std::isfinite(a.r[k])) return fail(PM_ERR_INVALID_ARGUMENT, w + ": bad pose");
   for (int k = 0; k < 3; ++k)
-    if (!std::isfinite(a.t[k])) return fail(PM_ERR_INVALID_ARGUMENT, "pm_depth_fuse: bad pose");
-  hipStream_t st = (hipStream_t)stream;
+    if (!std::isfinite(a.t[k])) return fail(PM_ERR_INVALID_ARGUMENT, w + ": bad pose");
   const long N = (long)width * height;
-  fill_u64_kernel<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(zbuf, N, kEmpty);
-  pm_status s = check_launch("fill_u64_kernel");
-  if (s != PM_OK) return s;
+  pm_status s;
+  if (!zbuf_next) {
+    fill_u64_kernel<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(zbuf, N, kEmpty);
+    s = check_launch("fill_u64_kernel");
+    if (s != PM_OK) return s;
+  }
   dim3 grid((width + 255) / 256, height);
-  fuse_transform_kernel<<<grid, 256, 0, st>>>(a, pre.alpha, pre.beta, pre.mu, pre.sig, zbuf);
+  fuse_transform_kernel<<<grid, 256, 0, st>>>(a, pre.alpha, pre.beta, pre.mu, pre.sig, zbuf,
+                                              zbuf_next);
   s = check_launch("fuse_transform_kernel");
   if (s != PM_OK) return s;
   fuse_update_kernel<<<grid, 256, 0, st>>>(a, zbuf, table, pre.alpha, pre.beta, pre.mu,
                                             pre.sig, cur.alpha, cur.beta, cur.mu, cur.sig,
-                                            depth);
+                                            depth, fused);
   return check_launch("fuse_update_kernel");
+}
+
+extern "C" pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam,
+                                   const float r_p2c[9], const float t_p2c[3],
+                                   const float* depth, float* fused, uint64_t* zbuf,
+                                   uint64_t* zbuf_next, int32_t* table, pm_fusion_state pre,
+                                   pm_fusion_state cur, pm_stream stream) {
+  clear_error();
+  return depth_fuse_impl("pm_depth_fuse", width, height, cam, r_p2c, t_p2c, depth, fused, zbuf,
+                         zbuf_next, table, pre, cur, (hipStream_t)stream);
+}
+
+extern "C" pm_status pm_zbuf_reset(uint64_t* zbuf, size_t n, pm_stream stream) {
+  clear_error();
+  if (!zbuf && n) return fail(PM_ERR_INVALID_ARGUMENT, "pm_zbuf_reset: null pointer");
+  if (n == 0) return PM_OK;
+  fill_u64_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(zbuf, (long)n,
+                                                                                 kEmpty);
+  return check_launch("fill_u64_kernel");
 }

@@ -43,23 +43,14 @@ int storage_type(pm_cost_type t) {
   return t == PM_COST_F16 ? PM_16F : (t == PM_COST_BF16 ? PM_16BF : CV_32F);
 }
 
-// Host copy of a dense rows x cols image into a device buffer of the same shape.
-void upload_image(const Mat& img, const DeviceMat& dst, hipStream_t st, const char* who) {
-  if (img.empty() || img.rows != dst.rows || img.cols != dst.cols)
+// The checks the reference's cv::Mat inputs get implicitly: size and 8-bit grayscale type.
+void check_image(const Mat& img, int rows, int cols, const char* who) {
+  if (img.empty() || img.rows != rows || img.cols != cols)
     throw Error(PM_ERR_INVALID_ARGUMENT, std::string(who) + ": image size must be " +
-                                             std::to_string(dst.cols) + "x" +
-                                             std::to_string(dst.rows));
+                                             std::to_string(cols) + "x" +
+                                             std::to_string(rows));
   if (img.type != CV_8U)
     throw Error(PM_ERR_INVALID_ARGUMENT, std::string(who) + ": image must be 8-bit grayscale");
-  // a dense image goes as one copy (a strided pageable copy moves row by row)
-  if (img.step == (size_t)img.cols && dst.step == (size_t)dst.cols)
-    hip_check(hipMemcpyAsync(dst.data, img.data, (size_t)img.cols * img.rows,
-                             hipMemcpyHostToDevice, st),
-              "upload");
-  else
-    hip_check(hipMemcpy2DAsync(dst.data, dst.step, img.data, img.step, (size_t)img.cols,
-                               (size_t)img.rows, hipMemcpyHostToDevice, st),
-              "upload");
 }
 
 void mat_to_array(const Mat& m, int rows, int cols, double* out, const char* what) {
@@ -118,7 +109,42 @@ StereoMapper::StereoMapper(int width, int height, int dep_cnt)
 StereoMapper::~StereoMapper() {
   if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
   release();
+  for (uint8_t* p : pinned_)
+    if (p) (void)hipHostFree(p);
+  for (void* e : pinned_ev_)
+    if (e) (void)hipEventDestroy((hipEvent_t)e);
   if (stream_) (void)hipStreamDestroy((hipStream_t)stream_);
+}
+
+// Copies a host image into pinned staging slot `slot` and enqueues its upload to `dst` (W x H
+// bytes) on the mapper's stream.  The slot's previous upload is waited for first, so the
+// caller's buffer is free when this returns and no stream synchronisation is needed.
+void StereoMapper::stage_upload(const Mat& img, uint8_t* dst, size_t slot, const char* who) {
+  check_image(img, H_, W_, who);
+  const size_t bytes = (size_t)W_ * H_;
+  if (pinned_.size() <= slot) {
+    pinned_.resize(slot + 1, nullptr);
+    pinned_ev_.resize(slot + 1, nullptr);
+  }
+  if (!pinned_[slot]) {
+    void* p = nullptr;
+    hip_check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc");
+    pinned_[slot] = static_cast<uint8_t*>(p);
+    hipEvent_t e = nullptr;
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    pinned_ev_[slot] = e;
+  } else {
+    hip_check(hipEventSynchronize((hipEvent_t)pinned_ev_[slot]), "hipEventSynchronize");
+  }
+  uint8_t* p = pinned_[slot];
+  if (img.step == (size_t)W_) {
+    std::memcpy(p, img.data, bytes);
+  } else {
+    for (int y = 0; y < H_; ++y) std::memcpy(p + (size_t)y * W_, img.data + y * img.step, W_);
+  }
+  hipStream_t st = (hipStream_t)stream_;
+  hip_check(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, st), "upload");
+  hip_check(hipEventRecord((hipEvent_t)pinned_ev_[slot], st), "hipEventRecord");
 }
 
 void StereoMapper::alloc() {
@@ -132,10 +158,12 @@ void StereoMapper::alloc() {
   propogate_table = dalloc(H_, W_, CV_32S);
   DeviceMat* st[8] = {&alpha0, &beta0, &mu0, &sigma0, &alpha1, &beta1, &mu1, &sigma1};
   for (DeviceMat* m : st) *m = dalloc(1, (int)N, CV_32F);
-  zbuf_ = dalloc(1, (int)N, PM_64U);
+  zbuf_ = dalloc(2, (int)N, PM_64U);   // two z-buffers, used alternately by the fusion steps
   med_tmp_ = dalloc(H_, W_, CV_32F);
   wta_tmp_ = dalloc(H_, W_, CV_32F);
   hip_check(hipMemset(propogate_table.data, 0, propogate_table.bytes()), "hipMemset");
+  hip_check(hipMemset(zbuf_.data, 0xFF, zbuf_.bytes()), "hipMemset");   // empty entries
+  zpar_ = 0;
   for (DeviceMat* m : st) hip_check(hipMemset(m->data, 0, m->bytes()), "hipMemset");
 }
 
@@ -173,8 +201,7 @@ void StereoMapper::initIntrinsic(const Mat& _K1, float bf, float dep_sample) {
 
 // stereo_mapper.cpp:63-71: new reference frame; the cost volume restarts at -1
 void StereoMapper::initReference(const Mat& _img_l) {
-  hipStream_t st = (hipStream_t)stream_;
-  upload_image(_img_l, img_l, st, "initReference");
+  stage_upload(_img_l, img_l.data, 0, "initReference");
   if (raw_cost_type_ != cost_type_) {
     synchronize();
     dfree(raw_cost);
@@ -245,19 +272,15 @@ void StereoMapper::update(const Mat& _img_r, const Mat& R_l, const Mat& T_l, con
   const size_t slot = homs_.size();
   if (quads_.size() <= slot) quads_.resize(slot + 1);   // ring-staged slots stay unallocated
   if (!quads_[slot].data) quads_[slot] = dalloc(H_, 2 * W_, CV_32S);   // 8-byte texels
-  hipStream_t st = (hipStream_t)stream_;
-  // the u8 frame goes through the filtered-depth buffer's memory, which is free until output
-  DeviceMat staging = wta_tmp_;
-  staging.type = CV_8U;
-  staging.step = (size_t)W_;
-  upload_image(_img_r, staging, st, "update");
-  check(pm_make_quad(staging.data, reinterpret_cast<uint64_t*>(quads_[slot].data), W_, H_,
+  // the u8 frame goes through the WTA scratch's device memory, which is free until an
+  // output call; uploads and conversions of successive updates are ordered on the stream
+  stage_upload(_img_r, wta_tmp_.data, 1 + slot, "update");
+  check(pm_make_quad(wta_tmp_.data, reinterpret_cast<uint64_t*>(quads_[slot].data), W_, H_,
                      stream_));
   homs_.push_back(h);
   staged_.push_back(RingFrame{nullptr, -1});
   cost_ready_ = false;
   depth_ready_ = false;
-  synchronize();   // the caller may reuse its image buffer after update returns
 }
 
 // update from an ingested frame: only the homography is set up here; the cost pass reads
@@ -338,7 +361,6 @@ void StereoMapper::output(Mat& result) {
 void StereoMapper::outputFusion(Mat& result, Matrix3d Rw, Vector3d tw, float fx, float fy,
                                 float cx, float cy) {
   run_sgm_filter();
-  hipStream_t st = (hipStream_t)stream_;
   const size_t N = (size_t)W_ * H_;
   if (firstDepth) {   // :128-146
     firstDepth = false;
@@ -353,9 +375,7 @@ void StereoMapper::outputFusion(Mat& result, Matrix3d Rw, Vector3d tw, float fx,
     synchronize();
     return;
   }
-  // :149-152 (dense layouts: the filtered depth is copied, not converted)
-  hip_check(hipMemcpyAsync(fuseDepth.data, dep.data, dep.bytes(), hipMemcpyDeviceToDevice, st),
-            "copy depth");
+  // :149-152 (dense layouts: the fusion step reads dep and writes fuseDepth, no conversion)
   // :155-156, FP64 then cast to float (:167-170)
   const Matrix3d R_p_2_c = Rw.transpose() * R_b_2_w;
   const Vector3d t_p_2_c = Rw.transpose() * (t_b_2_w - tw);
@@ -370,11 +390,13 @@ void StereoMapper::outputFusion(Mat& result, Matrix3d Rw, Vector3d tw, float fx,
     return pm_fusion_state{reinterpret_cast<float*>(s[0]->data), reinterpret_cast<float*>(s[1]->data),
                            reinterpret_cast<float*>(s[2]->data), reinterpret_cast<float*>(s[3]->data)};
   };
+  uint64_t* zb = reinterpret_cast<uint64_t*>(zbuf_.data);
   check(pm_depth_fuse(W_, H_, pm_camera{fx, fy, cx, cy}, r, t,
-                      reinterpret_cast<float*>(fuseDepth.data),
-                      reinterpret_cast<uint64_t*>(zbuf_.data),
-                      reinterpret_cast<int32_t*>(propogate_table.data), state(pre), state(cur),
-                      stream_));
+                      reinterpret_cast<const float*>(dep.data),
+                      reinterpret_cast<float*>(fuseDepth.data), zb + N * zpar_,
+                      zb + N * (zpar_ ^ 1), reinterpret_cast<int32_t*>(propogate_table.data),
+                      state(pre), state(cur), stream_));
+  zpar_ ^= 1;
   synchronize();
   dep.download(result);   // :229-230 -- the filtered, not the fused, depth
   R_b_2_w = Rw;            // :237-241

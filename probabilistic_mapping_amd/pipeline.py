@@ -208,14 +208,26 @@ def _state(s: dict) -> C.FusionState:
 
 
 def depth_fuse(depth: torch.Tensor, zbuf: torch.Tensor, pre: dict, cur: dict, r_p2c, t_p2c,
-               cam, table: torch.Tensor | None = None) -> None:
-    """One Gaussian x Beta fusion step, in place on ``depth``, ``pre`` and ``cur``."""
+               cam, table: torch.Tensor | None = None, fused: torch.Tensor | None = None,
+               zbuf_next: torch.Tensor | None = None) -> None:
+    """One Gaussian x Beta fusion step on ``pre`` and ``cur``; the fused depth goes to
+    ``fused`` (default: ``depth`` itself, in place).  Without ``zbuf_next`` the step empties
+    ``zbuf`` first; with it, ``zbuf`` must be empty and ``zbuf_next`` is emptied for the next
+    step (see :func:`zbuf_reset`)."""
     H, W = depth.shape
     r = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(r_p2c, np.float32).reshape(9)])
     t = (ctypes.c_float * 3)(*[float(v) for v in np.asarray(t_p2c, np.float32).reshape(3)])
     tab = _ptr(table) if table is not None else ctypes.c_void_p(0)
+    out = fused if fused is not None else depth
+    nxt = _ptr(zbuf_next) if zbuf_next is not None else ctypes.c_void_p(0)
     C.check(C.load().pm_depth_fuse(W, H, C.Camera(*[float(c) for c in cam]), r, t, _ptr(depth),
-                                   _ptr(zbuf), tab, _state(pre), _state(cur), _stream()))
+                                   _ptr(out), _ptr(zbuf), nxt, tab, _state(pre), _state(cur),
+                                   _stream()))
+
+
+def zbuf_reset(zbuf: torch.Tensor) -> None:
+    """Empties a fusion z-buffer (int64 tensor) on the current stream."""
+    C.check(C.load().pm_zbuf_reset(_ptr(zbuf), zbuf.numel(), _stream()))
 
 
 def depth_fuse_cpu_rules(depth: torch.Tensor, pre: dict, cur: dict, r_p2c, t_p2c, cam,
@@ -297,7 +309,9 @@ class KeyframeSequence:
         self.dep = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.dep_raw = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.fused = torch.empty((H, W), dtype=torch.float32, device=self.device)
-        self.zbuf = torch.empty((H, W), dtype=torch.int64, device=self.device)
+        # two z-buffers used alternately: each fusion step empties the other for the next one
+        self.zbufs = [torch.empty((H, W), dtype=torch.int64, device=self.device)
+                      for _ in range(2)]
         self.table = torch.empty((H, W), dtype=torch.int32, device=self.device)
         self.states = [new_state(H, W, self.device), new_state(H, W, self.device)]
         self.reset()
@@ -307,6 +321,9 @@ class KeyframeSequence:
         self.pre = 0
         self.prev_pose = None
         self.prev_index = None
+        self.zpar = 0
+        for z in self.zbufs:
+            zbuf_reset(z)
 
     def _cost_args(self, i: int):
         """ctypes arguments of keyframe i's pm_calc_cost (built once; the stream is added per
@@ -369,9 +386,9 @@ class KeyframeSequence:
             self.prev_index = i
             self.fused.zero_()
             return
-        self.fused.copy_(self.dep)
-        # the pm_depth_fuse arguments of (keyframe, previous keyframe, state parity), built once
-        key = (i, self.prev_index, self.pre)
+        # the pm_depth_fuse arguments of (keyframe, previous keyframe, state parity, z-buffer
+        # parity, depth buffer), built once
+        key = (i, self.prev_index, self.pre, self.zpar, self.dep.data_ptr())
         if not hasattr(self, "_fuse_cache"):
             self._fuse_cache = {}
         if key not in self._fuse_cache:
@@ -380,12 +397,14 @@ class KeyframeSequence:
             rr = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(r, np.float32).reshape(9)])
             ttc = (ctypes.c_float * 3)(*[float(v) for v in np.asarray(tt, np.float32).reshape(3)])
             self._fuse_cache[key] = (self.W, self.H, C.Camera(*[float(c) for c in self.cam]), rr,
-                                     ttc, _ptr(self.fused), _ptr(self.zbuf), _ptr(self.table),
-                                     _state(pre), _state(cur))
+                                     ttc, _ptr(self.dep), _ptr(self.fused),
+                                     _ptr(self.zbufs[self.zpar]), _ptr(self.zbufs[self.zpar ^ 1]),
+                                     _ptr(self.table), _state(pre), _state(cur))
         C.check(C.load().pm_depth_fuse(*self._fuse_cache[key], _stream()))
         self.prev_pose = (R, t)
         self.prev_index = i
         self.pre ^= 1
+        self.zpar ^= 1
 
     def run_keyframe(self, i: int) -> None:
         self.depth_stage(i)
@@ -400,6 +419,21 @@ class KeyframeSequence:
     def state(self) -> dict:
         """The state written by the last fusion step."""
         return self.states[self.pre]
+
+
+def keyframe_spec(kf: dict, device="cuda") -> dict:
+    """A ``Batch`` keyframe from host data: ``ref`` and ``srcs`` (uint8 images), ``K``,
+    ``ref_pose`` and ``src_poses`` (camera->world R, T), ``planes``, ``idep_min`` and
+    ``idep_max``.  Uploads the frames, builds the sources' quad textures and the FP64
+    homography setup of each source (stereo_mapper.cpp:79-90)."""
+    ref = torch.from_numpy(np.ascontiguousarray(kf["ref"])).to(device)
+    quads = [make_quad(torch.from_numpy(np.ascontiguousarray(im)).to(device))
+             for im in kf["srcs"]]
+    homs = [homography(kf["K"], *kf["ref_pose"], R, T) for R, T in kf["src_poses"]]
+    D = int(kf["planes"])
+    return {"ref": ref, "quads": quads, "homs": homs, "planes": D,
+            "idep_min": float(kf["idep_min"]),
+            "dep_sample": float((kf["idep_max"] - kf["idep_min"]) / (D - 1))}
 
 
 class Batch:
@@ -524,7 +558,12 @@ class BatchGroups:
 
     @property
     def tuning(self) -> dict:
+        """The first group's launch shapes (``tunings`` lists every group's)."""
         return self.batches[0].tuning
+
+    @property
+    def tunings(self) -> list[dict]:
+        return [b.tuning for b in self.batches]
 
     @tuning.setter
     def tuning(self, values: dict):
@@ -532,11 +571,11 @@ class BatchGroups:
             b.tuning = values
 
     def autotune(self, reps: int = 3) -> dict:
-        """Tunes on the first group (the largest) and applies its choice to all."""
-        best = self.batches[0].autotune(reps)
-        for b in self.batches[1:]:
-            b.tuning = best
-        return best
+        """Tunes every group on its own keyframes (groups differ in size, source and plane
+        counts, and a shape valid for one may be refused for another); returns the first
+        group's choice."""
+        best = [b.autotune(reps) for b in self.batches]
+        return best[0]
 
     def run(self):
         for b in self.batches:

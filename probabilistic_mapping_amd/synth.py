@@ -13,6 +13,9 @@ Workloads:
 * :func:`trajectory_scene` -- 752x480, 12 random planes and boxes at 0.5-8 m, fx=fy=460,
   a 20-frame sinusoidal trajectory with 3-15 cm spacing, 5 nearest frames per reference,
   64 planes over inverse depth 0.125-2.0, 8-path SGM.
+* :func:`uneven_keyframes` -- 16 independent keyframes, each from its own scene and
+  trajectory, with the resolution drawn from {320x240, 640x480, 752x480, 1280x720}, the
+  source count from 2..8 and the plane count from {32, 64, 96, 128}.
 """
 from __future__ import annotations
 
@@ -258,3 +261,38 @@ def diff_scene(seed: int = 0, width: int = 640, height: int = 480, n_sources: in
         "knobs": dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0),
         "temperature": 0.05, "cost_scale": 255.0,
     }
+
+
+UNEVEN_RESOLUTIONS = ((320, 240), (640, 480), (752, 480), (1280, 720))
+UNEVEN_PLANES = (32, 64, 96, 128)
+# seed of the default 16-keyframe draw: it holds all four resolutions and plane counts, source
+# counts 2..8, and the largest keyframe shape (1280x720, 128 planes, 8 sources)
+UNEVEN_SEED = 4
+
+
+def uneven_keyframes(seed: int = UNEVEN_SEED, count: int = 16):
+    """``count`` independent keyframes of uneven shape.  Per keyframe the seeded draw picks a
+    resolution from ``UNEVEN_RESOLUTIONS``, a source count uniform in 2..8 and a plane count
+    from ``UNEVEN_PLANES``; the keyframe is the middle frame of its own seeded scene's
+    ``S + 1``-frame trajectory (fx = fy = 0.612 W, the 752x480 workload's ratio) and its
+    sources are the other ``S`` frames.  Each entry holds the frames, poses, intrinsics,
+    ground-truth depth of the reference and the pipeline parameters."""
+    rng = np.random.default_rng(5000 + seed)
+    out = []
+    for k in range(count):
+        W, H = UNEVEN_RESOLUTIONS[int(rng.integers(len(UNEVEN_RESOLUTIONS)))]
+        S = int(rng.integers(2, 9))
+        D = int(rng.choice(UNEVEN_PLANES))
+        sc = trajectory_scene(100 * seed + k + 50, width=W, height=H, n_frames=S + 1,
+                              n_sources=S, planes=D, fx=0.612 * W)
+        ref = S // 2
+        srcs = [j for j in range(S + 1) if j != ref]
+        out.append({
+            "width": W, "height": H, "planes": D, "num_sources": S,
+            "K": sc["scene"].K, "ref": sc["images"][ref], "ref_pose": sc["poses"][ref],
+            "gt_depth": sc["gt_depths"][ref], "srcs": [sc["images"][j] for j in srcs],
+            "src_poses": [sc["poses"][j] for j in srcs],
+            "idep_min": sc["idep_min"], "idep_max": sc["idep_max"], "paths": sc["paths"],
+            "knobs": sc["knobs"],
+        })
+    return out

@@ -85,12 +85,20 @@ def test_depth_fuse_validation():
     t = (ctypes.c_float * 3)(0, 0, 0)
     st = C.FusionState(0x1000, 0x2000, 0x3000, 0x4000)
     cam = C.Camera(100, 100, 3, 2)
-    assert lib.pm_depth_fuse(8, 6, cam, r, t, FAKE, FAKE, NULL, st, st, NULL) == 1   # alias
+    assert lib.pm_depth_fuse(8, 6, cam, r, t, FAKE, FAKE, FAKE, NULL, NULL, st, st,
+                             NULL) == 1   # state alias
     st2 = C.FusionState(0x5000, 0x6000, 0x7000, 0x8000)
-    assert lib.pm_depth_fuse(8, 6, C.Camera(0, 100, 3, 2), r, t, FAKE, FAKE, NULL, st, st2,
-                             NULL) == 1
+    assert lib.pm_depth_fuse(8, 6, C.Camera(0, 100, 3, 2), r, t, FAKE, FAKE, FAKE, NULL, NULL,
+                             st, st2, NULL) == 1
     bad = (ctypes.c_float * 9)(1, 0, 0, 0, float("inf"), 0, 0, 0, 1)
-    assert lib.pm_depth_fuse(8, 6, cam, bad, t, FAKE, FAKE, NULL, st, st2, NULL) == 1
+    assert lib.pm_depth_fuse(8, 6, cam, bad, t, FAKE, FAKE, FAKE, NULL, NULL, st, st2,
+                             NULL) == 1
+    # the next step's z-buffer must be a second buffer; a null output is refused
+    assert lib.pm_depth_fuse(8, 6, cam, r, t, FAKE, FAKE, FAKE, FAKE, NULL, st, st2,
+                             NULL) == 1
+    assert lib.pm_depth_fuse(8, 6, cam, r, t, FAKE, NULL, FAKE, NULL, NULL, st, st2,
+                             NULL) == 1
+    assert lib.pm_zbuf_reset(NULL, 4, NULL) == 1
 
 
 def test_cpu_rules_fusion_validation():

@@ -9,7 +9,7 @@ import json
 import numpy as np
 import pytest
 
-from oracle.sequence import OracleSequence, homography as np_homography
+from oracle.sequence import homography as np_homography
 from tests import parity
 
 pytestmark = pytest.mark.gpu
@@ -75,13 +75,13 @@ def test_batch_equals_single_keyframe(uneven, cost_type, paths, storage):
     for rows in (2, 4, 8):
         for planes in ((4, 8) if rows == 8 else (8, 16)):
             for pref in (8, 16):
-                for bpc in (0, 4):
+                for bpc, split in ((0, 0), (4, 1), (0, 1), (4, 0)):
                     for o in batch.outputs:
                         for v in o.values():
                             v.fill_(7)
                     batch.tuning = dict(cost_tile_rows=rows, cost_stage_planes=planes,
                                         sgm_prefetch=pref, cost_blocks_per_cu=bpc,
-                                        sgm_blocks_per_cu=bpc)
+                                        sgm_blocks_per_cu=bpc, cost_plane_split=split)
                     batch.run()
                     torch.cuda.synchronize()
                     _assert_equal(batch, expect)
@@ -220,6 +220,8 @@ def test_batch_rejects_bad_input(uneven):
     with pytest.raises(C.PmError):
         b.tuning = dict(cost_tile_rows=8, cost_stage_planes=16)
     with pytest.raises(C.PmError):
+        b.tuning = dict(cost_plane_split=2)
+    with pytest.raises(C.PmError):
         b.launch()   # nothing captured
     with pytest.raises(C.PmError):   # the sweep form takes at most 128 planes
         pipeline.Batch([dict(uneven[0], planes=129)], "f16", KNOBS, 8, "sweep")
@@ -305,41 +307,3 @@ def test_batched_sequence_with_median_matches_plain(trajectory_scene, storage):
         for k in g.state:
             assert torch.equal(g.state[k], p.state[k]), (i, k)
     assert not torch.equal(g.dep, g.depth_of(n - 1))   # the median changed the depth
-
-
-@pytest.mark.parametrize("cost_type,storage", [("f16", "exact"), ("bf16", "exact"),
-                                                ("f16", "compact"), ("bf16", "compact")])
-def test_storage_precision_error_report(trajectory_scene, cost_type, storage):
-    """Cost volume stored in FP16 / BF16, SGM and fusion accumulated in FP32 (compact: the
-    per-path excess stored as 16-bit levels), FP64 homography setup: errors of the whole
-    batched pipeline against the FP64 oracle on the first 5 keyframes of the 752x480x64,
-    5-source, 8-path sequence."""
-    from probabilistic_mapping_amd.pipeline import BatchedSequence
-    s = trajectory_scene
-    n = 5
-    args = (s["images"][:n], s["poses"][:n], [[j for j in src if j < n] or [1]
-                                               for src in s["sources"][:n]],
-            s["scene"].K, s["planes"], s["idep_min"], s["idep_max"], s["knobs"], s["paths"])
-    g = BatchedSequence(*args, cost_type=cost_type, sgm_storage=storage)
-    o = OracleSequence(*args, precision="f64")
-    g.reset()
-    g.batch.run()
-    report = []
-    for i in range(n):
-        g.fusion_step(i)
-        o.run_keyframe(i)
-        agg = g.batch.outputs[i]["agg"].cpu().numpy()
-        st = {"wta_agree": float((parity.wta_index(agg) == parity.wta_index(o.agg)).mean()),
-              "depth": parity.depth_parity(g.depth_of(i).cpu().numpy(), o.depth)}
-        if i > 0:
-            st["state"] = parity.state_parity({k: v.cpu().numpy() for k, v in g.state.items()},
-                                              o.state)
-        report.append(st)
-        _report(f"{cost_type} {storage} kf{i}", st)
-    depth_p999 = max(r["depth"]["rel"]["p999"] for r in report)
-    if cost_type == "f16":
-        # the accuracy bar of the FP16-storage form: <= 1e-3 relative depth error
-        assert depth_p999 <= 1e-3
-        assert min(r["wta_agree"] for r in report) >= 0.995
-    else:
-        assert min(r["wta_agree"] for r in report) >= 0.98

@@ -13,7 +13,7 @@ import json
 import numpy as np
 import pytest
 
-from oracle.sequence import OracleSequence, homography as np_homography
+from oracle.sequence import homography as np_homography
 from tests import parity
 
 pytestmark = pytest.mark.gpu
@@ -215,33 +215,109 @@ def test_case3_cpu_rules_sequence():
         set_knobs(**old)
 
 
-@pytest.mark.parametrize("cost_type", ["f32"])
-def test_trajectory_sequence(trajectory_scene, cost_type):
-    """752x480, 64 planes, 5 sources, 8 paths: the first 5 keyframes (4 fusion steps)."""
+def test_small_scene_fusion_step_against_oracle(small_scene):
+    """The 320x240, 2-source, 32-plane, 4-path workload through the drop-in StereoMapper with
+    one Bayesian fusion step from a uniform prior, against the FP64 oracle.  With the
+    reference's semantics the first outputFusion call only resets (stereo_mapper.cpp:128-146),
+    the second initialises every measured pixel to alpha = beta = 10 (inlier prior 0.5),
+    mu = d, sigma^2 = 2500 (depth_fusion.cu:161-174), and the third applies the update
+    (:176-224); the keyframe is re-observed at the same pose each time."""
+    from probabilistic_mapping_amd.mapper import StereoMapper, set_knobs
+    import oracle
+    s = small_scene
+    K = s["scene"].K
+    D = s["planes"]
+    ds = (s["idep_max"] - s["idep_min"]) / (D - 1)
+    fx, fy, cx, cy = s["scene"].intrinsics
+    old = set_knobs(idep_min=s["idep_min"], sgm_paths=4, median=0, **s["knobs"])
+    got = []
+    try:
+        m = StereoMapper(320, 240, D)
+        m.initIntrinsic(K, 0.0, ds)
+        for k in range(3):
+            m.initReference(s["ref"])
+            for img, (R, T) in zip(s["srcs"], s["src_poses"]):
+                m.update(img, *s["ref_pose"], R, T)
+            m.outputFusion(*s["ref_pose"], fx, fy, cx, cy)
+            got.append({"fused": m.download("fuse_depth"), "dep": m.download("dep"),
+                        "table": m.download("table"),
+                        "state": {k2: m.download(k2) for k2 in ("alpha", "beta", "mu", "sig")}})
+    finally:
+        set_knobs(**old)
+    homs = [np_homography(K, *s["ref_pose"], R, T) for R, T in s["src_poses"]]
+    o_cost = oracle.calc_cost(s["ref"], s["srcs"], homs, s["idep_min"], ds, D, "f64")
+    o_agg = oracle.sgm(s["ref"], o_cost, s["knobs"], 4, "f64")
+    o_dep = oracle.filter_cost(o_agg, s["knobs"]["var_scale"], s["idep_min"], ds, "f64")
+    H, W = o_dep.shape
+    st = [{k: np.zeros((H, W)) for k in ("alpha", "beta", "mu", "sig")} for _ in range(2)]
+    r, t = np.eye(3, dtype=np.float32), np.zeros(3, np.float32)
+    fused, tables = [], []
+    for step in range(2):   # calls 2 and 3 (call 1 only resets)
+        d = o_dep.copy()
+        tables.append(oracle.depth_fuse(d, st[step], st[step ^ 1], r, t, (fx, fy, cx, cy),
+                                        "f64"))
+        fused.append((d, {k: v.copy() for k, v in st[step ^ 1].items()}))
+    dep_st = parity.depth_parity(got[1]["dep"], o_dep)
+    _report("small fusion: filtered depth", dep_st)
+    _assert_depth(dep_st)
+    for call, (od, ostate), otab in zip((1, 2), fused, tables):
+        g = got[call]
+        st_ = parity.state_parity(g["state"], ostate)
+        st_["fused"] = parity.depth_parity(g["fused"], od)
+        st_["table_mismatch"] = float((g["table"] != otab).mean())
+        _report(f"small fusion call {call + 1}", st_)
+        assert st_["init_mismatch"] <= 1e-3
+        assert st_["table_mismatch"] <= 1e-3
+        for k in ("mu", "sig", "inlier"):
+            assert st_[k]["p999"] <= 1e-4, k
+        assert st_["fused"]["category_mismatch"] <= 1e-3
+        assert st_["fused"]["rel"]["p999"] <= 1e-4
+    # the initialising call outputs the measurement; the updating call moves the inlier
+    # ratio away from the uniform prior
+    a, b = got[1]["state"]["alpha"], got[1]["state"]["beta"]
+    init = a >= 1e-5
+    assert np.all(a[init] == 10) and np.all(b[init] == 10)
+    a2, b2 = got[2]["state"]["alpha"], got[2]["state"]["beta"]
+    upd = (a2 >= 1e-5) & init
+    assert np.abs(a2[upd] / (a2[upd] + b2[upd]) - 0.5).mean() > 1e-3
+
+
+def test_mapper_trajectory_equals_sequence(trajectory_scene):
+    """The C++ StereoMapper (host images in: initReference, update per source, outputFusion)
+    over 6 keyframes of the rotating 752x480 trajectory gives the device-resident sequence's
+    depth, fused depth and state bit for bit; the mapper composes R_p2c = R^T R_prev,
+    t_p2c = R^T (t_prev - t) in C++ (stereo_mapper.cpp:155-156), the sequence in numpy."""
+    from probabilistic_mapping_amd.mapper import StereoMapper, set_knobs
     from probabilistic_mapping_amd.pipeline import KeyframeSequence
     s = trajectory_scene
-    n = 5
-    args = (s["images"], s["poses"], s["sources"], s["scene"].K, s["planes"], s["idep_min"],
-            s["idep_max"], s["knobs"], s["paths"])
-    gseq = KeyframeSequence(*args, cost_type=cost_type)
-    oseq = OracleSequence(*args, precision="f64")
-    gseq.reset()
-    for i in range(n):
-        gseq.run_keyframe(i)
-        oseq.run_keyframe(i)
-        g_agg = gseq.agg.cpu().numpy()
-        wta_agree = float((parity.wta_index(g_agg) == parity.wta_index(oseq.agg)).mean())
-        st = {"wta_agree": wta_agree,
-              "cost": parity.cost_parity(gseq.cost.float().cpu().numpy(), oseq.cost),
-              "depth": parity.depth_parity(gseq.dep.cpu().numpy(), oseq.depth)}
-        if i > 0:
-            gstate = {k: v.cpu().numpy() for k, v in gseq.state.items()}
-            st["state"] = parity.state_parity(gstate, oseq.state)
-            st["fused"] = parity.depth_parity(gseq.fused.cpu().numpy(), oseq.fused)
-        _report(f"trajectory kf{i}", st)
-        assert wta_agree >= 0.999
-        _assert_depth(st["depth"])
-        if i > 0:
-            assert st["state"]["init_mismatch"] <= 1e-3
-            for k in ("mu", "sig", "inlier"):
-                assert st["state"][k]["p999"] <= 1e-4, k
+    n = 6
+    K = s["scene"].K
+    D = s["planes"]
+    ds = (s["idep_max"] - s["idep_min"]) / (D - 1)
+    fx, fy, cx, cy = s["scene"].intrinsics
+    seq = KeyframeSequence(s["images"], s["poses"], s["sources"], K, D, s["idep_min"],
+                           s["idep_max"], s["knobs"], s["paths"], cost_type="f32", median=True)
+    seq.reset()
+    old = set_knobs(idep_min=s["idep_min"], sgm_paths=s["paths"], median=1, **s["knobs"])
+    try:
+        H, W = s["images"][0].shape
+        m = StereoMapper(W, H, D)
+        m.initIntrinsic(K, 0.0, ds)
+        for i in range(n):
+            R, T = (np.asarray(a, np.float64) for a in s["poses"][i])
+            m.initReference(s["images"][i])
+            for j in s["sources"][i]:
+                m.update(s["images"][j], R, T, *s["poses"][j])
+            res = m.outputFusion(R, T.reshape(3), fx, fy, cx, cy)
+            seq.run_keyframe(i)
+            torch.cuda.synchronize()
+            if i == 0:
+                assert not res.any()
+                continue
+            assert np.array_equal(res, seq.dep.cpu().numpy()), i
+            assert np.array_equal(m.download("fuse_depth"), seq.fused.cpu().numpy()), i
+            for k in ("alpha", "beta", "mu", "sig"):
+                assert np.array_equal(m.download(k), seq.state[k].cpu().numpy()), (i, k)
+        m.close()
+    finally:
+        set_knobs(**old)

@@ -92,3 +92,53 @@ def test_true_depth_plane_has_lowest_regular_terms(small_scene):
     # smooth texture; a sign or convention error would put the minimum far away instead
     assert (best == k_true).mean() > 0.75
     assert (np.abs(best - k_true) <= 1).mean() > 0.9
+
+
+def test_validity_margin_identity_closed_form():
+    """H = I, t = 0 puts the nine warps of pixel (u, v) at fixed points for every plane:
+    (u, v), (u, v -+ 1), (u -+ 1, v), (u -+ 1, v -+ 1), (u +- 1, v -+ 1) and the two garbled
+    triples (calc_cost.cu:50-51, 70-71) at (1 + 1/u, 1) and (1, u / (u + 1)).  The margin is
+    the smallest distance of any of them to the bounds [0, W-1] x [0, H-1], in absolute value;
+    column 0 has w = 0 in the garbled warps and no margin."""
+    W, H, D = 9, 7, 3
+    img = np.random.default_rng(2).integers(0, 256, (H, W), dtype=np.uint8)
+    hom = (np.eye(3, dtype=np.float32), np.zeros(3, np.float32))
+    cost, mg = oracle.calc_cost(img, [img], [hom], 0.2, 0.1, D, "f64", margin=True)
+    for v in range(H):
+        for u in range(W):
+            if u == 0:
+                assert np.all(cost[v, u] == -1)
+                continue
+            pts = [(u, v), (u, v - 1), (u, v + 1), (u - 1, v), (1 + 1 / u, 1.0),
+                   (u - 1, v - 1), (u + 1, v + 1), (u - 1, v + 1), (1.0, u / (u + 1))]
+            m = min(min(x, W - 1 - x, y, H - 1 - y) for x, y in pts)
+            np.testing.assert_allclose(mg[v, u], abs(m), rtol=1e-6, atol=1e-7)
+            assert np.all((cost[v, u] >= 0) == (m >= 0))
+
+
+def test_validity_margin_bounds_source_set_changes():
+    """Moving every warp by a tiny amount (the translation scaled by 1 + 1e-5, i.e. well under
+    2e-3 px here) can change an entry's validity, or its set of contributing sources (a jump of
+    the mean far beyond the smooth change, which is below 0.6 cost units for such a move),
+    only where the margin is below that move."""
+    from probabilistic_mapping_amd import synth
+    s = synth.trajectory_scene(2, width=320, height=200, n_frames=7, n_sources=6, planes=32,
+                               fx=200.0)
+    K = s["scene"].K
+    i = 3
+    srcs = s["sources"][i]
+    homs = [homography(K, *s["poses"][i], *s["poses"][j]) for j in srcs]
+    moved = [(h, (np.asarray(t, np.float64) * (1 + 1e-5))) for h, t in homs]
+    ds = (s["idep_max"] - s["idep_min"]) / 31
+    imgs = [s["images"][j] for j in srcs]
+    args = (s["idep_min"], ds, 32, "f64")
+    c0, mg = oracle.calc_cost(s["images"][i], imgs, homs, *args, margin=True)
+    c1 = oracle.calc_cost(s["images"][i], imgs, moved, *args)
+    # the largest warp move: |dU| = |dt1 idep| / w over every source, plane and pixel
+    flip = (c0 >= 0) != (c1 >= 0)
+    both = (c0 >= 0) & (c1 >= 0)
+    jump = np.zeros_like(flip)
+    jump[both] = np.abs(c0[both] - c1[both]) > 0.6
+    assert flip.sum() + jump.sum() >= 5          # the sample has changes to explain
+    assert np.all(mg[flip | jump] < 2e-3)
+    assert (mg < 2e-3).mean() < 1e-3

@@ -157,3 +157,54 @@ def test_oracle_sgm_adjoint_equals_autograd(tiny, paths):
     want = cost.grad.numpy()
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
     assert np.abs(want).max() > 0
+
+
+def test_real_valued_reference_in_the_cpp_oracle(tiny):
+    """The C++ oracle with a float64 reference image (used for intensity finite differences
+    at full size) equals its uint8 path on integer intensities, and its FP64 central
+    differences in one intensity equal the torch restatement's autograd gradient."""
+    s = tiny
+    K = s["K"]
+    homs = []
+    for R, t in s["rel_poses"]:
+        H, th = diff_ref.relative_homography(K, R, t, torch.zeros(6, dtype=F64))
+        homs.append((H.detach().numpy().astype(np.float32), th.detach().numpy().astype(np.float32)))
+    D = s["planes"]
+    ds = (s["idep_max"] - s["idep_min"]) / (D - 1)
+    c8 = oracle.calc_cost(s["ref"], s["srcs"], homs, s["idep_min"], ds, D)
+    cf = oracle.calc_cost(s["ref"].astype(np.float64), s["srcs"], homs, s["idep_min"], ds, D)
+    assert np.array_equal(c8, cf)
+    kn = dict(s["knobs"], var_scale=1.0)   # the readout needs no filter knob
+    a8 = oracle.sgm(s["ref"], c8, kn, 8)
+    af = oracle.sgm(s["ref"].astype(np.float64), cf, kn, 8)
+    assert np.array_equal(a8, af)
+    # FD through the C++ oracle against the restatement's gradient at the same float-rounded
+    # homographies, on the pixel of largest gradient away from the tau_so switch
+    ref = torch.tensor(s["ref"], dtype=F64, requires_grad=True)
+    _loss(s, torch.zeros((2, 6), dtype=F64), ref).backward()
+    g = ref.grad.numpy()
+    idep = s["idep"]
+
+    def cpp_loss(refd):
+        cost = oracle.calc_cost(refd, s["srcs"], homs, s["idep_min"], ds, D)
+        agg = oracle.sgm(refd, cost, kn, 8)
+        inv = torch.from_numpy((cost < 0).any(axis=2))
+        _, loss, _ = diff_ref.readout(torch.from_numpy(agg), inv, idep,
+                                      s["gt_depth"].astype(np.float64), s["temperature"],
+                                      s["cost_scale"])
+        return float(loss)
+
+    img = s["ref"].astype(np.int64)
+    tau = s["knobs"]["tau_so"]
+    Hh, Ww = img.shape
+    for i in np.argsort(-np.abs(g).ravel()):
+        y, x = divmod(int(i), Ww)
+        nb = img[max(y - 1, 0):y + 2, max(x - 1, 0):x + 2]
+        if not (np.abs(nb - img[y, x]) == tau).any():
+            break
+    h = 1e-4
+    rp, rm = s["ref"].astype(np.float64), s["ref"].astype(np.float64)
+    rp[y, x] += h
+    rm[y, x] -= h
+    fd = (cpp_loss(rp) - cpp_loss(rm)) / (2 * h)
+    assert abs(fd - g[y, x]) <= 1e-5 * abs(g[y, x])
