@@ -194,6 +194,20 @@ pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam, const floa
 /* Empties n z-buffer entries (sets every bit), for pm_depth_fuse's alternating buffers. */
 pm_status pm_zbuf_reset(uint64_t* zbuf, size_t n, pm_stream stream);
 
+/* pm_depth_fuse without a z-buffer argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The z-buffer is a library-owned device array of
+ * PM_FUSE_NOALLOC_MAX_PIXELS entries (16 MiB, held by each GPU the library runs on until the
+ * process ends); width * height above that limit is refused with PM_ERR_INVALID_ARGUMENT.
+ * The call allocates nothing: the array's address is looked up on the first call on each
+ * device, and each call empties the array with a fill kernel before using it.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (other streams, other threads)
+ * race; serialise them.  Other arguments and results as pm_depth_fuse with zbuf_next = NULL. */
+#define PM_FUSE_NOALLOC_MAX_PIXELS (1 << 21)
+pm_status pm_depth_fuse_noalloc(int32_t width, int32_t height, pm_camera cam,
+                                const float r_p2c[9], const float t_p2c[3], const float* depth,
+                                float* fused, int32_t* table, pm_fusion_state pre,
+                                pm_fusion_state cur, pm_stream stream);
 
 /* The fusion step of outputFusionCPU with CASE >= 3 (stereo_mapper.cpp:719-800): the
  * reference's CPU rules (depthFuseCPU, :246-587) on the GPU -- propagation gate 0.5, a 7x7

@@ -131,6 +131,8 @@ SIGNATURES = {
     "pm_depth_fuse": (_i32, [_i32, _i32, Camera, _fp, _fp, _vp, _vp, _vp, _vp, _vp,
                              FusionState, FusionState, _vp]),
     "pm_zbuf_reset": (_i32, [_vp, ctypes.c_size_t, _vp]),
+    "pm_depth_fuse_noalloc": (_i32, [_i32, _i32, Camera, _fp, _fp, _vp, _vp, _vp, FusionState,
+                                     FusionState, _vp]),
     "pm_fill_f32": (_i32, [_vp, ctypes.c_size_t, ctypes.c_float, _vp]),
     "pm_batch_create": (_i32, [ctypes.POINTER(Keyframe), _i32, _i32, SgmParams, ctypes.c_float,
                                _i32, ctypes.POINTER(_vp)]),

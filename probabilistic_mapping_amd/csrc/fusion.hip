@@ -22,6 +22,7 @@
 // from other threads.  So a caller that runs steps back to back gives each step a second
 // buffer, zbuf_next, which the transform kernel empties pixel by pixel for the next step (the
 // two buffers alternate); without one, the step empties its z-buffer first (one more launch).
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -226,6 +227,57 @@ extern "C" pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam,
   clear_error();
   return depth_fuse_impl("pm_depth_fuse", width, height, cam, r_p2c, t_p2c, depth, fused, zbuf,
                          zbuf_next, table, pre, cur, (hipStream_t)stream);
+}
+
+// ---- allocation-free variant ---------------------------------------------------------------
+// The z-buffer of pm_depth_fuse_noalloc: a file-scope device array of PM_FUSE_NOALLOC_MAX_PIXELS
+// entries (16 MiB), which every GPU the library runs on holds until the process ends.  In host
+// code `scratch` names a host-side stand-in, not device memory: its device address is taken
+// once per device with hipGetSymbolAddress and only that address is used.
+static __device__ __align__(256) uint64_t scratch[PM_FUSE_NOALLOC_MAX_PIXELS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+pm_status scratch_address(uint64_t** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(PM_ERR_DEVICE, std::string("pm_depth_fuse_noalloc: hipGetDevice: ") +
+                                   hipGetErrorString(e));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PM_ERR_DEVICE, std::string("pm_depth_fuse_noalloc: hipGetSymbolAddress: ") +
+                                     hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<uint64_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PM_OK;
+}
+}  // namespace
+
+extern "C" pm_status pm_depth_fuse_noalloc(int32_t width, int32_t height, pm_camera cam,
+                                           const float r_p2c[9], const float t_p2c[3],
+                                           const float* depth, float* fused, int32_t* table,
+                                           pm_fusion_state pre, pm_fusion_state cur,
+                                           pm_stream stream) {
+  clear_error();
+  if (width <= 0 || height <= 0 || (long)width * height > PM_FUSE_NOALLOC_MAX_PIXELS)
+    return fail(PM_ERR_INVALID_ARGUMENT,
+                "pm_depth_fuse_noalloc: width * height must be in 1.." +
+                    std::to_string(PM_FUSE_NOALLOC_MAX_PIXELS));
+  uint64_t* zbuf = nullptr;
+  const pm_status s = scratch_address(&zbuf);
+  if (s != PM_OK) return s;
+  // zbuf_next = nullptr: the step empties the scratch z-buffer first, with a fill kernel
+  return depth_fuse_impl("pm_depth_fuse_noalloc", width, height, cam, r_p2c, t_p2c, depth,
+                         fused, zbuf, nullptr, table, pre, cur, (hipStream_t)stream);
 }
 
 extern "C" pm_status pm_zbuf_reset(uint64_t* zbuf, size_t n, pm_stream stream) {
