@@ -49,6 +49,48 @@ __device__ __forceinline__ float opaque_one() {
   return one;
 }
 
+// Wave-uniform selects on the scalar unit.  Written as C++ ternaries these became per-lane
+// v_cndmask instructions on the (issue-bound) vector unit in every scan step; the scalar unit
+// issues beside it.  All operands are wave-uniform and live in scalar registers.
+//   penalties: (P1s, P2s) where bit `bit` of `smooth` is set (|dI| < tau_so), else
+//   (P1n, P2n), sgm.cu:78-82.
+__device__ __forceinline__ void scalar_penalties(uint64_t smooth, uint32_t bit, float P1s,
+                                                 float P1n, float P2s, float P2n, float& P1,
+                                                 float& P2) {
+  asm("s_bitcmp1_b64 %2, %3\n\t"
+      "s_cselect_b32 %0, %4, %5\n\t"
+      "s_cselect_b32 %1, %6, %7"
+      : "=&s"(P1), "=&s"(P2)
+      : "s"(smooth), "s"(bit), "s"(P1s), "s"(P1n), "s"(P2s), "s"(P2n)
+      : "scc");
+}
+//   cost weight: 0 where the ballot of negative costs is non-zero (an invalid pixel adds
+//   COST_INF = 0 in place of its cost), else `one`.
+__device__ __forceinline__ float scalar_cost_weight(uint64_t neg_ballot, float one) {
+  float w;
+  asm("s_cmp_lg_u64 %1, 0\n\t"
+      "s_cselect_b32 %0, 0, %2"
+      : "=&s"(w)
+      : "s"(neg_ballot), "s"(one)
+      : "scc");
+  return w;
+}
+
+// a + b on the scalar unit, opaque to the optimiser.  The scan advances its pixel offset by
+// one step's stride per step; written as plain adds, the compiler folds them into
+// base + i * stride for each of the unrolled block's slots, keeps those constants in scalar
+// registers and, out of registers, spills them to vector lanes that every step must read
+// back with a vector instruction.  Both operands are wave-uniform; readfirstlane states it
+// where the compiler cannot prove it (it is free on a value already in a scalar register).
+__device__ __forceinline__ uint32_t scalar_add(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm("s_add_u32 %0, %1, %2"
+      : "=s"(r)
+      : "s"(__builtin_amdgcn_readfirstlane(a)), "s"(__builtin_amdgcn_readfirstlane(b))
+      : "scc");
+  return r;
+}
+
 __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, int& y) {
   if (a.dx != 0) {
     const int xs = a.dx > 0 ? 0 : a.W - 1;
@@ -146,24 +188,27 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       return (uint64_t)__ballot(abs((int)r - (int)prev) < a.tau_i);
   };
   // one step k >= 1 from slot i of a block buffer
-  auto step = [&](int k, int i, const CV (&c)[B][NV], const float (&g)[B][NV]) {
-    const uint32_t p = (uint32_t)(p0 + k * pstep) * (uint32_t)D;
+  // element offset of the current step's pixel, advanced by one stride per step
+  uint32_t pk = (uint32_t)p0 * (uint32_t)D;
+  const uint32_t pkstep = (uint32_t)pstep * (uint32_t)D;
+  auto step = [&](int, int i, const CV (&c)[B][NV], const float (&g)[B][NV]) {
+    pk = scalar_add(pk, pkstep);
+    const uint32_t p = pk;
     bool neg = false;
 #pragma unroll
     for (int j = 0; j < NV; ++j) neg = neg || CS::negative(c[i][j]);
-    const bool invalid = __ballot(neg) != 0;
+    const uint64_t negb = __ballot(neg);
+    const bool invalid = negb != 0;
     static_assert(kCostInf == 0.0f, "an invalid pixel adds 0 in place of its cost");
-    // wave-uniform selects written on integer bits, which the compiler keeps scalar
-    const float w = __uint_as_float(invalid ? 0u : __float_as_uint(one));
+    const float w = scalar_cost_weight(negb, one);
     // m = min_d L(q, d) (sgm.cu:55-66)
     float lm = L[0];
 #pragma unroll
     for (int j = 1; j < NV; ++j) lm = umin(lm, L[j]);
     const float m = wave_min_nonneg(lm);
     // gradient-adaptive penalties (sgm.cu:78-82), scalar
-    const bool sm = (smooth >> i) & 1ull;
-    const float P1 = __uint_as_float(sm ? __float_as_uint(a.P1s) : __float_as_uint(a.P1n));
-    const float P2 = __uint_as_float(sm ? __float_as_uint(a.P2s) : __float_as_uint(a.P2n));
+    float P1, P2;
+    scalar_penalties(smooth, (uint32_t)i, a.P1s, a.P1n, a.P2s, a.P2n, P1, P2);
     // L(q, d0-1) and L(q, d0+NV) from the neighbouring lanes (0xFFFFFFFF where none)
     const uint32_t lo = shr1_max(L[NV - 1]);
     const uint32_t hi = shl1_max(L[0]);
