@@ -18,9 +18,11 @@ plain FP32 form as ``value`` and the FP16 batched form beside it, with their rat
 the reference node's processImg loop (host images in, host depth out), with and without the
 upload-once frame ring.
 
-Prints one JSON line.  Extra objects: "roofline" (the SGM pass, HBM-bound), "stages" (per-stage
-ms and roofline fractions) and "cpu_baseline" (the float/OpenMP oracle on a bounded sample of
-the same workload, on this host's cores).
+Prints one JSON line.  Extra objects: "roofline" (the SGM pass, HBM-bound: the bytes it streams
+over its event time, against 8 TB/s, with its counter-measured traffic), "stages" (per-stage ms;
+the VALU-bound cost stage as a fraction of the chip's VALU issue slots) and "cpu_baseline" (the
+float/OpenMP oracle on a bounded sample of the same workload, built and timed on this host's
+physical cores).
 """
 from __future__ import annotations
 
@@ -36,17 +38,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed)
 METRIC = "fused depth frames/s @752x480x64 planes, plain vs FP16-batched; % HBM peak"
 WORKLOAD = ("752x480 frames, 64 planes (idep 0.125-2.0), 5 sources, 8-path SGM, filter, "
             "3x3 median (unless --median 0), Gaussian x Beta fusion, 20-keyframe sequence "
             "per step")
 COST_BYTES = {"f32": 4, "f16": 2, "bf16": 2}
-# HBM bytes per keyframe of the SGM pass (paths launch + combine/filter launch) on this
-# workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scales measured on a 1 GiB device
-# copy), profiles/sgm_pass_752x480x64_s5_p8.md.  Keyed by (form, cost storage, SGM storage).
+# Counter-measured figures per keyframe of this workload, from the rocprofv3 --pmc runs in
+# COUNTER_PROFILE (keyed by form, cost storage, SGM storage):
+#   * SGM_PASS_TRAFFIC: HBM bytes of the SGM pass (paths launch + combine/filter launch),
+#     FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 in bytes (scales from a device copy of known size);
+COUNTER_PROFILE = "profiles/counters_752x480x64_s5_p8.md"
 SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2038e9,
-                    ("batched", "f16", "compact"): 1.1560e9}
+                    ("batched", "f16", "compact"): 1.1427e9}
+#   * COST_VALU: the cost launch's SQ_INSTS_VALU (wave64 instructions) and the fraction of the
+#     chip's VALU issue slots they fill over the launch's duration in the profiled run (at the
+#     2.4 GHz peak clock).
+COST_VALU = {("plain", "f32", "exact"): (154.8e6, 0.907),
+             ("batched", "f16", "compact"): (151.6e6, 0.948)}
 
 
 def parse():
@@ -117,36 +125,44 @@ def attainable_copy(torch, mib: int) -> dict | None:
     return {"bytes": 2 * n, "ms": ms, "gbs": 2 * n / (ms * 1e-3) / 1e9}
 
 
-def sgm_pass_bytes(seq, cost_type: str) -> int:
-    """Algorithmic bytes of the SGM pass with its filter: the reference dataflow (each path
-    streams the cost and read-modify-writes the FP32 aggregate, P V (b_c + 8)) plus the filter
-    (V 4 read, N 4 written)."""
+def reference_dataflow_bytes(seq, cost_type: str) -> int:
+    """The survey's figure for the SGM pass with its filter: the reference's dataflow (each
+    path streams the cost and read-modify-writes the FP32 aggregate, P V (b_c + 8),
+    sgm.cu:52,101-108) plus the filter (V 4 read, N 4 written).  The passes here move fewer
+    bytes than that, so its rate is an effective figure, not a fraction of peak."""
     N = seq.H * seq.W
     V = N * seq.D
     return seq.paths * V * (COST_BYTES[cost_type] + 8) + V * 4 + N * 4
 
 
-def stage_report(seq, s, cost_type, ms_cost, ms_sgm, ms_fusion, storage):
+def stage_report(seq, s, cost_type, ms_cost, ms_sgm, ms_fusion, storage, form):
     H, W, D, P = seq.H, seq.W, seq.D, seq.paths
     S = len(s["sources"][0])
     N, V = H * W, H * W * D
     bc = COST_BYTES[cost_type]
-    sgm_bytes = sgm_pass_bytes(seq, cost_type)
+    design = sgm_pass_design_bytes(W, H, D, P, bc, storage)
+    ref_bytes = reference_dataflow_bytes(seq, cost_type)
     # compulsory bytes of the pass: the cost read once, the aggregate written once, the depth
     compulsory = V * (bc + 4) + N * 4
-    cost_flop = V * S * 9 * 20                          # ~20 FLOP per warp term
+    key = (form, cost_type, storage)
+    valu, valu_frac = COST_VALU.get(key, (None, None))
+    traffic = SGM_PASS_TRAFFIC.get(key)
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
     return {
-        "cost": {"ms": ms_cost, "bound": "valu",
-                 "achieved_tflops": cost_flop / (ms_cost * 1e-3) / 1e12,
-                 "peak_tflops": FP32_VALU_PEAK_TFLOPS,
-                 "hbm_gbs": (V * bc + (S + 1) * N) / (ms_cost * 1e-3) / 1e9},
+        "cost": {"ms": ms_cost, "bound": "valu", "valu_insts": valu,
+                 "valu_issue_frac": valu_frac, "valu_source": COUNTER_PROFILE,
+                 "hbm_gbs": gbs(V * bc + (S + 1) * N, ms_cost)},
         "sgm": {"ms": ms_sgm, "bound": "hbm", "storage": storage,
-                "achieved_gbs": sgm_bytes / (ms_sgm * 1e-3) / 1e9,
-                "frac": sgm_bytes / (ms_sgm * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "includes": "filter (WTA + subpixel)",
-                "compulsory_gbs": compulsory / (ms_sgm * 1e-3) / 1e9},
+                "design_bytes": design, "achieved_gbs": gbs(design, ms_sgm),
+                "frac": gbs(design, ms_sgm) / HBM_PEAK_GBS,
+                "traffic": traffic,
+                "traffic_frac": gbs(traffic, ms_sgm) / HBM_PEAK_GBS if traffic else None,
+                "reference_dataflow_bytes": ref_bytes,
+                "reference_dataflow_effective_gbs": gbs(ref_bytes, ms_sgm),
+                "compulsory_gbs": gbs(compulsory, ms_sgm)},
         "fusion": {"ms": ms_fusion, "bound": "hbm",
-                   "achieved_gbs": 104 * N / (ms_fusion * 1e-3) / 1e9},
+                   "achieved_gbs": gbs(104 * N, ms_fusion)},
     }
 
 
@@ -183,13 +199,13 @@ def measure_plain(args, torch, s, cost_type, storage):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
-    stages = stage_report(seq, s, cost_type, ms["cost"], ms["sgm"], ms["fusion"], storage)
+    stages = stage_report(seq, s, cost_type, ms["cost"], ms["sgm"], ms["fusion"], storage,
+                          "plain")
     stages["median"] = {"ms": ms["median"], "enabled": bool(args.median)}
     return {"value": args.steps * nkf / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
-            "stages": stages, "sgm_bytes": sgm_pass_bytes(seq, cost_type),
+            "stages": stages,
             "form": "plain", "cost_storage": cost_type, "sgm_storage": storage,
-            "keyframes_per_step": nkf,
-            "traffic": SGM_PASS_TRAFFIC.get(("plain", cost_type, storage))}
+            "keyframes_per_step": nkf}
 
 
 def measure_batched(args, torch, s, cost_type, storage):
@@ -254,16 +270,15 @@ def measure_batched(args, torch, s, cost_type, storage):
     inflight = measure_inflight(args, torch, s, seq, cost_type, storage, tunings) \
         if args.inflight > 1 else None
     # the SGM pass is stage 1 (paths) + stage 2 (filter; compact: combine + filter)
-    stages = stage_report(seq, s, cost_type, ms[0], ms[1] + ms[2], ms[3], storage)
+    stages = stage_report(seq, s, cost_type, ms[0], ms[1] + ms[2], ms[3], storage, "batched")
     stages["sgm"]["paths_ms"] = float(ms[1])
     stages["sgm"]["filter_ms"] = float(ms[2])
     return {"value": args.steps * nkf / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
-            "stages": stages, "sgm_bytes": sgm_pass_bytes(seq, cost_type),
+            "stages": stages,
             "form": f"batched (20 keyframes per step, {len(groups)} cache-sized groups of "
                     f"{len(groups[0].outputs)}, one hipGraph replay each)",
             "cost_storage": cost_type,
             "sgm_storage": storage, "keyframes_per_step": nkf, "tuning": tuning,
-            "traffic": SGM_PASS_TRAFFIC.get(("batched", cost_type, storage)),
             "inflight": inflight}
 
 
@@ -315,6 +330,17 @@ def measure_inflight(args, torch, s, first, cost_type, storage, tunings):
             "same_fused_result": bool(same)}
 
 
+def roofline(sg: dict, kernel: str) -> dict:
+    """The SGM pass against HBM peak: achieved = the bytes this pass streams (each launch's
+    reads and writes with no cache reuse, sgm_pass_design_bytes) over the event-timed pass;
+    traffic = the counter-measured HBM bytes of the same pass (COUNTER_PROFILE)."""
+    return {"bound": "hbm", "kernel": kernel, "achieved": sg["achieved_gbs"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sg["frac"],
+            "algorithmic_bytes": sg["design_bytes"], "traffic": sg["traffic"],
+            "traffic_source": COUNTER_PROFILE,
+            "reference_dataflow_effective_gbs": sg["reference_dataflow_effective_gbs"]}
+
+
 def report(args, main: dict, extra: dict | None, copy, base) -> dict:
     sg = main["stages"]["sgm"]
     out = {
@@ -333,11 +359,7 @@ def report(args, main: dict, extra: dict | None, copy, base) -> dict:
         "config": {"workload": WORKLOAD, "form": main["form"],
                    "cost_storage": main["cost_storage"], "sgm_storage": main["sgm_storage"],
                    "median": bool(args.median), "keyframes_per_step": main["keyframes_per_step"]},
-        "roofline": {"bound": "hbm",
-                     "kernel": "sgm pass (all paths in one launch + combine/filter)",
-                     "achieved": sg["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": sg["frac"], "algorithmic_bytes": main["sgm_bytes"],
-                     "traffic": main["traffic"]},
+        "roofline": roofline(sg, "sgm pass (all paths in one launch + combine/filter)"),
         "stages": main["stages"],
         "attainable_copy": copy,
         "cpu_baseline": base,
@@ -350,11 +372,8 @@ def report(args, main: dict, extra: dict | None, copy, base) -> dict:
         out["fp16_batched"] = {k: extra[k] for k in ("value", "ms_per_step", "form",
                                                      "cost_storage", "sgm_storage", "stages",
                                                      "tuning", "inflight")}
-        out["fp16_batched"]["roofline"] = {
-            "bound": "hbm", "kernel": "sgm pass (compact paths launch + combine/filter)",
-            "achieved": extra["stages"]["sgm"]["achieved_gbs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": extra["stages"]["sgm"]["frac"],
-            "algorithmic_bytes": extra["sgm_bytes"], "traffic": extra["traffic"]}
+        out["fp16_batched"]["roofline"] = roofline(
+            extra["stages"]["sgm"], "sgm pass (compact paths launch + combine/filter)")
         out["fp16_batched_vs_plain"] = extra["value"] / main["value"]
         if extra.get("inflight"):
             out["fp16_batched_inflight_vs_plain"] = extra["inflight"]["value"] / main["value"]
@@ -613,13 +632,31 @@ def bench_uneven(args, torch):
     print(json.dumps(out), flush=True)
 
 
+def write_mapper_workload(path, s, steps, warmup, median, ring):
+    """The workload file pm_bench_mapper reads (layout in csrc/pm_bench_mapper.cpp)."""
+    import struct
+    H, W = s["images"][0].shape
+    kn = s["knobs"]
+    with open(path, "wb") as f:
+        f.write(struct.pack("<10i", W, H, s["planes"], len(s["images"]), len(s["sources"][0]),
+                            steps, warmup, int(median), s["paths"], ring))
+        f.write(np.asarray(s["scene"].K, "<f8").tobytes())
+        f.write(np.array([s["idep_min"], s["idep_max"], kn["pi1"], kn["pi2"], kn["tau_so"],
+                          kn["sgm_q1"], kn["sgm_q2"], kn["var_scale"]], "<f8").tobytes())
+        for R, T in s["poses"]:
+            f.write(np.concatenate([np.asarray(R, "<f8").reshape(9),
+                                    np.asarray(T, "<f8").reshape(3)]).tobytes())
+        f.write(np.asarray(s["sources"], "<i4").tobytes())
+        for im in s["images"]:
+            f.write(np.ascontiguousarray(im, np.uint8).tobytes())
+
+
 def bench_mapper(args, torch):
     """The drop-in path timed from C++: the 20-keyframe workload through pm::StereoMapper
     (initReference, update per source, outputFusion returning the host depth), run by the
     pm_bench_mapper executable in a child process.  Two forms: host images at every use
     (pinned staging, as the reference uploads them) and the upload-once frame ring.  The last
     step's fused and filtered depth must equal the device-resident sequence's bit for bit."""
-    import struct
     import subprocess
     import tempfile
     from probabilistic_mapping_amd import build as pm_build, pipeline as pl, synth
@@ -643,19 +680,7 @@ def bench_mapper(args, torch):
     with tempfile.TemporaryDirectory() as tmp:
         for name, ring in (("host_images", 0), ("frame_ring", 1)):
             path = os.path.join(tmp, f"workload_{ring}.bin")
-            with open(path, "wb") as f:
-                f.write(struct.pack("<10i", W, H, D, F, S, args.steps, args.warmup,
-                                    int(bool(args.median)), P, ring))
-                f.write(np.asarray(s["scene"].K, "<f8").tobytes())
-                f.write(np.array([s["idep_min"], s["idep_max"], kn["pi1"], kn["pi2"],
-                                  kn["tau_so"], kn["sgm_q1"], kn["sgm_q2"], kn["var_scale"]],
-                                 "<f8").tobytes())
-                for R, T in s["poses"]:
-                    f.write(np.concatenate([np.asarray(R, "<f8").reshape(9),
-                                            np.asarray(T, "<f8").reshape(3)]).tobytes())
-                f.write(np.asarray(s["sources"], "<i4").tobytes())
-                for im in s["images"]:
-                    f.write(np.ascontiguousarray(im, np.uint8).tobytes())
+            write_mapper_workload(path, s, args.steps, args.warmup, bool(args.median), ring)
             out = subprocess.run([exe, path, tmp], capture_output=True, text=True, check=True)
             r = json.loads(out.stdout.strip().splitlines()[-1])
             fused = np.fromfile(os.path.join(tmp, "fused.f32"), np.float32).reshape(H, W)

@@ -7,7 +7,8 @@
 //     pass when an output method runs (the running mean of the reference is order-free);
 //   * initReference() and update() copy the host image into pinned staging and return
 //     without synchronising (the caller may reuse its buffer at once); output methods
-//     synchronise before they hand back a host result;
+//     synchronise before they hand back a host result, which, when the mapper creates it, lives
+//     in page-locked memory;
 //   * raw_cost may be stored as fp16 / bf16 (set_cost_type);
 //   * no GUI calls: outputFusion's imshow (stereo_mapper.cpp:232-235) is dropped;
 //   * errors raise pm::Error (a std::runtime_error with a pm_status) instead of exiting.
@@ -102,6 +103,7 @@ class StereoMapper {
   void fuse_cpu_rules(Mat& result, Matrix3d R, Vector3d t, float fx, float fy, float cx,
                       float cy);
   void stage_upload(const Mat& img, uint8_t* dst, size_t slot, const char* who);
+  void download_result(const DeviceMat& src, Mat& result);
 
   int W_, H_, D_;
   pm_cost_type cost_type_ = PM_COST_F32;
@@ -126,6 +128,7 @@ class StereoMapper {
   // image into a slot and return without waiting for the device
   std::vector<uint8_t*> pinned_;
   std::vector<void*> pinned_ev_;
+  Mat out_staging_;                        // page-locked staging of results for callers' Mats
 };
 
 }  // namespace pm

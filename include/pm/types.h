@@ -42,9 +42,24 @@ class Mat {
   void create(int r, int c, int t) {
     if (data && rows == r && cols == c && type == t) return;
     rows = r; cols = c; type = t; step = c * elem_size(t);
-    store_ = std::make_shared<std::vector<uint8_t>>(step * r);
-    data = store_->data();
+    auto v = std::make_shared<std::vector<uint8_t>>(step * r);
+    data = v->data();
+    store_ = std::shared_ptr<uint8_t>(v, data);
+    alloc_tag_ = nullptr;
   }
+  // create with storage from alloc (freed by release when the last Mat sharing it goes);
+  // tag identifies the allocator (the mapper uses it for page-locked result matrices)
+  void create_with(int r, int c, int t, uint8_t* (*alloc)(size_t), void (*release)(uint8_t*),
+                   const void* tag) {
+    if (data && rows == r && cols == c && type == t && alloc_tag_ == tag) return;
+    const size_t st = c * elem_size(t);
+    uint8_t* p = alloc(st * r);   // may throw: the Mat is unchanged then
+    rows = r; cols = c; type = t; step = st;
+    data = p;
+    store_ = std::shared_ptr<uint8_t>(data, release);
+    alloc_tag_ = tag;
+  }
+  const void* alloc_tag() const { return alloc_tag_; }
   Mat clone() const {
     Mat m(rows, cols, type);
     if (data) std::memcpy(m.data, data, step * rows);
@@ -62,7 +77,8 @@ class Mat {
   bool empty() const { return data == nullptr || rows == 0 || cols == 0; }
 
  private:
-  std::shared_ptr<std::vector<uint8_t>> store_;
+  std::shared_ptr<uint8_t> store_;
+  const void* alloc_tag_ = nullptr;
 };
 
 // Device buffer owned by its creator (StereoMapper).  Dense rows: step = cols * elem size.

@@ -53,6 +53,17 @@ void check_image(const Mat& img, int rows, int cols, const char* who) {
     throw Error(PM_ERR_INVALID_ARGUMENT, std::string(who) + ": image must be 8-bit grayscale");
 }
 
+// Page-locked host storage for result matrices: device-to-host copies into it are plain DMA.
+uint8_t* pinned_alloc(size_t n) {
+  void* p = nullptr;
+  hip_check(hipHostMalloc(&p, n ? n : 1, hipHostMallocDefault), "hipHostMalloc");
+  return static_cast<uint8_t*>(p);
+}
+void pinned_release(uint8_t* p) {
+  if (p) (void)hipHostFree(p);
+}
+const char kPinnedTag = 0;   // &kPinnedTag marks a Mat whose storage is pinned_alloc's
+
 void mat_to_array(const Mat& m, int rows, int cols, double* out, const char* what) {
   if (m.rows != rows || m.cols != cols || m.type != CV_64F)
     throw Error(PM_ERR_INVALID_ARGUMENT, std::string(what) + " must be a " +
@@ -184,6 +195,33 @@ void StereoMapper::set_cost_type(pm_cost_type t) {
   if (t != PM_COST_F32 && t != PM_COST_F16 && t != PM_COST_BF16)
     throw Error(PM_ERR_INVALID_ARGUMENT, "set_cost_type: unknown type");
   cost_type_ = t;
+}
+
+// A device result handed back to the caller, as cv::cuda::GpuMat::download does (result is
+// (re)created only when its shape or type differs).  A result the mapper created lives in
+// page-locked memory and receives the copy directly; a caller's matrix of the right shape
+// receives it through a page-locked staging buffer.
+void StereoMapper::download_result(const DeviceMat& src, Mat& result) {
+  hipStream_t st = (hipStream_t)stream_;
+  const bool fits = result.data && result.rows == src.rows && result.cols == src.cols &&
+                    result.type == src.type;
+  if (!fits)
+    result.create_with(src.rows, src.cols, src.type, pinned_alloc, pinned_release, &kPinnedTag);
+  if (result.alloc_tag() == &kPinnedTag && result.step == src.step) {
+    hip_check(hipMemcpyAsync(result.data, src.data, src.bytes(), hipMemcpyDeviceToHost, st),
+              "download");
+    synchronize();
+    return;
+  }
+  if (out_staging_.rows != src.rows || out_staging_.cols != src.cols ||
+      out_staging_.type != src.type)
+    out_staging_.create_with(src.rows, src.cols, src.type, pinned_alloc, pinned_release,
+                             &kPinnedTag);
+  hip_check(hipMemcpyAsync(out_staging_.data, src.data, src.bytes(), hipMemcpyDeviceToHost, st),
+            "download");
+  synchronize();
+  for (int y = 0; y < src.rows; ++y)
+    std::memcpy(result.data + y * result.step, out_staging_.data + y * src.step, src.step);
 }
 
 void StereoMapper::synchronize() const {
@@ -353,8 +391,7 @@ void StereoMapper::compute_depth() { run_sgm_filter(); }
 // stereo_mapper.cpp:93-109
 void StereoMapper::output(Mat& result) {
   run_sgm_filter();
-  synchronize();
-  dep.download(result);
+  download_result(dep, result);
 }
 
 // stereo_mapper.cpp:111-244
@@ -364,7 +401,8 @@ void StereoMapper::outputFusion(Mat& result, Matrix3d Rw, Vector3d tw, float fx,
   const size_t N = (size_t)W_ * H_;
   if (firstDepth) {   // :128-146
     firstDepth = false;
-    result.create(H_, W_, CV_32F);
+    if (!(result.data && result.rows == H_ && result.cols == W_ && result.type == CV_32F))
+      result.create_with(H_, W_, CV_32F, pinned_alloc, pinned_release, &kPinnedTag);
     result.setTo<float>(0.0f);
     R_b_2_w = Rw;
     t_b_2_w = tw;
@@ -397,8 +435,7 @@ void StereoMapper::outputFusion(Mat& result, Matrix3d Rw, Vector3d tw, float fx,
                       zb + N * (zpar_ ^ 1), reinterpret_cast<int32_t*>(propogate_table.data),
                       state(pre), state(cur), stream_));
   zpar_ ^= 1;
-  synchronize();
-  dep.download(result);   // :229-230 -- the filtered, not the fused, depth
+  download_result(dep, result);   // :229-230 -- the filtered, not the fused, depth
   R_b_2_w = Rw;            // :237-241
   t_b_2_w = tw;
   pre_index = cur_index;
@@ -419,8 +456,7 @@ void StereoMapper::wta(int case_mode, Mat& result) {
     check(pm_wta(sh, sgm_cost.data, PM_COST_F32, IDEP_MIN, DEP_SAMPLE,
                  reinterpret_cast<float*>(wta_tmp_.data), stream_));
   }
-  synchronize();
-  wta_tmp_.download(result);
+  download_result(wta_tmp_, result);
 }
 
 // stereo_mapper.cpp:719-800, the CASE >= 3 branch of outputFusionCPU: the CPU rules of
@@ -448,7 +484,8 @@ void StereoMapper::fuse_cpu_rules(Mat& result, Matrix3d Rw, Vector3d tw, float f
   }
   if (firstDepth) {   // :725-741
     firstDepth = false;
-    result.create(H_, W_, CV_32F);
+    if (!(result.data && result.rows == H_ && result.cols == W_ && result.type == CV_32F))
+      result.create_with(H_, W_, CV_32F, pinned_alloc, pinned_release, &kPinnedTag);
     result.setTo<float>(0.0f);
     R_b_2_w = Rw;
     t_b_2_w = tw;
@@ -475,8 +512,7 @@ void StereoMapper::fuse_cpu_rules(Mat& result, Matrix3d Rw, Vector3d tw, float f
                                 reinterpret_cast<int32_t*>(cpu_table.data), cpu_work_.data,
                                 cpu_work_.bytes(), state(pre_index), state(pre_index ^ 1),
                                 stream_));
-  synchronize();
-  cpu_result_.download(result);
+  download_result(cpu_result_, result);
   R_b_2_w = Rw;           // :794-798
   t_b_2_w = tw;
   pre_index = cur_index;

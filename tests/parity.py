@@ -73,14 +73,16 @@ def depth_parity(gpu, orc) -> dict:
 
 def state_parity(gs: dict, os_: dict, mask=None) -> dict:
     """Filter state: pixels initialised on both sides (alpha >= 1e-5), and inside ``mask`` if
-    given, must match; compares mean, variance and the inlier ratio alpha / (alpha + beta)."""
+    given, must match; compares mean, variance and the inlier ratio alpha / (alpha + beta).
+    ``init_mismatch`` is the fraction of the pixels inside ``mask`` initialised on one side
+    only."""
     ga, oa = gs["alpha"], os_["alpha"]
     gi, oi = ga >= 1e-5, oa >= 1e-5
     both = gi & oi
-    if mask is not None:
-        both = both & mask
-    out = {"init_mismatch": float((gi != oi).mean()), "initialised": float(oi.mean()),
-           "compared": float(both.mean())}
+    sel = np.ones_like(gi) if mask is None else np.asarray(mask, bool)
+    both = both & sel
+    out = {"init_mismatch": float((gi != oi)[sel].mean()) if sel.any() else 0.0,
+           "initialised": float(oi.mean()), "compared": float(both.mean())}
     for k in ("mu", "sig"):
         out[k] = summary(rel_err(gs[k][both], os_[k][both]))
     with np.errstate(invalid="ignore", divide="ignore"):
