@@ -48,13 +48,13 @@ COST_BYTES = {"f32": 4, "f16": 2, "bf16": 2}
 #   * SGM_PASS_TRAFFIC: HBM bytes of the SGM pass (paths launch + combine/filter launch),
 #     FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 in bytes (scales from a device copy of known size);
 COUNTER_PROFILE = "profiles/counters_752x480x64_s5_p8.md"
-SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2038e9,
+SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2034e9,
                     ("batched", "f16", "compact"): 1.1427e9}
 #   * COST_VALU: the cost launch's SQ_INSTS_VALU (wave64 instructions) and the fraction of the
 #     chip's VALU issue slots they fill over the launch's duration in the profiled run (at the
 #     2.4 GHz peak clock).
-COST_VALU = {("plain", "f32", "exact"): (154.8e6, 0.907),
-             ("batched", "f16", "compact"): (151.6e6, 0.948)}
+COST_VALU = {("plain", "f32", "exact"): (154.75e6, 0.902),
+             ("batched", "f16", "compact"): (151.62e6, 0.944)}
 
 
 def parse():
