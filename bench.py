@@ -14,9 +14,12 @@ runs the keyframes' cost, SGM and filter as autotuned, hipGraph-captured batches
 (grouped so that a group's cost volumes stay in the 256 MB Infinity Cache: 4 groups of 5 here),
 then the fusion steps (FP16 cost, compact SGM storage by default).  The default, "both", reports the
 plain FP32 form as ``value`` and the FP16 batched form beside it, with their ratio.
-``--form diff`` times the differentiable readout's forward + backward; ``--form driver`` times
-the reference node's processImg loop (host images in, host depth out), with and without the
-upload-once frame ring.
+``--form uneven`` times 16 independent keyframes of mixed size (320x240-1280x720, 2-8 sources,
+32-128 planes) as one FP16 batch, beside the same keyframes one at a time; ``--form mapper``
+times the 20-keyframe sequence through the C++ StereoMapper (initReference, update per source,
+outputFusion; host images in, host depth out) with a C++ driver; ``--form diff`` times the
+differentiable readout's forward + backward; ``--form driver`` times the reference node's
+processImg loop, with and without the upload-once frame ring.
 
 Prints one JSON line.  Extra objects: "roofline" (the SGM pass, HBM-bound: the bytes it streams
 over its event time, against 8 TB/s, with its counter-measured traffic), "stages" (per-stage ms;

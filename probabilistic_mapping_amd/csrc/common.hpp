@@ -160,6 +160,9 @@ __device__ __forceinline__ gptr<T> to_global(T* p) {
   return (gptr<T>)p;
 }
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // A device array read and written through a raw buffer resource: the base and size live in
 // scalar registers, an access is a 32-bit per-lane byte offset plus a 32-bit wave-uniform one
 // (no per-lane 64-bit address arithmetic), and the hardware range check turns an access past
@@ -187,6 +190,19 @@ struct Buf {
   }
   __device__ __forceinline__ void put_u16_nt(uint32_t v, uint32_t voff, uint32_t soff) const {
     __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)voff, (int)soff, 2);
+  }
+  // 8- and 16-byte stores (voff + soff a multiple of the width); AUX 2 = non-temporal
+  template <int AUX>
+  __device__ __forceinline__ void put_b64(u32x2 v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)voff, (int)soff, AUX);
+  }
+  template <int AUX>
+  __device__ __forceinline__ void put_b128(u32x4 v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)voff, (int)soff, AUX);
+  }
+  template <int AUX>
+  __device__ __forceinline__ void put_b32(uint32_t v, uint32_t voff, uint32_t soff) const {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)voff, (int)soff, AUX);
   }
 };
 // A cost element through a Buf (byte offsets).

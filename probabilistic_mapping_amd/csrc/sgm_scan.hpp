@@ -91,6 +91,57 @@ __device__ __forceinline__ uint32_t scalar_add(uint32_t a, uint32_t b) {
   return r;
 }
 
+// Stores of one step's NV planes per lane (plane off[j] of the pixel at element offset soff /
+// element size), with cache policy AUX (2 = non-temporal).  PACK: the planes are contiguous
+// and the pixel's first element is aligned to NV elements, so a lane writes them with one
+// 4/8/16-byte store and the wave's stores cover one contiguous span; written one element per
+// instruction, each store instruction covers every NV-th element and the memory system moves
+// whole granules for it (measured: 1.45-1.9x the bytes of the volume with two planes per
+// lane).  Lanes past D (act false) store nothing; with PACK all NV planes of a lane are
+// active or none is.
+template <int NV, int AUX>
+__device__ __forceinline__ void store_planes_f32(const Buf& b, const float (&v)[NV],
+                                                 const bool (&act)[NV], bool pack,
+                                                 const int (&off)[NV], uint32_t soff) {
+  if (NV == 2 && pack) {
+    if (act[0]) {
+      const u32x2 x = {__float_as_uint(v[0]), __float_as_uint(v[NV > 1 ? 1 : 0])};
+      b.put_b64<AUX>(x, (uint32_t)off[0] * 4u, soff);
+    }
+  } else if (NV == 4 && pack) {
+    if (act[0]) {
+      const u32x4 x = {__float_as_uint(v[0]), __float_as_uint(v[NV > 1 ? 1 : 0]),
+                       __float_as_uint(v[NV > 2 ? 2 : 0]), __float_as_uint(v[NV > 3 ? 3 : 0])};
+      b.put_b128<AUX>(x, (uint32_t)off[0] * 4u, soff);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      if (act[j]) b.put_b32<AUX>(__float_as_uint(v[j]), (uint32_t)off[j] * 4u, soff);
+  }
+}
+template <int NV, int AUX>
+__device__ __forceinline__ void store_planes_u16(const Buf& b, const uint32_t (&v)[NV],
+                                                 const bool (&act)[NV], bool pack,
+                                                 const int (&off)[NV], uint32_t soff) {
+  if (NV == 2 && pack) {
+    if (act[0]) b.put_b32<AUX>(v[0] | (v[NV > 1 ? 1 : 0] << 16), (uint32_t)off[0] * 2u, soff);
+  } else if (NV == 4 && pack) {
+    if (act[0]) {
+      const u32x2 x = {v[0] | (v[NV > 1 ? 1 : 0] << 16),
+                       v[NV > 2 ? 2 : 0] | (v[NV > 3 ? 3 : 0] << 16)};
+      b.put_b64<AUX>(x, (uint32_t)off[0] * 2u, soff);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+      if (act[j]) {
+        if (AUX == 2) b.put_u16_nt(v[j], (uint32_t)off[j] * 2u, soff);
+        else b.put_u16(v[j], (uint32_t)off[j] * 2u, soff);
+      }
+  }
+}
+
 __device__ __forceinline__ void scanline_start(const SgmArgs& a, int i, int& x, int& y) {
   if (a.dx != 0) {
     const int xs = a.dx > 0 ? 0 : a.W - 1;
@@ -148,6 +199,9 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
   }
   // byte offsets of the lane's planes in a volume of elt-byte elements
   auto lb = [&](int j, uint32_t elt) { return (uint32_t)off[j] * elt; };
+  // a lane's NV planes are contiguous; with D a multiple of NV each pixel's planes start on a
+  // multiple of NV elements, so they go out as one aligned wide store
+  const bool pack = NV > 1 && D % NV == 0;
   const float INF = __uint_as_float(0x7F800000u);
 
   // two register buffers of B steps each, used ping-pong: while one block computes, the
@@ -213,7 +267,8 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
     const uint32_t lo = shr1_max(L[NV - 1]);
     const uint32_t hi = shl1_max(L[0]);
     const float mP2 = m + P2;
-    float Ln[NV];
+    float Ln[NV], out[NV], lo_[NV];
+    uint32_t q[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
       const uint32_t left = j > 0 ? __float_as_uint(L[j - 1]) : lo;
@@ -221,17 +276,15 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       // min(L(q,d), L(q,d-1)+P1, L(q,d+1)+P1, m+P2) (sgm.cu:85-93)
       const float val = umin(umin(L[j], mP2), __uint_as_float(min(left, right)) + P1);
       Ln[j] = act[j] ? CS::add(c[i][j], val, w) - m : INF;   // sgm.cu:98
-      const float out = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);
-      if (AGG && act[j]) bagg.put_f32(out, lb(j, 4), p * 4u);   // sgm.cu:101-108
-      if (LOUT && act[j])
-        blout.put_f32_nt((LMASK && invalid) ? kCostInf : Ln[j], lb(j, 4), p * 4u);
-      if (EQ && act[j]) {
-        // val >= m exactly (every candidate is >= m), so E >= 0; rounding to nearest.  At an
-        // invalid pixel the level is ignored (the combine finds the pixel from its cost).
-        const uint32_t q = min(kExcessMaxQ, (uint32_t)fmaf(val - m, a.eq_inv, 0.5f));
-        beout.put_u16_nt(q, lb(j, 2), p * 2u);
-      }
+      out[j] = invalid ? kCostInf : (FIRST ? Ln[j] : g[i][j] + Ln[j]);   // sgm.cu:101-108
+      lo_[j] = (LMASK && invalid) ? kCostInf : Ln[j];
+      // val >= m exactly (every candidate is >= m), so E >= 0; rounding to nearest.  At an
+      // invalid pixel the level is ignored (the combine finds the pixel from its cost).
+      if (EQ) q[j] = min(kExcessMaxQ, (uint32_t)fmaf(val - m, a.eq_inv, 0.5f));
     }
+    if (AGG) store_planes_f32<NV, 0>(bagg, out, act, pack, off, p * 4u);
+    if (LOUT) store_planes_f32<NV, 2>(blout, lo_, act, pack, off, p * 4u);
+    if (EQ) store_planes_u16<NV, 2>(beout, q, act, pack, off, p * 2u);
 #pragma unroll
     for (int j = 0; j < NV; ++j) L[j] = Ln[j];
   };
