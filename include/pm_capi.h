@@ -194,6 +194,45 @@ pm_status pm_depth_fuse(int32_t width, int32_t height, pm_camera cam, const floa
 /* Empties n z-buffer entries (sets every bit), for pm_depth_fuse's alternating buffers. */
 pm_status pm_zbuf_reset(uint64_t* zbuf, size_t n, pm_stream stream);
 
+/* The two halves of pm_depth_fuse as separate calls, and a step that also starts the next
+ * one.  No reference counterpart: depth_fuse (depth_fusion.cu:8-280) runs transformMap,
+ * hole_filling, depth_predict and depthUpdate in one call.
+ *
+ * pm_depth_fuse_warp: the forward warp of a step alone (transformMap, depth_fusion.cu:8-78):
+ * gates and transforms `state` with the previous->current pose, rewrites state.mu / state.sig
+ * of moved pixels in place and inserts them into zbuf, which must be empty on entry.
+ * zbuf_clear (optional, another buffer) is emptied for a later step.
+ *
+ * pm_depth_fuse_step: the rest of a step (hole fill, predict, update, depth_fusion.cu:80-225)
+ * on the z-buffer a warp filled, with the options
+ *   median    : non-zero: the measurement is the 3x3 median of `depth` (pm_median3x3's
+ *               filter), computed in the same launch and written to depth_out if non-NULL;
+ *               depth, depth_out and fused must then be three different buffers.  Zero:
+ *               the measurement is depth itself (which may be fused, in place) and depth_out
+ *               must be NULL.
+ *   r_next, t_next : NULL, or the next step's previous->current pose: then the call also
+ *               runs the next step's warp on the state it has just written (cur), in the same
+ *               launch and with the same results as pm_depth_fuse_warp(cam, r_next, t_next,
+ *               cur, zbuf_next, zbuf_clear) after it.  zbuf_next must be empty on entry;
+ *               zbuf_clear (optional) is emptied for the step after next.  zbuf, zbuf_next and
+ *               zbuf_clear are three different buffers: neighbouring pixels still read this
+ *               step's zbuf while the launch runs.
+ * A chain warp(1) -> step(1, next 2) -> step(2, next 3) -> ... -> step(n) over three
+ * z-buffers in rotation (step i reads Z[i % 3], fills Z[(i + 1) % 3], empties Z[(i + 2) % 3])
+ * gives the same fused depths, tables and states as pm_depth_fuse at every step, except that
+ * a step's cur.mu / cur.sig already hold the warped values the next pm_depth_fuse would
+ * write into them (the last step's state is untouched).  Callers that know each pose before
+ * the previous step's output (a recorded sequence, a batch of keyframes) run one launch per
+ * keyframe this way. */
+pm_status pm_depth_fuse_warp(int32_t width, int32_t height, pm_camera cam, const float r_p2c[9],
+                             const float t_p2c[3], pm_fusion_state state, uint64_t* zbuf,
+                             uint64_t* zbuf_clear, pm_stream stream);
+pm_status pm_depth_fuse_step(int32_t width, int32_t height, pm_camera cam, const float* depth,
+                             int32_t median, float* depth_out, float* fused,
+                             const uint64_t* zbuf, int32_t* table, pm_fusion_state pre,
+                             pm_fusion_state cur, const float* r_next, const float* t_next,
+                             uint64_t* zbuf_next, uint64_t* zbuf_clear, pm_stream stream);
+
 /* pm_depth_fuse without a z-buffer argument, for callers that cannot allocate device memory
  * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
  * counterpart in the reference.  The z-buffer is a library-owned device array of

@@ -131,6 +131,9 @@ SIGNATURES = {
     "pm_depth_fuse": (_i32, [_i32, _i32, Camera, _fp, _fp, _vp, _vp, _vp, _vp, _vp,
                              FusionState, FusionState, _vp]),
     "pm_zbuf_reset": (_i32, [_vp, ctypes.c_size_t, _vp]),
+    "pm_depth_fuse_warp": (_i32, [_i32, _i32, Camera, _fp, _fp, FusionState, _vp, _vp, _vp]),
+    "pm_depth_fuse_step": (_i32, [_i32, _i32, Camera, _vp, _i32, _vp, _vp, _vp, _vp,
+                                  FusionState, FusionState, _fp, _fp, _vp, _vp, _vp]),
     "pm_depth_fuse_noalloc": (_i32, [_i32, _i32, Camera, _fp, _fp, _vp, _vp, _vp, FusionState,
                                      FusionState, _vp]),
     "pm_fill_f32": (_i32, [_vp, ctypes.c_size_t, ctypes.c_float, _vp]),

@@ -227,6 +227,48 @@ __device__ __forceinline__ float uniform(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// ---- depth maps -------------------------------------------------------------------------
+__device__ __forceinline__ bool valid_depth(float z) { return z > 0.0f && z < kDepInf; }
+
+// The 3x3 median filter at pixel (x, y): an invalid centre is kept as it is; else the lower
+// median of the valid depths of the window, all in registers.  Absent or invalid neighbours
+// become +inf, an odd-even transposition network (9 rounds, which sorts any 9 values) orders
+// the window, and the median is selected from the sorted values by the valid count.  Depths
+// and +inf are positive, so their bit patterns order like unsigned integers and the
+// compare-exchanges are integer min / max.
+__device__ __forceinline__ float median3x3_at(const float* __restrict__ in, int W, int H, int x,
+                                              int y) {
+  const float z = in[(long)y * W + x];
+  if (!valid_depth(z)) return z;
+  constexpr uint32_t kAbsent = 0x7F800000u;   // +inf
+  uint32_t v[9];
+  int n = 0;
+#pragma unroll
+  for (int j = -1; j <= 1; ++j)
+#pragma unroll
+    for (int i = -1; i <= 1; ++i) {
+      const int xx = x + i, yy = y + j;
+      const bool inside = xx >= 0 && yy >= 0 && xx < W && yy < H;
+      const float w = inside ? in[(long)yy * W + xx] : 0.0f;
+      const bool ok = inside && valid_depth(w);
+      v[(j + 1) * 3 + (i + 1)] = ok ? __float_as_uint(w) : kAbsent;
+      n += ok ? 1 : 0;
+    }
+#pragma unroll
+  for (int round = 0; round < 9; ++round)
+#pragma unroll
+    for (int a = round & 1; a + 1 < 9; a += 2) {
+      const uint32_t lo = min(v[a], v[a + 1]), hi = max(v[a], v[a + 1]);
+      v[a] = lo;
+      v[a + 1] = hi;
+    }
+  const int k = (n - 1) / 2;   // 0..4
+  uint32_t med = v[0];
+#pragma unroll
+  for (int i = 1; i <= 4; ++i) med = k == i ? v[i] : med;
+  return __uint_as_float(med);
+}
+
 // ---- cost storage ---------------------------------------------------------------------
 template <typename CT>
 __device__ __forceinline__ float load_cost(gptr<const CT> p);

@@ -375,47 +375,14 @@ void launch_filter_batch(int LP, const KfDesc* kfs, const int* kf, const int* fi
   }
 }
 
-__device__ __forceinline__ bool valid_depth(float z) { return z > 0.0f && z < kDepInf; }
-
-// The lower median of the valid depths of the 3x3 window (the centre is valid), all in
-// registers: absent or invalid neighbours become +inf, an odd-even transposition network (9
-// rounds, which sorts any 9 values) orders the window, and the median is selected from the
-// sorted values by the valid count.  Depths and +inf are positive, so their bit patterns
-// order like unsigned integers and the compare-exchanges are integer min / max.
+// The optional 3x3 median over valid depths (median3x3_at, in common.hpp), one thread per
+// pixel.
 __global__ __launch_bounds__(256) void median3x3_kernel(const float* __restrict__ in,
                                                         float* __restrict__ out, int W, int H) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= W) return;
-  const float z = in[(long)y * W + x];
-  if (!valid_depth(z)) { out[(long)y * W + x] = z; return; }
-  constexpr uint32_t kAbsent = 0x7F800000u;   // +inf
-  uint32_t v[9];
-  int n = 0;
-#pragma unroll
-  for (int j = -1; j <= 1; ++j)
-#pragma unroll
-    for (int i = -1; i <= 1; ++i) {
-      const int xx = x + i, yy = y + j;
-      const bool inside = xx >= 0 && yy >= 0 && xx < W && yy < H;
-      const float w = inside ? in[(long)yy * W + xx] : 0.0f;
-      const bool ok = inside && valid_depth(w);
-      v[(j + 1) * 3 + (i + 1)] = ok ? __float_as_uint(w) : kAbsent;
-      n += ok ? 1 : 0;
-    }
-#pragma unroll
-  for (int round = 0; round < 9; ++round)
-#pragma unroll
-    for (int a = round & 1; a + 1 < 9; a += 2) {
-      const uint32_t lo = min(v[a], v[a + 1]), hi = max(v[a], v[a + 1]);
-      v[a] = lo;
-      v[a + 1] = hi;
-    }
-  const int k = (n - 1) / 2;   // 0..4
-  uint32_t med = v[0];
-#pragma unroll
-  for (int i = 1; i <= 4; ++i) med = k == i ? v[i] : med;
-  out[(long)y * W + x] = __uint_as_float(med);
+  out[(long)y * W + x] = median3x3_at(in, W, H, x, y);
 }
 
 // CASE 0 / 1 winner-take-all (stereo_mapper.cpp:637-660, 681-704): the running minimum

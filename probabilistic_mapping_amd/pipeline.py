@@ -265,12 +265,15 @@ class KeyframeSequence:
     cost (over its sources) -> SGM -> filter (-> median) -> fusion on the current stream with
     no host synchronisation.  The first keyframe only resets the filter state
     (stereo_mapper.cpp:128-146); later ones fuse into the ping-pong state (:149-243).
+    ``lookahead``: each fusion step also runs the next keyframe's forward warp (see
+    ``fusion_stage``).
     """
 
     def __init__(self, images, poses, sources, K, planes, idep_min, idep_max, knobs, paths,
                  cost_type="f32", median=False, device="cuda", sgm_storage="exact",
-                 write_agg=True):
+                 write_agg=True, lookahead=False):
         self.device = torch.device(device)
+        self.lookahead = bool(lookahead)
         self.H, self.W = np.asarray(images[0]).shape
         self.D = planes
         self.idep_min = float(idep_min)
@@ -309,9 +312,10 @@ class KeyframeSequence:
         self.dep = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.dep_raw = torch.empty((H, W), dtype=torch.float32, device=self.device)
         self.fused = torch.empty((H, W), dtype=torch.float32, device=self.device)
-        # two z-buffers used alternately: each fusion step empties the other for the next one
+        # three z-buffers in rotation: fusion step k reads zbufs[k % 3], fills (or lets the
+        # next step's warp fill) zbufs[(k + 1) % 3] and empties the other one for later
         self.zbufs = [torch.empty((H, W), dtype=torch.int64, device=self.device)
-                      for _ in range(2)]
+                      for _ in range(3)]
         self.table = torch.empty((H, W), dtype=torch.int32, device=self.device)
         self.states = [new_state(H, W, self.device), new_state(H, W, self.device)]
         self.reset()
@@ -321,7 +325,8 @@ class KeyframeSequence:
         self.pre = 0
         self.prev_pose = None
         self.prev_index = None
-        self.zpar = 0
+        self.zk = 0            # fusion steps so far (z-buffer rotation)
+        self.warped = None     # keyframe whose warp the previous step already ran (lookahead)
         for z in self.zbufs:
             zbuf_reset(z)
 
@@ -344,12 +349,14 @@ class KeyframeSequence:
         """Plane-sweep cost of keyframe i over its sources into ``self.cost``."""
         C.check(C.load().pm_calc_cost(*self._cost_args(i), _stream()))
 
-    def depth_stage(self, i: int) -> torch.Tensor:
-        """cost -> SGM + filter (-> median) for keyframe i; returns the filtered depth."""
+    def depth_stage(self, i: int, median: bool = True) -> torch.Tensor:
+        """cost -> SGM + filter for keyframe i into ``dep_raw`` (with the median) or ``dep``;
+        ``median``: also run the median into ``dep`` here (``run_keyframe`` leaves it to the
+        fusion step, which computes it in its own launch)."""
         self.cost_stage(i)
         out = self.dep_raw if self.median else self.dep
         self.sgm_stage(i, out)
-        if self.median:
+        if self.median and median:
             median3x3(self.dep_raw, out=self.dep)
         return self.dep
 
@@ -374,40 +381,77 @@ class KeyframeSequence:
               else C.load().pm_sgm_filter_paths)
         C.check(fn(*self._sgm_cache[key], _stream()))
 
+    def _measurement(self, i: int) -> torch.Tensor:
+        """Keyframe i's filtered depth as the filter wrote it (before the median, if any)."""
+        return self.dep_raw if self.median else self.dep
+
+    def _median_out(self) -> torch.Tensor:
+        return self.dep
+
     def fusion_stage(self, i: int) -> None:
-        """One outputFusion step on ``self.dep`` (stereo_mapper.cpp:128-243)."""
+        """(The median, then) one outputFusion step of keyframe i (stereo_mapper.cpp:128-243):
+        the forward warp of the previous state (pm_depth_fuse_warp) and the update with the
+        median in the same launch (pm_depth_fuse_step).  With ``lookahead`` the update also
+        runs keyframe i+1's warp (its pose is known), so each later step is one launch; the
+        state's mu / sig then hold the values warped into keyframe i+1 until that step runs
+        (the fused depth, the table and the other state fields are those of keyframe i)."""
         R, t = self.poses[i]
+        raw = self._measurement(i)
+        out = self._median_out() if self.median else raw
         if self.first:
             self.first = False
             self.pre = 0
             for s in self.states:
                 s["alpha"].zero_()
+            if self.median:
+                median3x3(raw, out=out)
+            self.dep = out
             self.prev_pose = (R, t)
             self.prev_index = i
             self.fused.zero_()
             return
-        # the pm_depth_fuse arguments of (keyframe, previous keyframe, state parity, z-buffer
-        # parity, depth buffer), built once
-        key = (i, self.prev_index, self.pre, self.zpar, self.dep.data_ptr())
+        if self.warped is not None and self.warped != i:
+            raise ValueError(f"fusion_stage({i}): the previous step warped keyframe "
+                             f"{self.warped}")
+        nxt = self.lookahead and i + 1 < len(self.images)
+        k = self.zk % 3
+        key = (i, self.prev_index, self.pre, k, raw.data_ptr(), out.data_ptr(), nxt,
+               self.warped is None)
         if not hasattr(self, "_fuse_cache"):
             self._fuse_cache = {}
         if key not in self._fuse_cache:
+            def f32(a, n):
+                return (ctypes.c_float * n)(*[float(v) for v in
+                                              np.asarray(a, np.float32).reshape(n)])
+            cam = C.Camera(*[float(c) for c in self.cam])
             r, tt = pose_p2c(R, t, *self.prev_pose)
             pre, cur = self.states[self.pre], self.states[self.pre ^ 1]
-            rr = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(r, np.float32).reshape(9)])
-            ttc = (ctypes.c_float * 3)(*[float(v) for v in np.asarray(tt, np.float32).reshape(3)])
-            self._fuse_cache[key] = (self.W, self.H, C.Camera(*[float(c) for c in self.cam]), rr,
-                                     ttc, _ptr(self.dep), _ptr(self.fused),
-                                     _ptr(self.zbufs[self.zpar]), _ptr(self.zbufs[self.zpar ^ 1]),
-                                     _ptr(self.table), _state(pre), _state(cur))
-        C.check(C.load().pm_depth_fuse(*self._fuse_cache[key], _stream()))
+            z = [_ptr(self.zbufs[(k + d) % 3]) for d in range(3)]
+            warp = None if self.warped is not None else \
+                (self.W, self.H, cam, f32(r, 9), f32(tt, 3), _state(pre), z[0], z[1])
+            rn = tn = None
+            if nxt:
+                r2, t2 = pose_p2c(*self.poses[i + 1], R, t)
+                rn, tn = f32(r2, 9), f32(t2, 3)
+            step = (self.W, self.H, cam, _ptr(raw), int(self.median),
+                    _ptr(out) if self.median else None, _ptr(self.fused), z[0],
+                    _ptr(self.table), _state(pre), _state(cur), rn, tn,
+                    z[1] if nxt else None, z[2] if nxt else None)
+            self._fuse_cache[key] = (warp, step)
+        warp, step = self._fuse_cache[key]
+        lib = C.load()
+        if warp is not None:
+            C.check(lib.pm_depth_fuse_warp(*warp, _stream()))
+        C.check(lib.pm_depth_fuse_step(*step, _stream()))
+        self.dep = out
+        self.warped = i + 1 if nxt else None
         self.prev_pose = (R, t)
         self.prev_index = i
         self.pre ^= 1
-        self.zpar ^= 1
+        self.zk += 1
 
     def run_keyframe(self, i: int) -> None:
-        self.depth_stage(i)
+        self.depth_stage(i, median=False)
         self.fusion_stage(i)
 
     def run(self) -> None:
@@ -628,13 +672,14 @@ class BatchedSequence(KeyframeSequence):
         """Keyframe i's filtered depth as the batch wrote it (before the median, if any)."""
         return self.batch.outputs[i]["depth"]
 
+    def _measurement(self, i: int) -> torch.Tensor:
+        return self.depth_of(i)
+
+    def _median_out(self) -> torch.Tensor:
+        return self.dep_med
+
     def fusion_step(self, i: int) -> None:
         """(median ->) fusion of keyframe i, after the batch has run."""
-        d = self.depth_of(i)
-        if self.median:
-            median3x3(d, out=self.dep_med)
-            d = self.dep_med
-        self.dep = d
         self.fusion_stage(i)
 
     def run(self) -> None:

@@ -101,6 +101,52 @@ def test_depth_fuse_validation():
     assert lib.pm_zbuf_reset(NULL, 4, NULL) == 1
 
 
+def test_depth_fuse_warp_and_step_validation():
+    """pm_depth_fuse_warp / pm_depth_fuse_step refuse bad arguments before any launch."""
+    lib = C.load()
+    P = ctypes.c_void_p
+    r = (ctypes.c_float * 9)(1, 0, 0, 0, 1, 0, 0, 0, 1)
+    t = (ctypes.c_float * 3)(0, 0, 0)
+    bad = (ctypes.c_float * 9)(1, 0, 0, 0, float("nan"), 0, 0, 0, 1)
+    st = C.FusionState(0x1000, 0x2000, 0x3000, 0x4000)
+    st2 = C.FusionState(0x5000, 0x6000, 0x7000, 0x8000)
+    half = C.FusionState(0x1000, 0x2000, 0, 0x4000)
+    cam = C.Camera(100, 100, 3, 2)
+    dep, out, fused = P(0x10000), P(0x20000), P(0x30000)
+    z0, z1, z2, tab = P(0x40000), P(0x50000), P(0x60000), P(0x70000)
+    warp = lib.pm_depth_fuse_warp
+    assert warp(8, 6, cam, r, t, st, NULL, z1, NULL) == 1                  # no z-buffer
+    assert warp(8, 6, cam, r, t, st, z0, z0, NULL) == 1                    # clears its own
+    assert warp(8, 6, C.Camera(100, 0, 3, 2), r, t, st, z0, z1, NULL) == 1
+    assert warp(8, 6, cam, bad, t, st, z0, z1, NULL) == 1
+    assert warp(8, 6, cam, r, t, half, z0, z1, NULL) == 1
+    assert warp(0, 6, cam, r, t, st, z0, z1, NULL) == 1
+    step = lib.pm_depth_fuse_step
+    # state alias; null output; bad size
+    assert step(8, 6, cam, dep, 1, out, fused, z0, tab, st, st, None, None, NULL, NULL,
+                NULL) == 1
+    assert step(8, 6, cam, dep, 1, out, NULL, z0, tab, st, st2, None, None, NULL, NULL,
+                NULL) == 1
+    assert step(8, 70000, cam, dep, 1, out, fused, z0, tab, st, st2, None, None, NULL, NULL,
+                NULL) == 1
+    # the median reads neighbours: its input must not alias an output
+    assert step(8, 6, cam, dep, 1, out, dep, z0, tab, st, st2, None, None, NULL, NULL,
+                NULL) == 1
+    assert step(8, 6, cam, dep, 1, dep, fused, z0, tab, st, st2, None, None, NULL, NULL,
+                NULL) == 1
+    assert step(8, 6, cam, dep, 0, out, fused, z0, tab, st, st2, None, None, NULL, NULL,
+                NULL) == 1                                                  # depth_out w/o median
+    # the next step's warp: a complete pose, a finite one, three distinct z-buffers
+    assert step(8, 6, cam, dep, 1, out, fused, z0, tab, st, st2, r, None, z1, z2, NULL) == 1
+    assert step(8, 6, cam, dep, 1, out, fused, z0, tab, st, st2, r, t, NULL, z2, NULL) == 1
+    assert step(8, 6, cam, dep, 1, out, fused, z0, tab, st, st2, bad, t, z1, z2, NULL) == 1
+    assert step(8, 6, cam, dep, 1, out, fused, z0, tab, st, st2, r, t, z0, z2, NULL) == 1
+    assert step(8, 6, cam, dep, 1, out, fused, z0, tab, st, st2, r, t, z1, z1, NULL) == 1
+    assert step(8, 6, cam, dep, 1, out, fused, z0, tab, st, st2, None, None, NULL, z2,
+                NULL) == 1                                                  # clear w/o next
+    assert C.last_error() != ""
+
+
 def test_cpu_rules_fusion_validation():
     """pm_depth_fuse_cpu_rules (outputFusionCPU CASE >= 3) refuses bad arguments before any
     launch."""

@@ -61,6 +61,39 @@ def test_stereo_mapper_small_scene(small_scene):
     _assert_depth(st["depth"])
 
 
+def test_mapper_host_images_reusable_after_call(small_scene):
+    """initReference and update return before the device has used the image (the upload is
+    asynchronous), so they must take their own copy first: overwriting the caller's buffer
+    right after each call changes nothing.  Through the C ABI (pm_mapper_init_reference,
+    pm_mapper_update), against a mapper fed buffers that are never touched."""
+    from probabilistic_mapping_amd.mapper import StereoMapper, set_knobs
+    s = small_scene
+    D = s["planes"]
+    ds = (s["idep_max"] - s["idep_min"]) / (D - 1)
+    noise = np.random.default_rng(11)
+    old = set_knobs(idep_min=s["idep_min"], sgm_paths=4, median=0, **s["knobs"])
+    try:
+        def run(clobber):
+            m = StereoMapper(320, 240, D)
+            m.initIntrinsic(s["scene"].K, 0.0, ds)
+            buf = np.array(s["ref"], dtype=np.uint8, copy=True)
+            m.initReference(buf)
+            if clobber:
+                buf[...] = noise.integers(0, 256, buf.shape, dtype=np.uint8)
+            for img, (R, T) in zip(s["srcs"], s["src_poses"]):
+                buf = np.array(img, dtype=np.uint8, copy=True)
+                m.update(buf, *s["ref_pose"], R, T)
+                if clobber:
+                    buf[...] = noise.integers(0, 256, buf.shape, dtype=np.uint8)
+            return m.output(), m.download("raw_cost")
+        d0, c0 = run(False)
+        d1, c1 = run(True)
+    finally:
+        set_knobs(**old)
+    assert np.array_equal(c0, c1)
+    assert np.array_equal(d0, d1, equal_nan=True)
+
+
 def test_stereo_mapper_fusion_semantics(small_scene):
     """outputFusion: first keyframe returns zeros and only resets; later calls return the
     filtered (un-fused) depth, and the fused depth is a separate buffer."""
