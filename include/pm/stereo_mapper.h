@@ -97,7 +97,7 @@ class StereoMapper {
   void alloc();
   void release();
   void ensure_cost();
-  void run_sgm_filter();
+  void run_sgm_filter(bool median = true);
   pm_homography stage_homography(const Mat& R_l, const Mat& T_l, const Mat& R_r,
                                  const Mat& T_r);
   void fuse_cpu_rules(Mat& result, Matrix3d R, Vector3d t, float fx, float fy, float cx,
@@ -122,6 +122,8 @@ class StereoMapper {
   DeviceMat cpu_work_, cpu_result_;        // outputFusionCPU's CASE >= 3 step
   bool cost_ready_ = false;
   bool depth_ready_ = false;
+  bool raw_ready_ = false;     // SGM + filter done (into med_tmp_ if raw_in_med_, else dep)
+  bool raw_in_med_ = false;    // the median knob when the filter ran: dep = median(med_tmp_)
   bool have_reference_ = false;
   // pinned host staging of host images (slot 0: the reference, slot 1 + s: source s), each
   // with the event recorded after its last copy; initReference and update copy the caller's

@@ -250,7 +250,7 @@ void StereoMapper::initReference(const Mat& _img_l) {
   staged_.clear();
   have_reference_ = true;
   cost_ready_ = false;
-  depth_ready_ = false;
+  depth_ready_ = raw_ready_ = false;
 }
 
 // initReference from an ingested frame: a device copy into img_l, ordered after the frame's
@@ -274,7 +274,7 @@ void StereoMapper::initReference(const FrameRing& ring, int64_t frame) {
   staged_.clear();
   have_reference_ = true;
   cost_ready_ = false;
-  depth_ready_ = false;
+  depth_ready_ = raw_ready_ = false;
 }
 
 // the homography of update (stereo_mapper.cpp:79-90), also kept in R, T as the reference does
@@ -318,7 +318,7 @@ void StereoMapper::update(const Mat& _img_r, const Mat& R_l, const Mat& T_l, con
   homs_.push_back(h);
   staged_.push_back(RingFrame{nullptr, -1});
   cost_ready_ = false;
-  depth_ready_ = false;
+  depth_ready_ = raw_ready_ = false;
 }
 
 // update from an ingested frame: only the homography is set up here; the cost pass reads
@@ -335,7 +335,7 @@ void StereoMapper::update(const FrameRing& ring, int64_t frame, const Mat& R_l, 
   homs_.push_back(h);
   staged_.push_back(RingFrame{&ring, frame});
   cost_ready_ = false;
-  depth_ready_ = false;
+  depth_ready_ = raw_ready_ = false;
 }
 
 void StereoMapper::ensure_cost() {
@@ -363,26 +363,35 @@ void StereoMapper::ensure_cost() {
   cost_ready_ = true;
 }
 
-// sgm (stereo_mapper.cpp:118-120) + filter (:125-126) (+ the optional median)
-void StereoMapper::run_sgm_filter() {
+// sgm (stereo_mapper.cpp:118-120) + filter (:125-126) (+ the optional median).  median =
+// false leaves a pending median to the caller (outputFusion computes it inside its fusion
+// step's launch): the filter's depth is then in med_tmp_ and depth_ready_ stays false.
+void StereoMapper::run_sgm_filter(bool median) {
   if (depth_ready_) return;
-  ensure_cost();
-  pm_sgm_params p{pi1, pi2, tau_so, sgm_q1, sgm_q2, SGM_PATHS};
-  const pm_shape sh{W_, H_, D_};
-  float* fout = reinterpret_cast<float*>(MEDIAN_FILTER ? med_tmp_.data : dep.data);
-  // all paths in flight at once through the per-path workspace (same bits as pm_sgm_filter)
-  uint64_t need = 0;
-  check(pm_sgm_workspace_size(sh, SGM_PATHS, &need));
-  if (sgm_ws_.bytes() < need) {
-    hip_check(hipStreamSynchronize((hipStream_t)stream_), "hipStreamSynchronize");
-    dfree(sgm_ws_);
-    sgm_ws_ = dalloc(SGM_PATHS, (int)((long)W_ * H_ * D_), CV_32F);
+  if (!raw_ready_) {
+    ensure_cost();
+    pm_sgm_params p{pi1, pi2, tau_so, sgm_q1, sgm_q2, SGM_PATHS};
+    const pm_shape sh{W_, H_, D_};
+    raw_in_med_ = MEDIAN_FILTER != 0;
+    float* fout = reinterpret_cast<float*>(raw_in_med_ ? med_tmp_.data : dep.data);
+    // all paths in flight at once through the per-path workspace (same bits as pm_sgm_filter)
+    uint64_t need = 0;
+    check(pm_sgm_workspace_size(sh, SGM_PATHS, &need));
+    if (sgm_ws_.bytes() < need) {
+      hip_check(hipStreamSynchronize((hipStream_t)stream_), "hipStreamSynchronize");
+      dfree(sgm_ws_);
+      sgm_ws_ = dalloc(SGM_PATHS, (int)((long)W_ * H_ * D_), CV_32F);
+    }
+    check(pm_sgm_filter_paths(sh, img_l.data, raw_cost.data, raw_cost_type_, p, var_scale,
+                              IDEP_MIN, DEP_SAMPLE, sgm_ws_.data, sgm_ws_.bytes(),
+                              reinterpret_cast<float*>(sgm_cost.data), fout, stream_));
+    raw_ready_ = true;
   }
-  check(pm_sgm_filter_paths(sh, img_l.data, raw_cost.data, raw_cost_type_, p, var_scale,
-                            IDEP_MIN, DEP_SAMPLE, sgm_ws_.data, sgm_ws_.bytes(),
-                            reinterpret_cast<float*>(sgm_cost.data), fout, stream_));
-  if (MEDIAN_FILTER)
-    check(pm_median3x3(fout, reinterpret_cast<float*>(dep.data), W_, H_, stream_));
+  if (raw_in_med_) {
+    if (!median) return;
+    check(pm_median3x3(reinterpret_cast<const float*>(med_tmp_.data),
+                       reinterpret_cast<float*>(dep.data), W_, H_, stream_));
+  }
   depth_ready_ = true;
 }
 
@@ -397,9 +406,10 @@ void StereoMapper::output(Mat& result) {
 // stereo_mapper.cpp:111-244
 void StereoMapper::outputFusion(Mat& result, Matrix3d Rw, Vector3d tw, float fx, float fy,
                                 float cx, float cy) {
-  run_sgm_filter();
+  run_sgm_filter(/*median=*/false);   // a pending median runs inside the fusion step
   const size_t N = (size_t)W_ * H_;
   if (firstDepth) {   // :128-146
+    run_sgm_filter();
     firstDepth = false;
     if (!(result.data && result.rows == H_ && result.cols == W_ && result.type == CV_32F))
       result.create_with(H_, W_, CV_32F, pinned_alloc, pinned_release, &kPinnedTag);
@@ -428,12 +438,21 @@ void StereoMapper::outputFusion(Mat& result, Matrix3d Rw, Vector3d tw, float fx,
     return pm_fusion_state{reinterpret_cast<float*>(s[0]->data), reinterpret_cast<float*>(s[1]->data),
                            reinterpret_cast<float*>(s[2]->data), reinterpret_cast<float*>(s[3]->data)};
   };
+  // pm_depth_fuse as its two halves, with the pending median (if any) computed inside the
+  // second launch: the forward warp fills z-buffer zpar_ and empties the other one for the
+  // next keyframe, then hole fill, predict and update (same results as median3x3 +
+  // pm_depth_fuse)
   uint64_t* zb = reinterpret_cast<uint64_t*>(zbuf_.data);
-  check(pm_depth_fuse(W_, H_, pm_camera{fx, fy, cx, cy}, r, t,
-                      reinterpret_cast<const float*>(dep.data),
-                      reinterpret_cast<float*>(fuseDepth.data), zb + N * zpar_,
-                      zb + N * (zpar_ ^ 1), reinterpret_cast<int32_t*>(propogate_table.data),
-                      state(pre), state(cur), stream_));
+  const pm_camera cam{fx, fy, cx, cy};
+  check(pm_depth_fuse_warp(W_, H_, cam, r, t, state(pre), zb + N * zpar_, zb + N * (zpar_ ^ 1),
+                           stream_));
+  const bool med = !depth_ready_;
+  check(pm_depth_fuse_step(W_, H_, cam, reinterpret_cast<const float*>(med ? med_tmp_.data : dep.data),
+                           med ? 1 : 0, med ? reinterpret_cast<float*>(dep.data) : nullptr,
+                           reinterpret_cast<float*>(fuseDepth.data), zb + N * zpar_,
+                           reinterpret_cast<int32_t*>(propogate_table.data), state(pre),
+                           state(cur), nullptr, nullptr, nullptr, nullptr, stream_));
+  depth_ready_ = true;
   zpar_ ^= 1;
   download_result(dep, result);   // :229-230 -- the filtered, not the fused, depth
   R_b_2_w = Rw;            // :237-241
