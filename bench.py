@@ -56,8 +56,8 @@ SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2034e9,
 #   * COST_VALU: the cost launch's SQ_INSTS_VALU (wave64 instructions) and the fraction of the
 #     chip's VALU issue slots they fill over the launch's duration in the profiled run (at the
 #     2.4 GHz peak clock).
-COST_VALU = {("plain", "f32", "exact"): (154.75e6, 0.902),
-             ("batched", "f16", "compact"): (151.62e6, 0.944)}
+COST_VALU = {("plain", "f32", "exact"): (154.75e6, 0.904),
+             ("batched", "f16", "compact"): (151.62e6, 0.955)}
 
 
 def parse():
