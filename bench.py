@@ -56,6 +56,10 @@ SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2034e9,
 #   * COST_VALU: the cost launch's SQ_INSTS_VALU (wave64 instructions) and the fraction of the
 #     chip's VALU issue slots they fill over the launch's duration in the profiled run (at the
 #     2.4 GHz peak clock).
+#   * UNEVEN_SGM_TRAFFIC: HBM bytes of the uneven batch's SGM pass (its paths and combine/filter
+#     launches) per replay, default keyframe set (UNEVEN_PROFILE).
+UNEVEN_PROFILE = "profiles/uneven_batch_16kf_f16.md"
+UNEVEN_SGM_TRAFFIC = 21.306e9
 COST_VALU = {("plain", "f32", "exact"): (154.75e6, 0.904),
              ("batched", "f16", "compact"): (151.62e6, 0.955)}
 
@@ -574,6 +578,8 @@ def bench_uneven(args, torch):
     torch.cuda.empty_cache()
     design = sum(sgm_pass_design_bytes(W, H, D, P, 2, "compact") for W, H, D, _ in shapes)
     sgm_ms = float(ms[1] + ms[2])
+    # counter bytes exist for the default keyframe set only
+    traffic = UNEVEN_SGM_TRAFFIC if args.seed == 0 else None
 
     # one keyframe at a time through the single-keyframe entry points
     def one_at_a_time(cost_type, storage):
@@ -626,7 +632,8 @@ def bench_uneven(args, torch):
                                                 "combine/filter launch), whole batch",
                      "achieved": design / (sgm_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": design / (sgm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "algorithmic_bytes": design, "traffic": None},
+                     "algorithmic_bytes": design, "traffic": traffic,
+                     "traffic_source": UNEVEN_PROFILE if traffic else None},
         "one_at_a_time": {"f16_compact": seq_f16, "f32_exact": seq_f32},
         "batch_vs_one_at_a_time_f16": (args.steps * nkf / elapsed) / seq_f16["value"],
     }
