@@ -371,6 +371,13 @@ __device__ __forceinline__ uint32_t shr1_max(float lo) {
   return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)__float_as_uint(lo), 0x138, 0xF, 0xF,
                                                false);
 }
+// shr1_max carried across calls: `prev` is the previous result.  Lane 0 has no source lane and
+// is not written, so it keeps the 0xFFFFFFFF it was initialised with, and the per-call move of
+// the fill value into the destination disappears.
+__device__ __forceinline__ uint32_t shr1_max_carry(uint32_t prev, float lo) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)prev, (int)__float_as_uint(lo), 0x138, 0xF,
+                                               0xF, false);
+}
 __device__ __forceinline__ uint32_t shl1_max(float hi) {
   return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)__float_as_uint(hi), 0x130, 0xF, 0xF,
                                                false);

@@ -242,6 +242,8 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       return (uint64_t)__ballot(abs((int)r - (int)prev) < a.tau_i);
   };
   // one step k >= 1 from slot i of a block buffer
+  // L(q, d0-1) of the step, lane 0 = 0xFFFFFFFF throughout (see shr1_max_carry)
+  uint32_t lo_carry = 0xFFFFFFFFu;
   // element offset of the current step's pixel, advanced by one stride per step
   uint32_t pk = (uint32_t)p0 * (uint32_t)D;
   const uint32_t pkstep = (uint32_t)pstep * (uint32_t)D;
@@ -264,7 +266,8 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
     float P1, P2;
     scalar_penalties(smooth, (uint32_t)i, a.P1s, a.P1n, a.P2s, a.P2n, P1, P2);
     // L(q, d0-1) and L(q, d0+NV) from the neighbouring lanes (0xFFFFFFFF where none)
-    const uint32_t lo = shr1_max(L[NV - 1]);
+    lo_carry = shr1_max_carry(lo_carry, L[NV - 1]);
+    const uint32_t lo = lo_carry;
     const uint32_t hi = shl1_max(L[0]);
     const float mP2 = m + P2;
     float Ln[NV], out[NV], lo_[NV];
