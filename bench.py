@@ -133,10 +133,10 @@ def attainable_copy(torch, mib: int) -> dict | None:
 
 
 def reference_dataflow_bytes(seq, cost_type: str) -> int:
-    """The reference-dataflow figure for the SGM pass with its filter: the reference's dataflow (each
-    path streams the cost and read-modify-writes the FP32 aggregate, P V (b_c + 8),
-    sgm.cu:52,101-108) plus the filter (V 4 read, N 4 written).  The passes here move fewer
-    bytes than that, so its rate is an effective figure, not a fraction of peak."""
+    """Bytes of the SGM pass with its filter in the reference's dataflow: each path streams the
+    cost and read-modify-writes the FP32 aggregate, P V (b_c + 8) (sgm.cu:52,101-108), plus the
+    filter (V 4 read, N 4 written).  The passes here move fewer bytes than that, so its rate is
+    an effective figure, not a fraction of peak."""
     N = seq.H * seq.W
     V = N * seq.D
     return seq.paths * V * (COST_BYTES[cost_type] + 8) + V * 4 + N * 4
