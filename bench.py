@@ -288,8 +288,9 @@ def measure_batched(args, torch, s, cost_type, storage):
     stages["sgm"]["filter_ms"] = float(ms[2])
     return {"value": args.steps * nkf / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
             "ms_per_step_median": float(np.median(step_ms)), "stages": stages,
-            "form": f"batched (20 keyframes per step, {len(groups)} cache-sized groups of "
-                    f"{len(groups[0].outputs)}, one hipGraph replay each)",
+            "form": (f"batched (20 keyframes per step, {len(groups)} cache-sized groups of "
+                     f"{len(groups[0].outputs)}, one hipGraph replay each)" if len(groups) > 1
+                     else f"batched (20 keyframes per step in one batch, one hipGraph replay)"),
             "cost_storage": cost_type,
             "sgm_storage": storage, "keyframes_per_step": nkf, "tuning": tuning,
             "inflight": inflight}

@@ -644,7 +644,9 @@ class BatchedSequence(KeyframeSequence):
 
     ``group``: keyframes per batch; "cache" (default) sizes groups by ``cache_groups`` so each
     group's cost volumes stay cache-resident for its SGM pass, an integer fixes the size, and
-    0 puts every keyframe in one batch."""
+    0 puts every keyframe in one batch.  The sweep storage always takes one batch: each of its
+    launches covers one 16-row block of every keyframe, so it needs every keyframe in the
+    launch to fill the chip (groups of 5 at 752x480 give 60 workgroups for 256 CUs)."""
 
     def __init__(self, *args, group="cache", **kw):
         super().__init__(*args, **kw)
@@ -655,6 +657,8 @@ class BatchedSequence(KeyframeSequence):
         specs = [{"ref": self.images[i], "quads": [self.quads[j] for j in self.sources[i]],
                   "homs": self.homs[i], "planes": D, "idep_min": self.idep_min,
                   "dep_sample": self.dep_sample} for i in range(n)]
+        if self.sgm_storage == "sweep":
+            group = 0
         if group == "cache":
             per = H * W * D * COST_TYPES[self.cost_type][1].itemsize
             ranges = cache_groups([per] * n) or [range(0, n)]
