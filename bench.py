@@ -290,7 +290,7 @@ def measure_batched(args, torch, s, cost_type, storage):
             "ms_per_step_median": float(np.median(step_ms)), "stages": stages,
             "form": (f"batched (20 keyframes per step, {len(groups)} cache-sized groups of "
                      f"{len(groups[0].outputs)}, one hipGraph replay each)" if len(groups) > 1
-                     else f"batched (20 keyframes per step in one batch, one hipGraph replay)"),
+                     else "batched (20 keyframes per step in one batch, one hipGraph replay)"),
             "cost_storage": cost_type,
             "sgm_storage": storage, "keyframes_per_step": nkf, "tuning": tuning,
             "inflight": inflight}
