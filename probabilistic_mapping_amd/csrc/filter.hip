@@ -90,14 +90,34 @@ __global__ __launch_bounds__(256) void combine_filter_kernel(long N, int D, int 
   auto load = [&](long pc, bool own, int d0, float (&v)[4]) {
     const long e = pc * D + d0;
     if (vec4 && d0 + 3 < D) {
-      // the path volumes are read exactly once: non-temporal loads
+      // the path volumes are read exactly once: non-temporal loads.  For 8 and 4 paths all
+      // loads are issued before the sum (a loop over a run-time count waits for each load
+      // before issuing the next); the sum keeps the path order.
       typedef float vf4 __attribute__((ext_vector_type(4)));
-      const vf4 t0 = __builtin_nontemporal_load(reinterpret_cast<const vf4*>(lbuf + e));
-      float4 t = make_float4(t0.x, t0.y, t0.z, t0.w);
-      for (int r = 1; r < paths; ++r) {
-        const vf4 u =
-            __builtin_nontemporal_load(reinterpret_cast<const vf4*>(lbuf + (long)r * V + e));
-        t.x += u.x; t.y += u.y; t.z += u.z; t.w += u.w;
+      auto ld = [&](int r) {
+        return __builtin_nontemporal_load(reinterpret_cast<const vf4*>(lbuf + (long)r * V + e));
+      };
+      vf4 u[8];
+      if (paths == 8) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) u[r] = ld(r);
+      } else if (paths == 4) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) u[r] = ld(r);
+      } else {
+        u[0] = ld(0);
+      }
+      float4 t = make_float4(u[0].x, u[0].y, u[0].z, u[0].w);
+      if (paths == 8 || paths == 4) {
+#pragma unroll
+        for (int r = 1; r < 8; ++r) {
+          if (r < paths) { t.x += u[r].x; t.y += u[r].y; t.z += u[r].z; t.w += u[r].w; }
+        }
+      } else {
+        for (int r = 1; r < paths; ++r) {
+          const vf4 w = ld(r);
+          t.x += w.x; t.y += w.y; t.z += w.z; t.w += w.w;
+        }
       }
       if (own && agg) *reinterpret_cast<float4*>(agg + e) = t;
       v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
@@ -179,14 +199,31 @@ __device__ __forceinline__ bool excess_agg4(const CT* __restrict__ cost,
     float c[4];
     load_cost4<CT>(cost + e, c);
     uint32_t s[4] = {0u, 0u, 0u, 0u};
-    // the excess volumes are read exactly once: non-temporal loads
-    typedef uint32_t vu2 __attribute__((ext_vector_type(2)));
-    for (int r = 0; r < paths; ++r) {
-      const vu2 u =
-          __builtin_nontemporal_load(reinterpret_cast<const vu2*>(ebuf + (long)r * V + e));
+    // the excess volumes are read exactly once: non-temporal loads.  For 8 and 4 paths all
+    // loads are issued before the first is used (a loop over a run-time count waits for each
+    // one before issuing the next); the integer sum does not depend on the order.
+    auto add = [&](u32x2 u) {
       const uint32_t q[4] = {u.x & 0xFFFFu, u.x >> 16, u.y & 0xFFFFu, u.y >> 16};
 #pragma unroll
       for (int j = 0; j < 4; ++j) s[j] += q[j];
+    };
+    auto ld = [&](int r) {
+      return __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(ebuf + (long)r * V + e));
+    };
+    if (paths == 8) {
+      u32x2 u[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) u[r] = ld(r);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) add(u[r]);
+    } else if (paths == 4) {
+      u32x2 u[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) u[r] = ld(r);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) add(u[r]);
+    } else {
+      for (int r = 0; r < paths; ++r) add(ld(r));
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
