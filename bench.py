@@ -60,8 +60,8 @@ SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2034e9,
 #     launches) per replay, default keyframe set (UNEVEN_PROFILE).
 UNEVEN_PROFILE = "profiles/uneven_batch_16kf_f16.md"
 UNEVEN_SGM_TRAFFIC = 21.306e9
-COST_VALU = {("plain", "f32", "exact"): (154.75e6, 0.904),
-             ("batched", "f16", "compact"): (151.62e6, 0.955)}
+COST_VALU = {("plain", "f32", "exact"): (154.75e6, 0.911),
+             ("batched", "f16", "compact"): (151.62e6, 0.957)}
 
 
 def parse():
