@@ -31,9 +31,9 @@ __host__ __device__ inline int path_dy(int p) { return ((0x8225 >> (2 * p)) & 3)
 // ---- per-keyframe stage arguments -------------------------------------------------------
 // Plane-sweep cost inputs of one keyframe (passed by value to the single-keyframe kernels,
 // read from a device table by the batched ones).
-// Sources a TY-row cost tile can hold the per-plane warp records of (12 floats per source in
+// Sources a TY-row cost tile can hold the per-plane warp records of (16 floats per source in
 // the plane's TY x 64-float output slot).
-__host__ __device__ constexpr int cost_tile_max_sources(int TY) { return TY * 64 / 12; }
+__host__ __device__ constexpr int cost_tile_max_sources(int TY) { return TY * 64 / 16; }
 
 struct CostArgs {
   int W, H, D, S;

@@ -135,6 +135,53 @@ __device__ __forceinline__ InsideCorner tile_inside_setup(const CA& a, float u_l
   return c;
 }
 
+// The garbled warps as a polynomial in rw = 1/w.  A garbled sample point is
+// (U, V) = (1 + kx rw, 1 + ky rw) with per-(plane, source) constants kx, ky, and w =
+// h11 u + h12 v + cw, so over a block of pixels it moves along one ray from (1, 1).  Where
+// every sample of the block lies in one texel cell (i, j), the bilinear value there is, with
+// ax = (1 - i) + kx rw and ay = (1 - j) + ky rw,
+//   a + bx ax + ay (by + bxy ax) = c0 + c1 rw + c2 rw^2,
+//   c0 = a + bx px + py B,  c1 = bx kx + py E + ky B,  c2 = ky E,
+//   px = 1 - i, py = 1 - j, B = by + bxy px, E = bxy kx,
+// which a pixel evaluates with two FMAs instead of the coordinates, cell index, fractions,
+// gather and bilinear.  The bilinear surfaces of neighbouring cells agree on their common
+// edge, so a sample that FP32 rounding puts just past the cell is still evaluated to within
+// rounding.  garbled_poly tests the cell, and the warps' validity with the same margins as
+// tile_inside_mask, at the extremes of w over the block (gmin + cw, gmax + cw, where g =
+// h11 u + h12 v over the block's corners; U, V are monotone in w where w > 0), and returns
+// the coefficients.
+struct GarbledPoly {
+  float c0, c1, c2;
+  bool ok;
+};
+__device__ __forceinline__ GarbledPoly garbled_poly(const uint64_t* quad, uint32_t W, float Wm1,
+                                                    float Hm1, float gmin, float gmax, float cw,
+                                                    float kx, float ky) {
+  const float wlo = gmin + cw, whi = gmax + cw;
+  const float rlo = __builtin_amdgcn_rcpf(whi), rhi = __builtin_amdgcn_rcpf(wlo);
+  const float ua = fmaf(kx, rlo, 1.0f), ub = fmaf(kx, rhi, 1.0f);
+  const float va = fmaf(ky, rlo, 1.0f), vb = fmaf(ky, rhi, 1.0f);
+  const float umin = fminf(ua, ub), umax = fmaxf(ua, ub);
+  const float vmin = fminf(va, vb), vmax = fmaxf(va, vb);
+  const float fi = floorf(umin), fj = floorf(vmin);
+  GarbledPoly p;
+  p.ok = wlo >= kInsideW && umin >= kInsideMargin && umax <= Wm1 - kInsideMargin &&
+         vmin >= kInsideMargin && vmax <= Hm1 - kInsideMargin && floorf(umax) == fi &&
+         floorf(vmax) == fj;
+  const uint32_t texel = p.ok ? (uint32_t)fj * W + (uint32_t)fi : 0u;
+  const uint2 t = reinterpret_cast<const uint2*>(quad)[texel];
+  const __half2 lo = *reinterpret_cast<const __half2*>(&t.x);
+  const __half2 hi = *reinterpret_cast<const __half2*>(&t.y);
+  const float a = __low2float(lo), bx = __high2float(lo);
+  const float by = __low2float(hi), bxy = __high2float(hi);
+  const float px = 1.0f - fi, py = 1.0f - fj;
+  const float B = fmaf(bxy, px, by), E = bxy * kx;
+  p.c0 = fmaf(py, B, fmaf(bx, px, a));
+  p.c1 = fmaf(ky, B, fmaf(py, E, bx * kx));
+  p.c2 = ky * E;
+  return p;
+}
+
 __device__ __forceinline__ uint32_t tile_inside_mask(const InsideCorner& c, float idep,
                                                      float Wm1, float Hm1) {
   const float X = fmaf(c.tx, idep, c.X0), Y = fmaf(c.ty, idep, c.Y0);
@@ -266,15 +313,31 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
     const float4* hp = reinterpret_cast<const float4*>(a.hm[lane]);
     ha = hp[0]; hb = hp[1]; hc = hp[2];
   }
+  // The garbled-warp polynomial (garbled_poly) serves the uint8 product forms; the
+  // differentiable form keeps the gathers its adjoint differentiates.
+  constexpr bool kPoly = std::is_same<RT, uint8_t>::value && !CNT;
+  // lane s: h11 u + h12 v over the corners of the strip's 8-row band holding the tile (u over
+  // all 64 lanes).  The band, not the tile, decides where the polynomial is used and fixes its
+  // coefficients, so every tile height (2, 4 or 8 rows) gives the same bits.
+  float gmin = 0.0f, gmax = 0.0f;
+  if (kPoly && lane < a.S) {
+    const float ul = (float)(strip * 62 - 1), uh = (float)(strip * 62 + 62);
+    const float vl = (float)(y0 & ~7), vh = (float)((y0 & ~7) + 7);
+    const float g00 = fmaf(ha.y, vl, ha.x * ul), g10 = fmaf(ha.y, vl, ha.x * uh);
+    const float g01 = fmaf(ha.y, vh, ha.x * ul), g11 = fmaf(ha.y, vh, ha.x * uh);
+    gmin = fminf(fminf(g00, g10), fminf(g01, g11));
+    gmax = fmaxf(fmaxf(g00, g10), fmaxf(g01, g11));
+  }
   for (int dc0 = dbeg; dc0 < dend; dc0 += DC) {
   const int dcn = min(DC, dend - dc0);
   // Per-(plane, source) constants of the warps.  At the start of each plane, lane s computes
-  // source s's record and parks it in the plane's own output slot of the tile (12 floats per
-  // source from the slot's start, so S <= TY * 64 / 12); the plane's results overwrite it once
+  // source s's record and parks it in the plane's own output slot of the tile (16 floats per
+  // source from the slot's start, so S <= TY * 64 / 16); the plane's results overwrite it once
   // the plane is done.  With x = h11 u + h12 v + h13 and (tx, ty, tz) = t idep:
   //   r0 = (h13 + tx, h23 + ty, h33 + tz, h13 + h31 + tz)   centre warps; w of term 4 at u = 0
   //   r1 = (h13 + h11 - h32 + tz, k4x, k4y, k8x)            w of term 8 at u = 0
-  //   r2 = (k8y, -, -, -)
+  //   r2 = (k8y, c0, c1, c2 of term 4)                      garbled_poly coefficients
+  //   r3 = (c0, c1, c2 of term 8, -)
   // Term 4 = (x + h11 + tx, x + h21 + ty, x + h31 + tz) has U = 1 + k4x / w, V = 1 + k4y / w with
   // k4x = h11 - h31 + tx - tz, k4y = h21 - h31 + ty - tz; term 8 = (x + h11 - h12 + tx,
   // x + h11 - h22 + ty, x + h11 - h32 + tz) has k8x = h32 - h12 + tx - tz, k8y = h32 - h22 + ty - tz.
@@ -284,14 +347,31 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
   for (int d = dc0; d < dc0 + dcn; ++d) {
     const float idep = a.idep_min + (float)d * a.dep_sample;
     const int j = d - dc0;
+    bool poly_ok = false;
     if (lane < a.S) {
       const float tx = hc.y * idep, ty = hc.z * idep, tz = hc.w * idep;
       const float dxz = tx - tz, dyz = ty - tz;
-      *rec(j, 3 * lane) = make_float4(ha.z + tx, hb.y + ty, hc.x + tz, (ha.z + hb.z) + tz);
-      *rec(j, 3 * lane + 1) = make_float4((ha.z + (ha.x - hb.w)) + tz, (ha.x - hb.z) + dxz,
-                                          (ha.w - hb.z) + dyz, (hb.w - ha.y) + dxz);
-      *rec(j, 3 * lane + 2) = make_float4((hb.w - hb.x) + dyz, 0.0f, 0.0f, 0.0f);
+      const float4 r0 = make_float4(ha.z + tx, hb.y + ty, hc.x + tz, (ha.z + hb.z) + tz);
+      const float4 r1 = make_float4((ha.z + (ha.x - hb.w)) + tz, (ha.x - hb.z) + dxz,
+                                    (ha.w - hb.z) + dyz, (hb.w - ha.y) + dxz);
+      float4 r2 = make_float4((hb.w - hb.x) + dyz, 0.0f, 0.0f, 0.0f);
+      float4 r3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if constexpr (kPoly) {
+        const GarbledPoly p4 = garbled_poly(a.quad[lane], (uint32_t)W, Wm1, Hm1, gmin, gmax,
+                                            r0.w, r1.y, r1.z);
+        const GarbledPoly p8 = garbled_poly(a.quad[lane], (uint32_t)W, Wm1, Hm1, gmin, gmax,
+                                            r1.x, r1.w, r2.x);
+        poly_ok = p4.ok && p8.ok;
+        r2.y = p4.c0; r2.z = p4.c1; r2.w = p4.c2;
+        r3.x = p8.c0; r3.y = p8.c1; r3.z = p8.c2;
+      }
+      *rec(j, 4 * lane) = r0;
+      *rec(j, 4 * lane + 1) = r1;
+      *rec(j, 4 * lane + 2) = r2;
+      *rec(j, 4 * lane + 3) = r3;
     }
+    // bit s: source s's garbled warps are valid and one polynomial each over the whole tile
+    const uint32_t gpoly = kPoly ? (uint32_t)__ballot(poly_ok) : 0u;
     __builtin_amdgcn_wave_barrier();
     const uint32_t inside = tile_inside_mask(corner, idep, Wm1, Hm1);
     float acc[TY], cnt[TY];
@@ -301,7 +381,7 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
     for (int s = 0; s < a.S; ++s) {
       const float* hm = a.hm[s];
       const float h11 = hm[0], h12 = hm[1], h21 = hm[3], h22 = hm[4], h31 = hm[6], h32 = hm[7];
-      const float4 r0 = *rec(j, 3 * s), r1 = *rec(j, 3 * s + 1), r2 = *rec(j, 3 * s + 2);
+      const float4 r0 = *rec(j, 4 * s), r1 = *rec(j, 4 * s + 1), r2 = *rec(j, 4 * s + 2);
       const QuadTex q(a.quad[s], (uint32_t)W, (uint32_t)H);
       // column parts of H (u, r, 1) + t idep, and of the garbled warps' w
       const float Xu = fmaf(fu, h11, r0.x);
@@ -310,9 +390,13 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
       const float w4u = fmaf(fu, h11, r0.w);
       const float w8u = fmaf(fu, h11, r1.x);
       const float k4x = r1.y, k4y = r1.z, k8x = r1.w, k8y = r2.x;
-      // one pass over the centre rows; INSIDE: every warp of the tile is valid (no tests)
-      auto pass = [&](auto inside_tag) {
+      // one pass over the centre rows; INSIDE: every warp of the tile is valid (no tests);
+      // POLY: the garbled terms are garbled_poly's polynomials (valid on the whole tile)
+      auto pass = [&](auto inside_tag, auto poly_tag) {
         constexpr bool INSIDE = decltype(inside_tag)::value;
+        constexpr bool POLY = decltype(poly_tag)::value;
+        float4 r3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (POLY) r3 = *rec(j, 4 * s + 3);
         auto centre = [&](int k) {   // the centre warp of row y0 - 1 + k
           const float fr = (float)(y0 - 1 + k);
           const float w = fmaf(h32, fr, Zu);
@@ -326,6 +410,10 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
           return gather_uv<INSIDE>(q, (uint32_t)W, Wm1, Hm1, fmaf(kx, rw, 1.0f),
                                    fmaf(ky, rw, 1.0f), w);
         };
+        auto garbled_value = [&](int t, float wu, float c0, float c1, float c2) {
+          const float rw = __builtin_amdgcn_rcpf(fmaf(h12, (float)(y0 + t), wu));
+          return fmaf(fmaf(c2, rw, c1), rw, c0);
+        };
         // Row k's gathers are issued one row ahead (before row k-1's arithmetic), and the
         // scheduling barrier keeps the compiler from sinking them back to their uses: each
         // row's loads are in flight while the previous row computes.
@@ -337,7 +425,7 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
         for (int k = 0; k < TY + 2; ++k) {   // centre row r = y0 - 1 + k
           if (k + 1 < TY + 2) {
             gc[k + 1] = centre(k + 1);
-            if (k >= 1) {
+            if (!POLY && k >= 1) {
               g4[k - 1] = garbled(k - 1, w4u, k4x, k4y);
               g8[k - 1] = garbled(k - 1, w8u, k8x, k8y);
             }
@@ -350,8 +438,10 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
           if (k >= 2) {
             const int t = k - 2;                    // output row v = r - 1
             const float Dn = I[k] - S;              // I(u,r-2) - S(u,r): term 2 of row v
-            const float G4 = IR[t + 1] - gather_value<INSIDE>(g4[t]);
-            const float G8 = IR[t] - gather_value<INSIDE>(g8[t]);
+            const float G4 = IR[t + 1] - (POLY ? garbled_value(t, w4u, r2.y, r2.z, r2.w)
+                                               : gather_value<INSIDE>(g4[t]));
+            const float G8 = IR[t] - (POLY ? garbled_value(t, w8u, r3.x, r3.y, r3.z)
+                                           : gather_value<INSIDE>(g8[t]));
             // terms 5, 3, 7 are the left neighbour's |A| over rows v-1, v, v+1: one column sum
             // moved one lane right
             const float col = fabsf(A0) + fabsf(A1) + fabsf(A2);
@@ -371,12 +461,17 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
           Up0 = Up1; Up1 = Up2;
         }
       };
-      if ((inside >> s) & 1u) {
-        pass(std::true_type{});
-        ++n_inside;
+      const bool ins = (inside >> s) & 1u;
+      if (kPoly && ((gpoly >> s) & 1u)) {
+        if constexpr (kPoly) {
+          if (ins) pass(std::true_type{}, std::true_type{});
+          else pass(std::false_type{}, std::true_type{});
+        }
       } else {
-        pass(std::false_type{});
+        if (ins) pass(std::true_type{}, std::false_type{});
+        else pass(std::false_type{}, std::false_type{});
       }
+      n_inside += ins ? 1 : 0;
     }
 #pragma unroll
     for (int t = 0; t < TY; ++t) cnt[t] += (float)n_inside;
