@@ -307,72 +307,67 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
   const InsideCorner corner =
       tile_inside_setup(a, (float)(strip * 62 - 1), (float)(strip * 62 + 62), (float)(y0 - 1),
                         (float)(y0 + TY), (float)y0, (float)(y0 + TY - 1));
-  // lane s holds source s's (h11, h12, h13, h21), (h22, h23, h31, h32), (h33, t1, t2, t3)
-  float4 ha = make_float4(0.0f, 0.0f, 0.0f, 0.0f), hb = ha, hc = ha;
-  if (lane < a.S) {
-    const float4* hp = reinterpret_cast<const float4*>(a.hm[lane]);
-    ha = hp[0]; hb = hp[1]; hc = hp[2];
-  }
   // The garbled-warp polynomial (garbled_poly) serves the uint8 product forms; the
   // differentiable form keeps the gathers its adjoint differentiates.
   constexpr bool kPoly = std::is_same<RT, uint8_t>::value && !CNT;
-  // lane s: h11 u + h12 v over the corners of the strip's 8-row band holding the tile (u over
-  // all 64 lanes).  The band, not the tile, decides where the polynomial is used and fixes its
-  // coefficients, so every tile height (2, 4 or 8 rows) gives the same bits.
-  float gmin = 0.0f, gmax = 0.0f;
-  if (kPoly && lane < a.S) {
-    const float ul = (float)(strip * 62 - 1), uh = (float)(strip * 62 + 62);
-    const float vl = (float)(y0 & ~7), vh = (float)((y0 & ~7) + 7);
-    const float g00 = fmaf(ha.y, vl, ha.x * ul), g10 = fmaf(ha.y, vl, ha.x * uh);
-    const float g01 = fmaf(ha.y, vh, ha.x * ul), g11 = fmaf(ha.y, vh, ha.x * uh);
-    gmin = fminf(fminf(g00, g10), fminf(g01, g11));
-    gmax = fmaxf(fmaxf(g00, g10), fmaxf(g01, g11));
-  }
+  static_assert((DC & (DC - 1)) == 0, "staging chunk must be a power of two");
   for (int dc0 = dbeg; dc0 < dend; dc0 += DC) {
   const int dcn = min(DC, dend - dc0);
-  // Per-(plane, source) constants of the warps.  At the start of each plane, lane s computes
-  // source s's record and parks it in the plane's own output slot of the tile (16 floats per
-  // source from the slot's start, so S <= TY * 64 / 16); the plane's results overwrite it once
-  // the plane is done.  With x = h11 u + h12 v + h13 and (tx, ty, tz) = t idep:
+  // Per-(plane, source) constants of the warps.  At the start of each chunk of DC planes, the
+  // lanes compute the records of every (source, plane) pair of the chunk and park each in its
+  // plane's own output slot of the tile (16 floats per source from the slot's start, so
+  // S <= TY * 64 / 16); a plane's results overwrite its slot once the plane is done.  With
+  // x = h11 u + h12 v + h13 and (tx, ty, tz) = t idep:
   //   r0 = (h13 + tx, h23 + ty, h33 + tz, h13 + h31 + tz)   centre warps; w of term 4 at u = 0
   //   r1 = (h13 + h11 - h32 + tz, k4x, k4y, k8x)            w of term 8 at u = 0
   //   r2 = (k8y, c0, c1, c2 of term 4)                      garbled_poly coefficients
-  //   r3 = (c0, c1, c2 of term 8, -)
+  //   r3 = (c0, c1, c2 of term 8, 1 if both polynomials hold on the whole band else 0)
   // Term 4 = (x + h11 + tx, x + h21 + ty, x + h31 + tz) has U = 1 + k4x / w, V = 1 + k4y / w with
   // k4x = h11 - h31 + tx - tz, k4y = h21 - h31 + ty - tz; term 8 = (x + h11 - h12 + tx,
   // x + h11 - h22 + ty, x + h11 - h32 + tz) has k8x = h32 - h12 + tx - tz, k8y = h32 - h22 + ty - tz.
   auto rec = [&](int j, int f) {   // float4 f of plane slot j
     return reinterpret_cast<float4*>(tile + ((f >> 4) * DC + j) * 64 + (f & 15) * 4);
   };
+  for (int p = lane; p < a.S * DC; p += 64) {   // pair p: source p / DC, plane dc0 + p % DC
+    const int s = p / DC, j = p % DC;
+    if (j >= dcn) continue;
+    // (h11, h12, h13, h21), (h22, h23, h31, h32), (h33, t1, t2, t3)
+    const float4* hp = reinterpret_cast<const float4*>(a.hm[s]);
+    const float4 ha = hp[0], hb = hp[1], hc = hp[2];
+    const float idep = a.idep_min + (float)(dc0 + j) * a.dep_sample;
+    const float tx = hc.y * idep, ty = hc.z * idep, tz = hc.w * idep;
+    const float dxz = tx - tz, dyz = ty - tz;
+    const float4 r0 = make_float4(ha.z + tx, hb.y + ty, hc.x + tz, (ha.z + hb.z) + tz);
+    const float4 r1 = make_float4((ha.z + (ha.x - hb.w)) + tz, (ha.x - hb.z) + dxz,
+                                  (ha.w - hb.z) + dyz, (hb.w - ha.y) + dxz);
+    float4 r2 = make_float4((hb.w - hb.x) + dyz, 0.0f, 0.0f, 0.0f);
+    float4 r3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (kPoly) {
+      // h11 u + h12 v over the corners of the strip's 8-row band holding the tile (u over all
+      // 64 lanes).  The band, not the tile, decides where the polynomial is used and fixes its
+      // coefficients, so every tile height (2, 4 or 8 rows) gives the same bits.
+      const float ul = (float)(strip * 62 - 1), uh = (float)(strip * 62 + 62);
+      const float vl = (float)(y0 & ~7), vh = (float)((y0 & ~7) + 7);
+      const float g00 = fmaf(ha.y, vl, ha.x * ul), g10 = fmaf(ha.y, vl, ha.x * uh);
+      const float g01 = fmaf(ha.y, vh, ha.x * ul), g11 = fmaf(ha.y, vh, ha.x * uh);
+      const float gmin = fminf(fminf(g00, g10), fminf(g01, g11));
+      const float gmax = fmaxf(fmaxf(g00, g10), fmaxf(g01, g11));
+      const GarbledPoly p4 = garbled_poly(a.quad[s], (uint32_t)W, Wm1, Hm1, gmin, gmax, r0.w,
+                                          r1.y, r1.z);
+      const GarbledPoly p8 = garbled_poly(a.quad[s], (uint32_t)W, Wm1, Hm1, gmin, gmax, r1.x,
+                                          r1.w, r2.x);
+      r2.y = p4.c0; r2.z = p4.c1; r2.w = p4.c2;
+      r3 = make_float4(p8.c0, p8.c1, p8.c2, (p4.ok && p8.ok) ? 1.0f : 0.0f);
+    }
+    *rec(j, 4 * s) = r0;
+    *rec(j, 4 * s + 1) = r1;
+    *rec(j, 4 * s + 2) = r2;
+    *rec(j, 4 * s + 3) = r3;
+  }
+  __builtin_amdgcn_wave_barrier();
   for (int d = dc0; d < dc0 + dcn; ++d) {
     const float idep = a.idep_min + (float)d * a.dep_sample;
     const int j = d - dc0;
-    bool poly_ok = false;
-    if (lane < a.S) {
-      const float tx = hc.y * idep, ty = hc.z * idep, tz = hc.w * idep;
-      const float dxz = tx - tz, dyz = ty - tz;
-      const float4 r0 = make_float4(ha.z + tx, hb.y + ty, hc.x + tz, (ha.z + hb.z) + tz);
-      const float4 r1 = make_float4((ha.z + (ha.x - hb.w)) + tz, (ha.x - hb.z) + dxz,
-                                    (ha.w - hb.z) + dyz, (hb.w - ha.y) + dxz);
-      float4 r2 = make_float4((hb.w - hb.x) + dyz, 0.0f, 0.0f, 0.0f);
-      float4 r3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if constexpr (kPoly) {
-        const GarbledPoly p4 = garbled_poly(a.quad[lane], (uint32_t)W, Wm1, Hm1, gmin, gmax,
-                                            r0.w, r1.y, r1.z);
-        const GarbledPoly p8 = garbled_poly(a.quad[lane], (uint32_t)W, Wm1, Hm1, gmin, gmax,
-                                            r1.x, r1.w, r2.x);
-        poly_ok = p4.ok && p8.ok;
-        r2.y = p4.c0; r2.z = p4.c1; r2.w = p4.c2;
-        r3.x = p8.c0; r3.y = p8.c1; r3.z = p8.c2;
-      }
-      *rec(j, 4 * lane) = r0;
-      *rec(j, 4 * lane + 1) = r1;
-      *rec(j, 4 * lane + 2) = r2;
-      *rec(j, 4 * lane + 3) = r3;
-    }
-    // bit s: source s's garbled warps are valid and one polynomial each over the whole tile
-    const uint32_t gpoly = kPoly ? (uint32_t)__ballot(poly_ok) : 0u;
-    __builtin_amdgcn_wave_barrier();
     const uint32_t inside = tile_inside_mask(corner, idep, Wm1, Hm1);
     float acc[TY], cnt[TY];
     int n_inside = 0;   // sources that took the inside pass (each counts on every row)
@@ -462,7 +457,10 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
         }
       };
       const bool ins = (inside >> s) & 1u;
-      if (kPoly && ((gpoly >> s) & 1u)) {
+      // the record's polynomial flag, read by every lane from one LDS word: wave-uniform
+      const bool gp =
+          kPoly && __builtin_amdgcn_readfirstlane(__float_as_uint(rec(j, 4 * s + 3)->w)) != 0u;
+      if (gp) {
         if constexpr (kPoly) {
           if (ins) pass(std::true_type{}, std::true_type{});
           else pass(std::false_type{}, std::true_type{});
