@@ -20,17 +20,16 @@
 #include "common.hpp"
 #include "cost_tile.hpp"
 
-#ifndef PM_COST_TILE_ROWS
-#define PM_COST_TILE_ROWS 4
-#endif
-#ifndef PM_COST_STAGE_PLANES
-#define PM_COST_STAGE_PLANES 8
-#endif
-
 namespace pm {
 
 // Tile count below which a single-keyframe launch splits each tile's planes over two blocks.
 constexpr long kCostSplitTiles = 4096;
+
+// Single-keyframe launch shape (tile rows, LDS staging planes) per cost storage type, measured
+// on 752x480x64 keyframes with 5 sources: FP32 0.234 ms with 4 x 8, 0.223 ms with 8 x 4;
+// FP16 0.219 ms with 4 x 8, 0.220 ms with 8 x 4.
+template <typename CT> struct SingleShape { static constexpr int TY = 4, DC = 8; };
+template <> struct SingleShape<float> { static constexpr int TY = 8, DC = 4; };
 
 __global__ __launch_bounds__(256) void make_quad_kernel(const uint8_t* __restrict__ img,
                                                         uint64_t* __restrict__ quad, int W,
@@ -130,6 +129,20 @@ bool launch_cost_batch(pm_cost_type ct, int tile_rows, int stage_planes, const K
 
 using namespace pm;
 
+namespace pm {
+// One keyframe's tiles are about 1.5 rounds of resident blocks at 752x480: splitting each
+// tile's planes over several blocks (4 planes per wave at the least, 4-aligned) leaves a short
+// tail.  Measured at 752x480x64 with 4-row tiles: 0.32 ms unsplit, 0.289 ms in two, 0.284 ms
+// in four.
+template <typename CT>
+void launch_cost_single(const CostArgs& a, CT* cost, hipStream_t st) {
+  constexpr int TY = SingleShape<CT>::TY, DC = SingleShape<CT>::DC;
+  const long tiles = cost_tiles<TY>(a.W, a.H);
+  const int npg = tiles >= kCostSplitTiles ? 1 : (a.D % 64 == 0 ? 4 : (a.D % 32 == 0 ? 2 : 1));
+  cost_tile_kernel<CT, TY, DC><<<(unsigned)(tiles * npg), 256, 0, st>>>(a, cost, npg);
+}
+}  // namespace pm
+
 extern "C" pm_status pm_make_quad(const uint8_t* image, uint64_t* quad, int32_t width,
                                   int32_t height, pm_stream stream) {
   clear_error();
@@ -170,25 +183,16 @@ extern "C" pm_status pm_calc_cost(pm_shape shape, const uint8_t* ref, int32_t nu
       if (!std::isfinite(a.hm[s][k]))
         return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: non-finite homography");
   }
-  constexpr int TY = PM_COST_TILE_ROWS;
-  constexpr int DC = PM_COST_STAGE_PLANES;
-  const long tiles = cost_tiles<TY>(a.W, a.H);
-  // one keyframe's tiles are about 1.5 rounds of resident blocks at 752x480: splitting each
-  // tile's planes over several blocks (4 planes per wave at the least, 4-aligned) leaves a
-  // short tail.  Measured at 752x480x64: 0.32 ms unsplit, 0.289 ms in two, 0.284 ms in four.
-  const int npg = tiles >= kCostSplitTiles ? 1 : (a.D % 64 == 0 ? 4 : (a.D % 32 == 0 ? 2 : 1));
-  const unsigned blocks = (unsigned)(tiles * npg);
   hipStream_t st = (hipStream_t)stream;
   switch (cost_type) {
     case PM_COST_F32:
-      cost_tile_kernel<float, TY, DC><<<blocks, 256, 0, st>>>(a, (float*)cost, npg);
+      launch_cost_single<float>(a, (float*)cost, st);
       break;
     case PM_COST_F16:
-      cost_tile_kernel<__half, TY, DC><<<blocks, 256, 0, st>>>(a, (__half*)cost, npg);
+      launch_cost_single<__half>(a, (__half*)cost, st);
       break;
     case PM_COST_BF16:
-      cost_tile_kernel<__hip_bfloat16, TY, DC><<<blocks, 256, 0, st>>>(a, (__hip_bfloat16*)cost,
-                                                                        npg);
+      launch_cost_single<__hip_bfloat16>(a, (__hip_bfloat16*)cost, st);
       break;
     default:
       return fail(PM_ERR_INVALID_ARGUMENT, "pm_calc_cost: unknown cost type");
