@@ -288,8 +288,13 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
   const int y0 = (blk / nstrips) * TY;
   const int W = a.W, H = a.H, D = a.D;
   const int u = strip * 62 - 1 + lane;
+  // The block's 4 dper planes (dper per wave) go in rounds of chunks: in round c the four waves
+  // take four adjacent chunks, so their stores of a pixel's planes land side by side at about
+  // the same time and leave the L2 as whole write granules (with each wave on a contiguous
+  // range of its own, FP16 chunks of 4 planes went out as 8-byte pieces: 4x the volume's bytes).
   const int dper = (D + 4 * npg - 1) / (4 * npg);
-  const int dbeg = (pg * 4 + wave) * dper, dend = min(D, dbeg + dper);
+  const int bbeg = pg * 4 * dper, bend = min(D, bbeg + 4 * dper);
+  const int nfull = dper / DC, rem = dper - nfull * DC;
   const float Wm1 = (float)(W - 1), Hm1 = (float)(H - 1);
   const float INF = __uint_as_float(0x7F800000u);
   const float fu = (float)u;
@@ -311,8 +316,11 @@ __device__ __forceinline__ void cost_tile(const CostArgs& a, int blk, CT* __rest
   // differentiable form keeps the gathers its adjoint differentiates.
   constexpr bool kPoly = std::is_same<RT, uint8_t>::value && !CNT;
   static_assert((DC & (DC - 1)) == 0, "staging chunk must be a power of two");
-  for (int dc0 = dbeg; dc0 < dend; dc0 += DC) {
-  const int dcn = min(DC, dend - dc0);
+  for (int c = 0; c < nfull + (rem > 0 ? 1 : 0); ++c) {
+  const int csz = c < nfull ? DC : rem;
+  const int dc0 = bbeg + c * 4 * DC + wave * csz;
+  const int dcn = min(csz, bend - dc0);
+  if (dcn <= 0) break;
   // Per-(plane, source) constants of the warps.  At the start of each chunk of DC planes, the
   // lanes compute the records of every (source, plane) pair of the chunk and park each in its
   // plane's own output slot of the tile (16 floats per source from the slot's start, so
