@@ -51,7 +51,7 @@ COST_BYTES = {"f32": 4, "f16": 2, "bf16": 2}
 #   * SGM_PASS_TRAFFIC: HBM bytes of the SGM pass (paths launch + combine/filter launch),
 #     FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 in bytes (scales from a device copy of known size);
 COUNTER_PROFILE = "profiles/counters_752x480x64_s5_p8.md"
-SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2034e9,
+SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2033e9,
                     ("batched", "f16", "compact"): 1.1427e9}
 #   * COST_VALU: the cost launch's SQ_INSTS_VALU (wave64 instructions) and the fraction of the
 #     chip's VALU issue slots they fill over the launch's duration in the profiled run (at the
@@ -60,8 +60,8 @@ SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2034e9,
 #     launches) per replay, default keyframe set (UNEVEN_PROFILE).
 UNEVEN_PROFILE = "profiles/uneven_batch_16kf_f16.md"
 UNEVEN_SGM_TRAFFIC = 21.306e9
-COST_VALU = {("plain", "f32", "exact"): (154.75e6, 0.911),
-             ("batched", "f16", "compact"): (151.62e6, 0.957)}
+COST_VALU = {("plain", "f32", "exact"): (112.98e6, 0.819),
+             ("batched", "f16", "compact"): (112.27e6, 0.936)}
 
 
 def parse():

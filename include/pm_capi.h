@@ -289,7 +289,7 @@ typedef struct pm_keyframe {
 } pm_keyframe;
 
 /* Launch shapes (picked by pm_batch_autotune): cost tile rows 2, 4 or 8 with an LDS staging
- * chunk of 8 or 16 planes (4 or 8 with 8 rows; 2 rows only when no keyframe has more than 10
+ * chunk of 8 or 16 planes (4 or 8 with 8 rows; 2 rows only when no keyframe has more than 8
  * sources), SGM prefetch depth 8 or 16 steps, and persistent grids of blocks_per_cu x
  * (compute units) blocks (0 = one block per work item).  cost_plane_split 1 splits each cost
  * tile's planes into groups of 32 (keyframes whose plane count is a multiple of 16), 0 keeps

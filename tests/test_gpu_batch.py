@@ -227,15 +227,23 @@ def test_batch_rejects_bad_input(uneven):
         pipeline.Batch([dict(uneven[0], planes=129)], "f16", KNOBS, 8, "sweep")
 
 
-def test_batch_max_sources_tile_rows(uneven):
-    """A keyframe with 16 sources: 2-row cost tiles hold the warp records of at most 10
-    sources, so that shape is refused (and skipped by the autotuner); 4 and 8 rows run it."""
-    from probabilistic_mapping_amd import _capi as C, pipeline
-    kf = dict(uneven[3])
+def _with_sources(kf, count):
+    """The keyframe with its sources repeated up to ``count``, each repeat with its own
+    translation scale."""
+    kf = dict(kf)
     n = len(kf["quads"])
-    kf["quads"] = [kf["quads"][i % n] for i in range(16)]
+    kf["quads"] = [kf["quads"][i % n] for i in range(count)]
     kf["homs"] = [(h, (t * (1.0 + 0.03 * i)).astype(np.float32))
-                  for i, (h, t) in enumerate(kf["homs"][i % n] for i in range(16))]
+                  for i, (h, t) in enumerate(kf["homs"][i % n] for i in range(count))]
+    return kf
+
+
+def test_batch_max_sources_tile_rows(uneven):
+    """A keyframe with 16 sources: 2-row cost tiles hold the warp records of at most 8
+    sources, so that shape is refused (and skipped by the autotuner); 4 and 8 rows run it.
+    With 9 sources 2 rows are refused too; with 8 they run and give the same bits."""
+    from probabilistic_mapping_amd import _capi as C, pipeline
+    kf = _with_sources(uneven[3], 16)
     expect = [_single(pipeline, kf, "f16", 8)]
     b = pipeline.Batch([kf], "f16", KNOBS, 8)
     with pytest.raises(C.PmError):
@@ -247,6 +255,15 @@ def test_batch_max_sources_tile_rows(uneven):
         _assert_equal(b, expect)
     best = b.autotune(reps=1)
     assert best["cost_tile_rows"] != 2
+    with pytest.raises(C.PmError):
+        pipeline.Batch([_with_sources(uneven[3], 9)], "f16", KNOBS, 8).tuning = dict(
+            cost_tile_rows=2, cost_stage_planes=8)
+    kf8 = _with_sources(uneven[3], 8)
+    b8 = pipeline.Batch([kf8], "f16", KNOBS, 8)
+    b8.tuning = dict(cost_tile_rows=2, cost_stage_planes=8)
+    b8.run()
+    torch.cuda.synchronize()
+    _assert_equal(b8, [_single(pipeline, kf8, "f16", 8)])
 
 
 def test_batched_sequence_groups_match_one_batch(trajectory_scene):
