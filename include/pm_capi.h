@@ -496,7 +496,10 @@ pm_status pm_diff_memory(const pm_diff* h, uint64_t* bytes);
 pm_status pm_diff_forward(pm_diff* h, const float* ref, const uint64_t* const* src_quads,
                           const pm_homography* homs, float idep_min, float dep_sample,
                           const float* gt, float* depth, double* loss, pm_stream stream);
-/* grad_ref: device float [H][W]; grad_homs: device double [S][12]; both overwritten. */
+/* grad_ref: device float [H][W]; grad_homs: device double [S][12]; both overwritten.  With
+ * recompute = 0 the backward consumes the forward's stored volumes (the aggregate becomes dL/dagg
+ * and the per-path L volumes the per-path adjoints): a second backward needs another forward
+ * (PM_ERR_INVALID_STATE otherwise).  With recompute = 1 it recomputes them and may run again. */
 pm_status pm_diff_backward(pm_diff* h, float grad_loss, float* grad_ref, double* grad_homs,
                            pm_stream stream);
 

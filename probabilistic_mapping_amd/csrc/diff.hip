@@ -85,6 +85,77 @@ __global__ __launch_bounds__(256) void sgm_diff_kernel(const SgmArgs a, const fl
   sgm_scanline<NV, float, FIRST, FULL, 16 / NV, float, AGG, LOUT>(a, line, cost, agg);
 }
 
+// Every path of the float-reference SGM in one launch (store mode).  Global line g counts the
+// lines of path 0, then path 1, ...; path r writes its L (unmasked: the backward routes
+// through it) to lbuf + r V, so no two waves touch the same output and all paths run at once.
+// agg_sum_kernel then forms the aggregate.
+template <int NV, bool FULL>
+__global__ __launch_bounds__(256) void sgm_diff_paths_kernel(const SgmArgs base, int paths,
+                                                             const float* __restrict__ cost,
+                                                             float* __restrict__ lbuf) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int g = blockIdx.x * 4 + wave, path = 0, n = 0;
+  for (; path < paths; ++path, g -= n) {
+    n = sgm_lines(base.W, base.H, path_dx(path), path_dy(path));
+    if (g < n) break;
+  }
+  if (path == paths) return;
+  SgmArgs a = base;
+  a.dx = path_dx(path);
+  a.dy = path_dy(path);
+  a.nlines = n;
+  a.lout = lbuf + (size_t)path * ((size_t)base.W * base.H * base.D);
+  sgm_scanline<NV, float, true, FULL, 16 / NV, float, false, true>(a, g, cost, nullptr);
+}
+
+// agg = the paths' L volumes summed in path order, 0 at a pixel with a negative cost on any
+// plane: what the path-after-path accumulation (sgm.cu:101-108, sgm_diff_kernel with AGG)
+// leaves, in the same order.  16 lanes per pixel, lane k holding planes k, k+16, ...
+template <int PPL>
+__global__ __launch_bounds__(256) void agg_sum_kernel(long N, int D, int paths,
+                                                      const float* __restrict__ cost,
+                                                      const float* __restrict__ lbuf,
+                                                      float* __restrict__ agg) {
+  const long V = N * D;
+  const long p = ((long)blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int k = threadIdx.x & 15;
+  if (p >= N) return;   // the whole 16-lane group
+  float sum[PPL];
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    const int d = k + 16 * j;
+    sum[j] = 0.0f;
+    if (d < D) {
+      const long e = p * D + d;
+      bad = bad || cost[e] < 0.0f;
+      float v = lbuf[e];
+      for (int r = 1; r < paths; ++r) v += lbuf[r * V + e];
+      sum[j] = v;
+    }
+  }
+  float b = bad ? 1.0f : 0.0f;
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) b = fmaxf(b, __shfl_xor(b, off, 16));
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    const int d = k + 16 * j;
+    if (d < D) agg[p * D + d] = b > 0.0f ? kCostInf : sum[j];
+  }
+}
+
+// dL/dC = the paths' adjoint volumes summed in path order (each already 0 at invalid pixels):
+// what the path-after-path accumulation of sgm_adjoint_kernel leaves, in the same order.
+__global__ __launch_bounds__(256) void grad_cost_sum_kernel(long V, int paths,
+                                                            const float* __restrict__ gbuf,
+                                                            float* __restrict__ gC) {
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < V; e += (long)gridDim.x * 256) {
+    float v = gbuf[e];
+    for (int r = 1; r < paths; ++r) v += gbuf[r * V + e];
+    gC[e] = v;
+  }
+}
+
 // 16 lanes per pixel, lane k holding planes k, k+16, ...; four pixels per wave.
 struct ReadoutArgs {
   long N;
@@ -248,12 +319,13 @@ struct SgmAdjArgs {
   float* gC;            // dL/dC, accumulated over paths
 };
 
+// FIRST: the adjoint is written to A.gC (0 at invalid pixels), else added to it.  With FIRST,
+// A.gC may be A.L itself: step k writes pixel p_k after reading L at p_k's predecessor, and
+// the loads ahead (lower k) read predecessors below the block being written, so every L value
+// is read before it is overwritten.
 template <int NV, bool FIRST, bool FULL, int B>
-__global__ __launch_bounds__(256) void sgm_adjoint_kernel(const SgmAdjArgs A) {
+__device__ __forceinline__ void sgm_adjoint_scanline(const SgmAdjArgs& A, int line) {
   const SgmArgs& a = A.s;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int line = blockIdx.x * 4 + wave;
-  if (line >= a.nlines) return;
   const int lane = threadIdx.x & 63;
   const float* __restrict__ ref = static_cast<const float*>(a.ref);
   int x, y;
@@ -388,6 +460,37 @@ __global__ __launch_bounds__(256) void sgm_adjoint_kernel(const SgmAdjArgs A) {
       if (base - i >= 0) step(base - i, i, ga1, lp1, gc1, vm1, sm1);
     base = nb2;
   }
+}
+
+// One path's reverse sweep, accumulated into A.gC (recompute mode: path after path).
+template <int NV, bool FIRST, bool FULL, int B>
+__global__ __launch_bounds__(256) void sgm_adjoint_kernel(const SgmAdjArgs A) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int line = blockIdx.x * 4 + wave;
+  if (line >= A.s.nlines) return;
+  sgm_adjoint_scanline<NV, FIRST, FULL, B>(A, line);
+}
+
+// Every path's reverse sweep in one launch (store mode): path r reads its L from
+// base.L + r V and writes its adjoint over it (grad_cost_sum_kernel then sums them).
+template <int NV, bool FULL, int B>
+__global__ __launch_bounds__(256) void sgm_adjoint_paths_kernel(const SgmAdjArgs base,
+                                                                int paths) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int g = blockIdx.x * 4 + wave, path = 0, n = 0;
+  for (; path < paths; ++path, g -= n) {
+    n = sgm_lines(base.s.W, base.s.H, path_dx(path), path_dy(path));
+    if (g < n) break;
+  }
+  if (path == paths) return;
+  SgmAdjArgs A = base;
+  A.s.dx = path_dx(path);
+  A.s.dy = path_dy(path);
+  A.s.nlines = n;
+  float* vol = const_cast<float*>(base.L) + (size_t)path * ((size_t)base.s.W * base.s.H * base.s.D);
+  A.L = vol;
+  A.gC = vol;
+  sgm_adjoint_scanline<NV, true, FULL, B>(A, g);
 }
 
 // Cost adjoint.  Tiles as in the forward (lanes = columns x0-1 .. x0+62, TY output rows, the
@@ -640,6 +743,45 @@ pm_status launch_sgm_adjoint(const SgmAdjArgs& A, bool first, hipStream_t st) {
   return check_launch("sgm_adjoint_kernel");
 }
 
+template <int NV, bool FULL>
+pm_status launch_sgm_adjoint_paths(const SgmAdjArgs& A, int paths, hipStream_t st) {
+  int lines = 0;
+  for (int path = 0; path < paths; ++path)
+    lines += sgm_lines(A.s.W, A.s.H, kPathDx[path], kPathDy[path]);
+  constexpr int B = 8 / NV;
+  sgm_adjoint_paths_kernel<NV, FULL, B><<<(unsigned)((lines + 3) / 4), 256, 0, st>>>(A, paths);
+  return check_launch("sgm_adjoint_paths_kernel");
+}
+
+pm_status sgm_adjoint_paths(const SgmAdjArgs& A, int paths, hipStream_t st) {
+  const int D = A.s.D;
+  if (D == 64) return launch_sgm_adjoint_paths<1, true>(A, paths, st);
+  if (D < 64) return launch_sgm_adjoint_paths<1, false>(A, paths, st);
+  if (D == 128) return launch_sgm_adjoint_paths<2, true>(A, paths, st);
+  if (D < 128) return launch_sgm_adjoint_paths<2, false>(A, paths, st);
+  return launch_sgm_adjoint_paths<4, false>(A, paths, st);
+}
+
+template <int NV, bool FULL>
+pm_status launch_sgm_diff_paths(const SgmArgs& a, int paths, const float* cost, float* lbuf,
+                                hipStream_t st) {
+  int lines = 0;
+  for (int path = 0; path < paths; ++path)
+    lines += sgm_lines(a.W, a.H, kPathDx[path], kPathDy[path]);
+  sgm_diff_paths_kernel<NV, FULL><<<(unsigned)((lines + 3) / 4), 256, 0, st>>>(a, paths, cost,
+                                                                               lbuf);
+  return check_launch("sgm_diff_paths_kernel");
+}
+
+pm_status sgm_diff_paths(const SgmArgs& a, int paths, const float* cost, float* lbuf,
+                         hipStream_t st) {
+  if (a.D == 64) return launch_sgm_diff_paths<1, true>(a, paths, cost, lbuf, st);
+  if (a.D < 64) return launch_sgm_diff_paths<1, false>(a, paths, cost, lbuf, st);
+  if (a.D == 128) return launch_sgm_diff_paths<2, true>(a, paths, cost, lbuf, st);
+  if (a.D < 128) return launch_sgm_diff_paths<2, false>(a, paths, cost, lbuf, st);
+  return launch_sgm_diff_paths<4, false>(a, paths, cost, lbuf, st);
+}
+
 pm_status sgm_adjoint(const SgmAdjArgs& A, bool first, hipStream_t st) {
   const int D = A.s.D;
   if (D == 64) return launch_sgm_adjoint<1, true>(A, first, st);
@@ -728,9 +870,20 @@ void dispatch_ppl(int D, F&& f) {
   else f(std::integral_constant<int, 16>());
 }
 
-// Runs every path of the SGM; agg: accumulate the aggregate; store: write path r's L to
-// h->L + r V.
+// Runs every path of the SGM into the aggregate; store: also write path r's L to h->L + r V,
+// all paths in one launch and the aggregate summed from their volumes afterwards (without
+// store, path after path, accumulating the aggregate).
 pm_status run_sgm_forward(pm_diff* h, bool store, hipStream_t st) {
+  if (store) {
+    pm_status s = sgm_diff_paths(diff_sgm_args(h, 0), h->sgm.paths, h->cost, h->L, st);
+    if (s != PM_OK) return s;
+    const unsigned blocks = (unsigned)((h->N * 16 + 255) / 256);
+    dispatch_ppl(h->shape.planes, [&](auto ppl) {
+      agg_sum_kernel<decltype(ppl)::value><<<blocks, 256, 0, st>>>(
+          h->N, h->shape.planes, h->sgm.paths, h->cost, h->L, h->agg);
+    });
+    return check_launch("agg_sum_kernel");
+  }
   for (int path = 0; path < h->sgm.paths; ++path) {
     SgmArgs a = diff_sgm_args(h, path);
     a.lout = store ? h->L + (size_t)path * h->V : nullptr;
@@ -900,21 +1053,35 @@ extern "C" pm_status pm_diff_backward(pm_diff* h, float grad_loss, float* grad_r
                [&](auto ppl) { launch_readout_adjoint<decltype(ppl)::value>(r, h->agg, st); });
   s = check_launch("readout_adjoint_kernel");
   if (s != PM_OK) return s;
-  for (int path = 0; path < h->sgm.paths; ++path) {
-    SgmAdjArgs A{};
-    A.s = diff_sgm_args(h, path);
-    if (h->params.recompute) {   // this path's L
+  if (h->params.recompute) {
+    // path after path: this path's L again, then its reverse sweep added into dL/dC
+    for (int path = 0; path < h->sgm.paths; ++path) {
+      SgmAdjArgs A{};
+      A.s = diff_sgm_args(h, path);
       SgmArgs a = A.s;
       a.lout = h->L;
       s = sgm_diff_path(a, true, false, true, h->cost, nullptr, st);
       if (s != PM_OK) return s;
       A.L = h->L;
-    } else {
-      A.L = h->L + (size_t)path * h->V;
+      A.gagg = h->agg; A.pvalid = h->pvalid; A.gC = h->gC;
+      s = sgm_adjoint(A, path == 0, st);
+      if (s != PM_OK) return s;
     }
-    A.gagg = h->agg; A.pvalid = h->pvalid; A.gC = h->gC;
-    s = sgm_adjoint(A, path == 0, st);
+  } else {
+    // every path's reverse sweep at once, each over its own L volume, then their sum
+    SgmAdjArgs A{};
+    A.s = diff_sgm_args(h, 0);
+    A.L = h->L;
+    A.gagg = h->agg; A.pvalid = h->pvalid; A.gC = nullptr;
+    s = sgm_adjoint_paths(A, h->sgm.paths, st);
     if (s != PM_OK) return s;
+    grad_cost_sum_kernel<<<(unsigned)std::min<long>((h->V + 255) / 256, 65536), 256, 0, st>>>(
+        h->V, h->sgm.paths, h->L, h->gC);
+    s = check_launch("grad_cost_sum_kernel");
+    if (s != PM_OK) return s;
+    // the aggregate now holds dL/dagg and the path volumes hold adjoints: another backward
+    // needs another forward
+    h->have_forward = false;
   }
   CostAdjArgs C{};
   C.c = h->c; C.gC = h->gC; C.cnt = h->cnt; C.refpart = h->refpart; C.posepart = h->posepart;

@@ -99,7 +99,9 @@ class SoftArgminDepth:
         return depth, loss
 
     def backward(self, grad_loss=1.0):
-        """Returns (dL/dref [H, W] float32, dL/d(H, t) [S, 12] float64), on the device."""
+        """Returns (dL/dref [H, W] float32, dL/d(H, t) [S, 12] float64), on the device.  In the
+        store mode it consumes the forward's stored volumes: another backward needs another
+        forward."""
         H, W, _ = self.shape
         dev = self._keep[0].device
         gref = torch.empty((H, W), dtype=torch.float32, device=dev)

@@ -199,6 +199,33 @@ def test_store_and_recompute_agree(medium):
     assert a["loss"] == b["loss"]
 
 
+def test_backward_after_backward(medium):
+    """Store mode: the backward consumes the forward's volumes (the aggregate becomes dL/dagg,
+    the path volumes the path adjoints), so a second backward is refused until the next
+    forward, which then gives the same gradients again.  Recompute mode rebuilds them: a second
+    backward runs and repeats the first bit for bit."""
+    from probabilistic_mapping_amd import _capi as C
+    s, g, _ = medium
+    args = (g["ref"], g["quads"], g["homs"], s["idep_min"], g["ds"], g["gt"])
+    op = _make_op(s, recompute=False)
+    op.forward(*args)
+    first = [t.cpu().numpy() for t in op.backward(1.0)]
+    with pytest.raises(C.PmError):
+        op.backward(1.0)
+    op.forward(*args)
+    again = [t.cpu().numpy() for t in op.backward(1.0)]
+    op.close()
+    for a, b in zip(first, again):
+        assert np.array_equal(a, b)
+    op = _make_op(s, recompute=True)
+    op.forward(*args)
+    r1 = [t.cpu().numpy() for t in op.backward(1.0)]
+    r2 = [t.cpu().numpy() for t in op.backward(1.0)]
+    op.close()
+    for a, b, c in zip(r1, r2, first):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
 def test_autograd_function(medium):
     """DepthLoss through torch.autograd: the pose gradient matches the restatement's autograd
     through the FP64 matrix exponential; the intensity gradient is the adjoint's."""
