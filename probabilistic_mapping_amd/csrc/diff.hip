@@ -144,12 +144,68 @@ __global__ __launch_bounds__(256) void agg_sum_kernel(long N, int D, int paths,
   }
 }
 
+// The same with D a multiple of 4: lane k of a pixel's 16 sums the 4-plane chunks k, k+16, ...
+// with 16-byte loads, so a group reads whole 256-byte runs of the pixel's planes (per-plane
+// lanes read each 128-byte line in two halves at different times: 1.7x the bytes).
+template <int CPL>
+__global__ __launch_bounds__(256) void agg_sum4_kernel(long N, int D, int paths,
+                                                       const float* __restrict__ cost,
+                                                       const float* __restrict__ lbuf,
+                                                       float* __restrict__ agg) {
+  const long V = N * D;
+  const int D4 = D / 4;
+  const long p = ((long)blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int k = threadIdx.x & 15;
+  if (p >= N) return;   // the whole 16-lane group
+  float4 sum[CPL];
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    const int c = k + 16 * j;
+    sum[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (c < D4) {
+      const long e = p * D + 4 * c;
+      const float4 cv = *reinterpret_cast<const float4*>(cost + e);
+      bad = bad || cv.x < 0.0f || cv.y < 0.0f || cv.z < 0.0f || cv.w < 0.0f;
+      float4 v = *reinterpret_cast<const float4*>(lbuf + e);
+      for (int r = 1; r < paths; ++r) {
+        const float4 w = *reinterpret_cast<const float4*>(lbuf + r * V + e);
+        v.x += w.x; v.y += w.y; v.z += w.z; v.w += w.w;
+      }
+      sum[j] = v;
+    }
+  }
+  float b = bad ? 1.0f : 0.0f;
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) b = fmaxf(b, __shfl_xor(b, off, 16));
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    const int c = k + 16 * j;
+    if (c < D4)
+      *reinterpret_cast<float4*>(agg + p * D + 4 * c) =
+          b > 0.0f ? make_float4(kCostInf, kCostInf, kCostInf, kCostInf) : sum[j];
+  }
+}
+
 // dL/dC = the paths' adjoint volumes summed in path order (each already 0 at invalid pixels):
 // what the path-after-path accumulation of sgm_adjoint_kernel leaves, in the same order.
-__global__ __launch_bounds__(256) void grad_cost_sum_kernel(long V, int paths,
+// Element e4 of the float4 view; V is a multiple of 4 when vec4 is set.
+__global__ __launch_bounds__(256) void grad_cost_sum_kernel(long V, int paths, bool vec4,
                                                             const float* __restrict__ gbuf,
                                                             float* __restrict__ gC) {
-  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < V; e += (long)gridDim.x * 256) {
+  const long stride = (long)gridDim.x * 256;
+  if (vec4) {
+    for (long e = ((long)blockIdx.x * 256 + threadIdx.x) * 4; e < V; e += stride * 4) {
+      float4 v = *reinterpret_cast<const float4*>(gbuf + e);
+      for (int r = 1; r < paths; ++r) {
+        const float4 w = *reinterpret_cast<const float4*>(gbuf + r * V + e);
+        v.x += w.x; v.y += w.y; v.z += w.z; v.w += w.w;
+      }
+      *reinterpret_cast<float4*>(gC + e) = v;
+    }
+    return;
+  }
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < V; e += stride) {
     float v = gbuf[e];
     for (int r = 1; r < paths; ++r) v += gbuf[r * V + e];
     gC[e] = v;
@@ -748,7 +804,7 @@ pm_status launch_sgm_adjoint_paths(const SgmAdjArgs& A, int paths, hipStream_t s
   int lines = 0;
   for (int path = 0; path < paths; ++path)
     lines += sgm_lines(A.s.W, A.s.H, kPathDx[path], kPathDy[path]);
-  constexpr int B = 8 / NV;
+  constexpr int B = 8 / NV;   // 16 / NV: twice the registers, half the waves, no faster
   sgm_adjoint_paths_kernel<NV, FULL, B><<<(unsigned)((lines + 3) / 4), 256, 0, st>>>(A, paths);
   return check_launch("sgm_adjoint_paths_kernel");
 }
@@ -878,10 +934,21 @@ pm_status run_sgm_forward(pm_diff* h, bool store, hipStream_t st) {
     pm_status s = sgm_diff_paths(diff_sgm_args(h, 0), h->sgm.paths, h->cost, h->L, st);
     if (s != PM_OK) return s;
     const unsigned blocks = (unsigned)((h->N * 16 + 255) / 256);
-    dispatch_ppl(h->shape.planes, [&](auto ppl) {
-      agg_sum_kernel<decltype(ppl)::value><<<blocks, 256, 0, st>>>(
-          h->N, h->shape.planes, h->sgm.paths, h->cost, h->L, h->agg);
-    });
+    const int D = h->shape.planes;
+    if (D % 4 == 0) {
+      const int d4 = D / 4;
+      if (d4 <= 16)
+        agg_sum4_kernel<1><<<blocks, 256, 0, st>>>(h->N, D, h->sgm.paths, h->cost, h->L, h->agg);
+      else if (d4 <= 32)
+        agg_sum4_kernel<2><<<blocks, 256, 0, st>>>(h->N, D, h->sgm.paths, h->cost, h->L, h->agg);
+      else
+        agg_sum4_kernel<4><<<blocks, 256, 0, st>>>(h->N, D, h->sgm.paths, h->cost, h->L, h->agg);
+    } else {
+      dispatch_ppl(D, [&](auto ppl) {
+        agg_sum_kernel<decltype(ppl)::value><<<blocks, 256, 0, st>>>(
+            h->N, D, h->sgm.paths, h->cost, h->L, h->agg);
+      });
+    }
     return check_launch("agg_sum_kernel");
   }
   for (int path = 0; path < h->sgm.paths; ++path) {
@@ -1075,8 +1142,10 @@ extern "C" pm_status pm_diff_backward(pm_diff* h, float grad_loss, float* grad_r
     A.gagg = h->agg; A.pvalid = h->pvalid; A.gC = nullptr;
     s = sgm_adjoint_paths(A, h->sgm.paths, st);
     if (s != PM_OK) return s;
-    grad_cost_sum_kernel<<<(unsigned)std::min<long>((h->V + 255) / 256, 65536), 256, 0, st>>>(
-        h->V, h->sgm.paths, h->L, h->gC);
+    const bool vec4 = h->V % 4 == 0;
+    const long items = vec4 ? h->V / 4 : h->V;
+    grad_cost_sum_kernel<<<(unsigned)std::min<long>((items + 255) / 256, 65536), 256, 0, st>>>(
+        h->V, h->sgm.paths, vec4, h->L, h->gC);
     s = check_launch("grad_cost_sum_kernel");
     if (s != PM_OK) return s;
     // the aggregate now holds dL/dagg and the path volumes hold adjoints: another backward
