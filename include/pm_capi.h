@@ -294,7 +294,9 @@ typedef struct pm_keyframe {
  * (compute units) blocks (0 = one block per work item).  cost_plane_split 1 splits each cost
  * tile's planes into groups of 32 (keyframes whose plane count is a multiple of 16), 0 keeps
  * a tile whole.  The cost work list is divided over the chip's 8 XCDs by work (tiles x planes
- * x sources), each XCD taking one contiguous range. */
+ * x sources), each XCD taking one contiguous range.  sgm_rows 1 runs the COMPACT storage's
+ * paths of keyframes with 16-bit costs and 32, 64, 96 or 128 planes four scanlines per wave
+ * (16 lanes per scanline), 0 one scanline per wave; both give the same bits. */
 typedef struct pm_batch_tuning {
   int32_t cost_tile_rows;
   int32_t cost_stage_planes;
@@ -302,6 +304,7 @@ typedef struct pm_batch_tuning {
   int32_t cost_blocks_per_cu;
   int32_t sgm_blocks_per_cu;
   int32_t cost_plane_split;
+  int32_t sgm_rows;
 } pm_batch_tuning;
 
 typedef struct pm_batch pm_batch;

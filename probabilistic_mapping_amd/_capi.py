@@ -77,7 +77,8 @@ class Keyframe(ctypes.Structure):
 class BatchTuning(ctypes.Structure):
     _fields_ = [("cost_tile_rows", ctypes.c_int32), ("cost_stage_planes", ctypes.c_int32),
                 ("sgm_prefetch", ctypes.c_int32), ("cost_blocks_per_cu", ctypes.c_int32),
-                ("sgm_blocks_per_cu", ctypes.c_int32), ("cost_plane_split", ctypes.c_int32)]
+                ("sgm_blocks_per_cu", ctypes.c_int32), ("cost_plane_split", ctypes.c_int32),
+                ("sgm_rows", ctypes.c_int32)]
 
 
 class Knobs(ctypes.Structure):

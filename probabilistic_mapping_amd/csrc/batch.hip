@@ -32,10 +32,19 @@ struct SgmClass {
   int first_off[8] = {};     // offsets of first[] per path in the pool
   int total[8] = {};
   // compact form: one work list over (keyframe, path) segments (sweep form: the horizontal
-  // paths only)
+  // paths only); rest_*: the same over the keyframes no rows class takes
   int seg_off = 0, seg_first_off = 0, nseg = 0, seg_total = 0;
+  int rest_seg_off = 0, rest_seg_first_off = 0, rest_nseg = 0, rest_seg_total = 0;
   // sweep form: workgroups (keyframe, band) and the most row blocks of any keyframe
   int sw_first_off = 0, sw_total = 0, sw_blocks = 0;
+};
+
+// Compact form, four scanlines per wave (16-bit costs, D = 16 nv): segments (keyframe, path)
+// of 16-scanline items.
+struct RowsClass {
+  int nv = 4;
+  std::vector<int> kf;
+  int seg_off = 0, seg_first_off = 0, nseg = 0, seg_total = 0;
 };
 
 // The cost work list for one (tile rows, plane split) choice: item offsets and plane groups
@@ -69,8 +78,9 @@ struct pm_batch {
   int* d_pool = nullptr;
   pm::CostList cost[3][2];              // [tile rows 2, 4, 8][plane split off, on]
   std::vector<pm::SgmClass> sgm;
+  std::vector<pm::RowsClass> rows;   // compact form with 16-bit costs only
   std::vector<pm::FilterClass> filt;
-  pm_batch_tuning tune{4, 8, 16, 0, 0, 0};
+  pm_batch_tuning tune{4, 8, 16, 0, 0, 0, 1};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -101,7 +111,8 @@ bool valid_tuning(const pm_batch_tuning& t, int max_sources) {
          cost_rows_fit(t.cost_tile_rows, max_sources) &&
          (t.sgm_prefetch == 8 || t.sgm_prefetch == 16) && t.cost_blocks_per_cu >= 0 &&
          t.sgm_blocks_per_cu >= 0 && t.cost_blocks_per_cu <= 64 && t.sgm_blocks_per_cu <= 64 &&
-         (t.cost_plane_split == 0 || t.cost_plane_split == 1);
+         (t.cost_plane_split == 0 || t.cost_plane_split == 1) &&
+         (t.sgm_rows == 0 || t.sgm_rows == 1);
 }
 
 // Plane groups of a keyframe's cost tiles: groups of 32 planes (16 when D is a multiple of 16
@@ -145,12 +156,27 @@ pm_status run_sweeps(pm_batch* b, bool up, hipStream_t st) {
 
 pm_status run_sgm(pm_batch* b, hipStream_t st) {
   if (b->storage == PM_SGM_COMPACT || b->storage == PM_SGM_SWEEP) {
+    const bool rows = b->tune.sgm_rows != 0 && !b->rows.empty();
+    if (rows)
+      for (const RowsClass& c : b->rows) {
+        if (!launch_sgm_excess_rows_batch(b->ct, b->tune.sgm_prefetch, c.nv, b->d_kfs,
+                                          b->d_pool + c.seg_off, b->d_pool + c.seg_first_off,
+                                          c.nseg, c.seg_total,
+                                          grid_for(c.seg_total, b->tune.sgm_blocks_per_cu,
+                                                   b->cus), st))
+          return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch: unsupported sgm tuning");
+        const pm_status s = check_launch("sgm_excess_rows_batch_kernel");
+        if (s != PM_OK) return s;
+      }
     for (const SgmClass& c : b->sgm) {
-      if (c.seg_total == 0) continue;
-      if (!launch_sgm_excess_batch(b->ct, b->tune.sgm_prefetch, c.nv, c.full, b->d_kfs,
-                                   b->d_pool + c.seg_off, b->d_pool + c.seg_first_off, c.nseg,
-                                   c.seg_total,
-                                   grid_for(c.seg_total, b->tune.sgm_blocks_per_cu, b->cus), st))
+      const int total = rows ? c.rest_seg_total : c.seg_total;
+      if (total == 0) continue;
+      const int nseg = rows ? c.rest_nseg : c.nseg;
+      const int* seg = b->d_pool + (rows ? c.rest_seg_off : c.seg_off);
+      const int* first = b->d_pool + (rows ? c.rest_seg_first_off : c.seg_first_off);
+      if (!launch_sgm_excess_batch(b->ct, b->tune.sgm_prefetch, c.nv, c.full, b->d_kfs, seg,
+                                   first, nseg, total,
+                                   grid_for(total, b->tune.sgm_blocks_per_cu, b->cus), st))
         return fail(PM_ERR_INVALID_ARGUMENT, "pm_batch: unsupported sgm tuning");
       const pm_status s = check_launch("sgm_excess_batch_kernel");
       if (s != PM_OK) return s;
@@ -383,6 +409,24 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     c.seg_first_off = put(sfirst);
     c.nseg = (int)sfirst.size();
     c.seg_total = acc;
+    // the same over the keyframes that the rows classes do not take
+    std::vector<int> rseg, rfirst;
+    int racc = 0;
+    if (storage == PM_SGM_COMPACT)
+      for (int kf : c.kf) {
+        const CostArgs& a = b->host[kf].c;
+        if (rows_supported(cost_type, a.D)) continue;
+        for (int p = 0; p < b->paths; ++p) {
+          rseg.push_back(kf);
+          rseg.push_back(p);
+          rfirst.push_back(racc);
+          racc += (sgm_lines(a.W, a.H, kPathDx[p], kPathDy[p]) + 3) / 4;
+        }
+      }
+    c.rest_seg_off = put(rseg);
+    c.rest_seg_first_off = put(rfirst);
+    c.rest_nseg = (int)rfirst.size();
+    c.rest_seg_total = racc;
     std::vector<int> wfirst(c.kf.size());
     int wacc = 0;
     for (size_t j = 0; j < c.kf.size(); ++j) {
@@ -393,6 +437,37 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
     }
     c.sw_first_off = put(wfirst);
     c.sw_total = wacc;
+  }
+  // rows classes (compact storage, 16-bit costs, D = 32, 64, 96, 128)
+  if (storage == PM_SGM_COMPACT)
+    for (int k = 0; k < count; ++k) {
+      const CostArgs& a = b->host[k].c;
+      if (!rows_supported(cost_type, a.D)) continue;
+      const int nv = rows_nv(a.D);
+      auto it = std::find_if(b->rows.begin(), b->rows.end(),
+                             [&](const RowsClass& c) { return c.nv == nv; });
+      if (it == b->rows.end()) {
+        b->rows.push_back(RowsClass{});
+        it = b->rows.end() - 1;
+        it->nv = nv;
+      }
+      it->kf.push_back(k);
+    }
+  for (RowsClass& c : b->rows) {
+    std::vector<int> seg, sfirst;
+    int acc = 0;
+    for (int kf : c.kf)
+      for (int p = 0; p < b->paths; ++p) {
+        const CostArgs& a = b->host[kf].c;
+        seg.push_back(kf);
+        seg.push_back(p);
+        sfirst.push_back(acc);
+        acc += (sgm_lines(a.W, a.H, kPathDx[p], kPathDy[p]) + 15) / 16;
+      }
+    c.seg_off = put(seg);
+    c.seg_first_off = put(sfirst);
+    c.nseg = (int)sfirst.size();
+    c.seg_total = acc;
   }
   for (FilterClass& c : b->filt) {
     c.kf_off = put(c.kf);
@@ -585,20 +660,24 @@ extern "C" pm_status pm_batch_autotune(pm_batch* b, pm_stream stream, int32_t re
         }
   best_ms = 1e30f;
   const int pref[2] = {8, 16};
-  for (int f : pref)
-    for (int g : bpc) {
-      if (status != PM_OK) break;
-      b->tune = win;
-      b->tune.sgm_prefetch = f;
-      b->tune.sgm_blocks_per_cu = g;
-      float ms = 0;
-      status = time_stage(run_sgm, &ms);
-      if (status == PM_OK && ms < best_ms) {
-        best_ms = ms;
-        win.sgm_prefetch = f;
-        win.sgm_blocks_per_cu = g;
+  const int nrows = b->rows.empty() ? 1 : 2;
+  for (int rw = 0; rw < nrows; ++rw)
+    for (int f : pref)
+      for (int g : bpc) {
+        if (status != PM_OK) break;
+        b->tune = win;
+        b->tune.sgm_prefetch = f;
+        b->tune.sgm_blocks_per_cu = g;
+        b->tune.sgm_rows = rw;
+        float ms = 0;
+        status = time_stage(run_sgm, &ms);
+        if (status == PM_OK && ms < best_ms) {
+          best_ms = ms;
+          win.sgm_prefetch = f;
+          win.sgm_blocks_per_cu = g;
+          win.sgm_rows = rw;
+        }
       }
-    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->tune = status == PM_OK ? win : cur;

@@ -134,6 +134,12 @@ void launch_combine_excess_batch(pm_cost_type ct, int LP, const KfDesc* kfs, con
 bool launch_sgm_sweep_batch(pm_cost_type ct, int nv, bool full, bool up, const KfDesc* kfs,
                             const int* kf, const int* first, int n, int total, int rb,
                             hipStream_t st);
+// Compact form, four scanlines per wave: 16-bit costs and D = 32, 64, 96 or 128 (nv = D / 16).
+bool rows_supported(pm_cost_type ct, int D);
+int rows_nv(int D);
+bool launch_sgm_excess_rows_batch(pm_cost_type ct, int prefetch, int nv, const KfDesc* kfs,
+                                  const int* seg, const int* first, int nseg, int total,
+                                  int grid, hipStream_t st);
 bool launch_sgm_excess_batch(pm_cost_type ct, int prefetch, int nv, bool full,
                              const KfDesc* kfs, const int* seg, const int* first, int nseg,
                              int total, int grid, hipStream_t st);
@@ -161,6 +167,7 @@ __device__ __forceinline__ gptr<T> to_global(T* p) {
 }
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // A device array read and written through a raw buffer resource: the base and size live in
@@ -191,14 +198,43 @@ struct Buf {
   __device__ __forceinline__ void put_u16_nt(uint32_t v, uint32_t voff, uint32_t soff) const {
     __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, (int)voff, (int)soff, 2);
   }
-  // 8- and 16-byte stores (voff + soff a multiple of the width); AUX 2 = non-temporal
+  // 8-byte stores (voff + soff a multiple of 8); AUX 2 = non-temporal
   template <int AUX>
   __device__ __forceinline__ void put_b64(u32x2 v, uint32_t voff, uint32_t soff) const {
     __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)voff, (int)soff, AUX);
   }
+  // Stores of more than 8 bytes.  A VALU write to such a store's data registers less than two
+  // wait states after the store can replace data the store has not read yet; compiled from the
+  // store builtins, the scheduler placed such writes directly behind the store, and a few
+  // dwords of the last lanes of a row came out wrong.  These stores are therefore written as
+  // asm with their own wait states: two after the store for its data, and five before it,
+  // since its scalar offset may come from a v_readfirstlane just ahead of the asm, and a
+  // vector memory instruction reading an SGPR that a VALU instruction wrote needs five (the
+  // compiler inserts neither inside an asm statement).  AUX 2 = non-temporal.  soff is
+  // wave-uniform, as for every Buf access.
   template <int AUX>
   __device__ __forceinline__ void put_b128(u32x4 v, uint32_t voff, uint32_t soff) const {
-    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)voff, (int)soff, AUX);
+    static_assert(AUX == 0 || AUX == 2, "default or non-temporal policy");
+    if constexpr (AUX == 2)
+      asm volatile("s_nop 4\n\tbuffer_store_dwordx4 %0, %1, %2, %3 offen nt\n\ts_nop 1"
+                   :: "v"(v), "v"(voff), "s"(r), "s"(__builtin_amdgcn_readfirstlane(soff))
+                   : "memory");
+    else
+      asm volatile("s_nop 4\n\tbuffer_store_dwordx4 %0, %1, %2, %3 offen\n\ts_nop 1"
+                   :: "v"(v), "v"(voff), "s"(r), "s"(__builtin_amdgcn_readfirstlane(soff))
+                   : "memory");
+  }
+  template <int AUX>
+  __device__ __forceinline__ void put_b96(u32x3 v, uint32_t voff, uint32_t soff) const {
+    static_assert(AUX == 0 || AUX == 2, "default or non-temporal policy");
+    if constexpr (AUX == 2)
+      asm volatile("s_nop 4\n\tbuffer_store_dwordx3 %0, %1, %2, %3 offen nt\n\ts_nop 1"
+                   :: "v"(v), "v"(voff), "s"(r), "s"(__builtin_amdgcn_readfirstlane(soff))
+                   : "memory");
+    else
+      asm volatile("s_nop 4\n\tbuffer_store_dwordx3 %0, %1, %2, %3 offen\n\ts_nop 1"
+                   :: "v"(v), "v"(voff), "s"(r), "s"(__builtin_amdgcn_readfirstlane(soff))
+                   : "memory");
   }
   template <int AUX>
   __device__ __forceinline__ void put_b32(uint32_t v, uint32_t voff, uint32_t soff) const {

@@ -115,6 +115,91 @@ __global__ __launch_bounds__(256) void sgm_excess_batch_kernel(const KfDesc* __r
   }
 }
 
+// Four scanlines per wave (sgm_rows_excess): compact form with 16-bit costs and D = 32, 64,
+// 96 or 128.  Register blocks of B steps: PF 16 allows 32 cost dwords per buffer, PF 8 16.
+template <int NV, int PF>
+constexpr int rows_block() {
+  constexpr int b = PF * 2 / (NV / 2);
+  return b < 2 ? 2 : (b > 16 ? 16 : b);
+}
+bool rows_supported(pm_cost_type ct, int D) {
+  return (ct == PM_COST_F16 || ct == PM_COST_BF16) && D % 32 == 0 && D >= 32 && D <= 128;
+}
+int rows_nv(int D) { return D / 16; }
+
+// Single keyframe, every path in one launch: global group g (4 scanlines) counts the groups of
+// path 0, then path 1, ...
+template <int NV, typename CT, int B>
+__global__ __launch_bounds__(256) void sgm_excess_rows_kernel(const SgmArgs base, int paths,
+                                                              const CT* __restrict__ cost,
+                                                              uint16_t* __restrict__ ebuf) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int g = blockIdx.x * 4 + wave, path = 0, n = 0;
+  for (; path < paths; ++path, g -= n) {
+    n = (sgm_lines(base.W, base.H, path_dx(path), path_dy(path)) + 3) / 4;
+    if (g < n) break;
+  }
+  if (path == paths) return;
+  SgmArgs a = base;
+  a.dx = path_dx(path);
+  a.dy = path_dy(path);
+  a.nlines = sgm_lines(a.W, a.H, a.dx, a.dy);
+  a.eout = ebuf + (size_t)path * ((size_t)base.W * base.H * base.D);
+  sgm_rows_excess<NV, CT, B>(a, g * 4, cost);
+}
+
+// Batched: segments (keyframe, path) as in sgm_excess_batch_kernel, of 16-scanline items.
+template <int NV, typename CT, int B>
+__global__ __launch_bounds__(256) void sgm_excess_rows_batch_kernel(
+    const KfDesc* __restrict__ kfs, const int* __restrict__ seg, const int* __restrict__ first,
+    int nseg, int total) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int i = blockIdx.x; i < total; i += gridDim.x) {
+    const int j = owner_of(first, nseg, i);
+    const KfDesc& k = kfs[seg[2 * j]];
+    const int path = seg[2 * j + 1];
+    SgmArgs a;
+    a.W = k.c.W; a.H = k.c.H; a.D = k.c.D; a.dx = path_dx(path); a.dy = path_dy(path);
+    a.nlines = sgm_lines(a.W, a.H, a.dx, a.dy);
+    a.ref = k.c.ref;
+    a.P1n = uniform(k.P1n); a.P2n = uniform(k.P2n); a.P1s = uniform(k.P1s);
+    a.P2s = uniform(k.P2s); a.tau_i = __builtin_amdgcn_readfirstlane(k.tau_i);
+    a.eout = k.ebuf + (size_t)path * a.W * a.H * a.D;
+    a.eq_inv = uniform(k.einv);
+    const int line0 = (i - first[j]) * 16 + wave * 4;
+    if (line0 < a.nlines) sgm_rows_excess<NV, CT, B>(a, line0, static_cast<const CT*>(k.cost));
+  }
+}
+
+template <typename CT, int PF>
+bool sgm_rows_batch_p(int nv, const KfDesc* kfs, const int* seg, const int* first, int nseg,
+                      int total, int grid, hipStream_t st) {
+  switch (nv) {
+    case 2: sgm_excess_rows_batch_kernel<2, CT, rows_block<2, PF>()><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total); return true;
+    case 4: sgm_excess_rows_batch_kernel<4, CT, rows_block<4, PF>()><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total); return true;
+    case 6: sgm_excess_rows_batch_kernel<6, CT, rows_block<6, PF>()><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total); return true;
+    case 8: sgm_excess_rows_batch_kernel<8, CT, rows_block<8, PF>()><<<grid, 256, 0, st>>>(kfs, seg, first, nseg, total); return true;
+    default: return false;
+  }
+}
+
+bool launch_sgm_excess_rows_batch(pm_cost_type ct, int prefetch, int nv, const KfDesc* kfs,
+                                  const int* seg, const int* first, int nseg, int total,
+                                  int grid, hipStream_t st) {
+  if (prefetch != 8 && prefetch != 16) return false;
+  switch (ct) {
+    case PM_COST_F16:
+      return prefetch == 8 ? sgm_rows_batch_p<__half, 8>(nv, kfs, seg, first, nseg, total, grid, st)
+                           : sgm_rows_batch_p<__half, 16>(nv, kfs, seg, first, nseg, total, grid, st);
+    case PM_COST_BF16:
+      return prefetch == 8
+                 ? sgm_rows_batch_p<__hip_bfloat16, 8>(nv, kfs, seg, first, nseg, total, grid, st)
+                 : sgm_rows_batch_p<__hip_bfloat16, 16>(nv, kfs, seg, first, nseg, total, grid, st);
+    default:
+      return false;
+  }
+}
+
 template <typename CT, int B>
 bool sgm_excess_batch_b(int nv, bool full, const KfDesc* kfs, const int* seg, const int* first,
                         int nseg, int total, int grid, hipStream_t st) {
@@ -280,9 +365,31 @@ void launch_excess_t(const SgmArgs& a, int paths, const CT* cost, uint16_t* ebuf
   sgm_excess_kernel<NV, CT, FULL, B><<<blocks, 256, 0, st>>>(a, paths, cost, ebuf);
 }
 
+template <int NV, typename CT>
+void launch_excess_rows_t(const SgmArgs& a, int paths, const CT* cost, uint16_t* ebuf,
+                          hipStream_t st) {
+  long groups = 0;
+  for (int path = 0; path < paths; ++path)
+    groups += (sgm_lines(a.W, a.H, kPathDx[path], kPathDy[path]) + 3) / 4;
+  const unsigned blocks = (unsigned)((groups + 3) / 4);
+  sgm_excess_rows_kernel<NV, CT, rows_block<NV, PM_SGM_PREFETCH>()><<<blocks, 256, 0, st>>>(
+      a, paths, cost, ebuf);
+}
+
 template <typename CT>
 pm_status launch_sgm_excess(const SgmArgs& a, int paths, const CT* cost, uint16_t* ebuf,
                             hipStream_t st) {
+  if constexpr (sizeof(CT) == 2) {
+    if (a.D % 32 == 0 && a.D <= 128) {
+      switch (a.D) {
+        case 32: launch_excess_rows_t<2, CT>(a, paths, cost, ebuf, st); break;
+        case 64: launch_excess_rows_t<4, CT>(a, paths, cost, ebuf, st); break;
+        case 96: launch_excess_rows_t<6, CT>(a, paths, cost, ebuf, st); break;
+        default: launch_excess_rows_t<8, CT>(a, paths, cost, ebuf, st); break;
+      }
+      return check_launch("sgm_excess_rows_kernel");
+    }
+  }
   if (a.D == 64) launch_excess_t<1, CT, true>(a, paths, cost, ebuf, st);
   else if (a.D == 128) launch_excess_t<2, CT, true>(a, paths, cost, ebuf, st);
   else if (a.D < 64) launch_excess_t<1, CT, false>(a, paths, cost, ebuf, st);

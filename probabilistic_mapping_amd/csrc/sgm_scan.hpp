@@ -347,4 +347,312 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
   }
 }
 
+// ---- Four scanlines per wave (compact form, 16-bit costs) -------------------------------
+// A wave runs four scanlines of one path side by side: lanes 16r .. 16r+15 (row r of the
+// wave) hold scanline line0 + r, lane k of a row holding planes NV*k .. NV*k+NV-1 (D = 16 NV).
+// Against one scanline per wave (lane = plane) a step's minimum over planes is NV-1 in-lane
+// minima plus four DPP stages that leave the row's minimum in every lane of the row (no
+// cross-row broadcast, no readlane), and the per-step selects and shifts are shared by NV
+// planes.  The recurrence and the stored levels are those of sgm_scanline<..., EQ = true>:
+// the same operations in the same order on every element.
+
+// min over the 16 lanes of each row, in every lane of the row
+__device__ __forceinline__ uint32_t row_min16(uint32_t u) {
+  u = umin_dpp<0xB1>(u);    // quad_perm [1,0,3,2]
+  u = umin_dpp<0x4E>(u);    // quad_perm [2,3,0,1]
+  u = umin_dpp<0x124>(u);   // row_ror:4
+  u = umin_dpp<0x128>(u);   // row_ror:8
+  return u;
+}
+// lane I of each row, in every lane of the row (row_newbcast:I)
+template <int I>
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x150 + I, 0xF, 0xF, true);
+}
+// row_bcast<i> for a slot index of a fully unrolled loop (the switch folds away)
+template <int B>
+__device__ __forceinline__ uint32_t row_bcast_at(uint32_t v, int i) {
+  switch (i) {
+    case 0: return row_bcast<0>(v);
+    case 1: return row_bcast<1 % B>(v);
+    case 2: return row_bcast<2 % B>(v);
+    case 3: return row_bcast<3 % B>(v);
+    case 4: return row_bcast<4 % B>(v);
+    case 5: return row_bcast<5 % B>(v);
+    case 6: return row_bcast<6 % B>(v);
+    case 7: return row_bcast<7 % B>(v);
+    case 8: return row_bcast<8 % B>(v);
+    case 9: return row_bcast<9 % B>(v);
+    case 10: return row_bcast<10 % B>(v);
+    case 11: return row_bcast<11 % B>(v);
+    case 12: return row_bcast<12 % B>(v);
+    case 13: return row_bcast<13 % B>(v);
+    case 14: return row_bcast<14 % B>(v);
+    default: return row_bcast<15 % B>(v);
+  }
+}
+// Bits 16r .. 16r+15 set where any bit of row r is set in b (wave-uniform, scalar unit).
+__device__ __forceinline__ uint64_t row_any_mask(uint64_t b) {
+  b |= b >> 1;
+  b |= b >> 2;
+  b |= b >> 4;
+  b |= b >> 8;
+  b &= 0x0001000100010001ull;
+  return (b << 16) - b;
+}
+// 0 in the lanes whose bit of `mask` is set, else 1.0; opaque to the optimiser, so the cost's
+// fma(c, w, x) stays an fma (v_fma_mix for FP16 costs).
+__device__ __forceinline__ float lane_weight(uint64_t mask) {
+  float w;
+  asm("v_cndmask_b32_e64 %0, 1.0, 0, %1" : "=v"(w) : "s"(mask));
+  return w;
+}
+
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+
+// One lane's NV 16-bit costs of one step, as NV/2 dwords (plane 2i in the low half of word i).
+template <int NW>
+struct CostWords {
+  uint32_t w[NW];
+};
+template <int NW>
+__device__ __forceinline__ CostWords<NW> load_words(const Buf& b, uint32_t voff, uint32_t soff) {
+  CostWords<NW> c;
+  if constexpr (NW == 1) {
+    c.w[0] = __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)voff, (int)soff, 0);
+  } else if constexpr (NW == 2) {
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)voff, (int)soff, 0);
+    c.w[0] = v.x; c.w[1] = v.y;
+  } else if constexpr (NW == 3) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(b.r, (int)voff, (int)soff, 0);
+    c.w[0] = v[0]; c.w[1] = v[1]; c.w[2] = v[2];
+  } else {
+    static_assert(NW == 4, "2 to 8 planes per lane");
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(b.r, (int)voff, (int)soff, 0);
+    c.w[0] = v[0]; c.w[1] = v[1]; c.w[2] = v[2]; c.w[3] = v.w;
+  }
+  return c;
+}
+template <int NW, int AUX>
+__device__ __forceinline__ void store_words(const Buf& b, const uint32_t (&w)[NW], uint32_t voff,
+                                            uint32_t soff) {
+  if constexpr (NW == 1) {
+    __builtin_amdgcn_raw_buffer_store_b32(w[0], b.r, (int)voff, (int)soff, AUX);
+  } else if constexpr (NW == 2) {
+    const u32x2 v = {w[0], w[1]};
+    __builtin_amdgcn_raw_buffer_store_b64(v, b.r, (int)voff, (int)soff, AUX);
+  } else if constexpr (NW == 3) {
+    const u32x3 v = {w[0], w[1], w[2]};
+    b.put_b96<AUX>(v, voff, soff);
+  } else {
+    const u32x4 v = {w[0], w[1], w[2], w[3]};
+    b.put_b128<AUX>(v, voff, soff);
+  }
+}
+
+// Plane j of a lane's cost words, as float, and whether any of the lane's planes is negative.
+template <typename CT>
+struct Cost16;
+template <>
+struct Cost16<__half> {
+  template <int NW>
+  static __device__ __forceinline__ float plane(const CostWords<NW>& c, int j) {
+    h16x2 h;
+    __builtin_memcpy(&h, &c.w[j >> 1], 4);
+    return (float)((j & 1) ? h.y : h.x);
+  }
+  template <int NW>
+  static __device__ __forceinline__ bool any_negative(const CostWords<NW>& c) {
+    h16x2 m;
+    __builtin_memcpy(&m, &c.w[0], 4);
+#pragma unroll
+    for (int i = 1; i < NW; ++i) {
+      h16x2 h;
+      __builtin_memcpy(&h, &c.w[i], 4);
+      m = __builtin_elementwise_min(m, h);   // v_pk_min_f16; minNum ignores a NaN operand
+    }
+    return m.x < (_Float16)0 || m.y < (_Float16)0;
+  }
+};
+template <>
+struct Cost16<__hip_bfloat16> {
+  template <int NW>
+  static __device__ __forceinline__ float plane(const CostWords<NW>& c, int j) {
+    return __uint_as_float((j & 1) ? (c.w[j >> 1] & 0xFFFF0000u) : (c.w[j >> 1] << 16));
+  }
+  template <int NW>
+  static __device__ __forceinline__ bool any_negative(const CostWords<NW>& c) {
+    bool neg = false;
+#pragma unroll
+    for (int j = 0; j < 2 * NW; ++j) neg = neg || plane<NW>(c, j) < 0.0f;
+    return neg;
+  }
+};
+
+// Scanlines line0 .. line0+3 of path (a.dx, a.dy), compact form: each writes its excess
+// levels to a.eout.  Requires D == 16 NV with NV even (2..8) and a 16-bit cost type.
+// Scanlines past a.nlines (the last wave of a path) compute a copy of the last scanline and
+// store nothing.  Blocks of B steps in which every row's scanline still runs use one
+// wave-uniform pixel offset per step (the row with the lowest start pixel's) plus a constant
+// per-lane offset; the other blocks address every lane on its own, clamped to its scanline.
+template <int NV, typename CT, int B>
+__device__ __forceinline__ void sgm_rows_excess(const SgmArgs& a, int line0,
+                                                const CT* __restrict__ cost) {
+  static_assert(sizeof(CT) == 2 && NV % 2 == 0 && NV >= 2 && NV <= 8, "16-bit costs, NV even");
+  static_assert(B >= 2 && B <= 16, "one block step per row lane");
+  constexpr int NW = NV / 2;
+  using CC = Cost16<CT>;
+  const gptr<const uint8_t> ref = to_global(static_cast<const uint8_t*>(a.ref));
+  const uint32_t D = (uint32_t)a.D;
+  const uint32_t V = (uint32_t)a.W * (uint32_t)a.H * D;
+  const Buf bcost(cost, V * 2u);
+  const Buf beout(a.eout, V * 2u);
+  const int lane = threadIdx.x & 63;
+  const int k = lane & 15;
+  const int row_line = line0 + (lane >> 4);
+  const bool live = row_line < a.nlines;
+  int x, y;
+  scanline_start(a, live ? row_line : a.nlines - 1, x, y);
+  int n = 1 << 30;
+  if (a.dx > 0) n = min(n, a.W - x);
+  if (a.dx < 0) n = min(n, x + 1);
+  if (a.dy > 0) n = min(n, a.H - y);
+  if (a.dy < 0) n = min(n, y + 1);
+  const int nst = live ? n : 0;   // steps this row stores
+  const int pstep = a.dy * a.W + a.dx;
+  const int p0 = y * a.W + x;
+  // wave-uniform extents: every row runs its steps < nmin; the wave runs to nmax
+  int nmin = 1 << 30, nmax = 0, pbase = 1 << 30;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int nr = __builtin_amdgcn_readlane(nst, 16 * r);
+    nmin = min(nmin, nr);
+    nmax = max(nmax, nr);
+    pbase = min(pbase, __builtin_amdgcn_readlane(p0, 16 * r));
+  }
+  const uint32_t lofs = (uint32_t)k * (uint32_t)NV * 2u;             // the lane's planes
+  const uint32_t voff = (uint32_t)(p0 - pbase) * D * 2u + lofs;      // ... in its row's pixel
+  // byte offset of step s of this row, clamped to its scanline (own addressing)
+  auto own = [&](int s) { return (uint32_t)(p0 + min(s, n - 1) * pstep) * D * 2u + lofs; };
+
+  float L[NV], Lq[NV];
+  uint32_t lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;   // row-edge lanes keep these (see below)
+  uint32_t Iprev;                                // intensity of the row's previous step
+  // step 0: L = C' (sgm.cu:21-43); the stored excess is 0
+  {
+    const CostWords<NW> c = load_words<NW>(bcost, own(0), 0u);
+    Iprev = (uint32_t)ref[p0];
+    const uint64_t inv = row_any_mask(__ballot(CC::any_negative(c)));
+    const bool invalid = (inv >> lane) & 1;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) L[j] = invalid ? kCostInf : CC::plane(c, j);
+    const uint32_t z[NW] = {};
+    if (nst > 0) store_words<NW, 2>(beout, z, own(0), 0u);
+  }
+
+  CostWords<NW> c0[B], c1[B];
+  uint32_t P1v, P2v;   // lane k: the penalties of step base + k of the row's current block
+  // penalties of block [base, base + B): lane k compares the intensity of step base + k with
+  // step base + k - 1 (lane 0 with the carried intensity), sgm.cu:75-82
+  auto penalties = [&](int base) {
+    const uint32_t I = (uint32_t)ref[p0 + min(base + min(k, B - 1), n - 1) * pstep];
+    const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)Iprev, (int)I, 0x111, 0xF,
+                                                                0xF, false);   // row_shr:1
+    Iprev = row_bcast<B - 1>(I);
+    const bool smooth = abs((int)I - (int)prev) < a.tau_i;
+    P1v = __float_as_uint(smooth ? a.P1s : a.P1n);
+    P2v = __float_as_uint(smooth ? a.P2s : a.P2n);
+  };
+  // uniform addressing: steps past nmin - 1 re-read step nmin - 1 (in bounds for every row)
+  const uint32_t pkstep = (uint32_t)pstep * D * 2u;
+  auto load_uniform = [&](int base, CostWords<NW> (&c)[B]) {
+    uint32_t pk = (uint32_t)(pbase + min(base, nmin - 1) * pstep) * D * 2u;
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      c[i] = load_words<NW>(bcost, voff, pk);
+      pk = scalar_add(pk, base + i + 1 < nmin ? pkstep : 0u);
+    }
+  };
+  auto load_own = [&](int base, CostWords<NW> (&c)[B]) {
+#pragma unroll
+    for (int i = 0; i < B; ++i) c[i] = load_words<NW>(bcost, own(base + i), 0u);
+  };
+  // one step: the recurrence on the lane's planes and their packed levels
+  auto step = [&](const CostWords<NW>& c, uint32_t P1u, uint32_t P2u, uint32_t (&qw)[NW]) {
+    const uint64_t inv = row_any_mask(__ballot(CC::any_negative(c)));
+    static_assert(kCostInf == 0.0f, "an invalid pixel adds 0 in place of its cost");
+    const float w = lane_weight(inv);
+    uint32_t lm = __float_as_uint(L[0]);
+#pragma unroll
+    for (int j = 1; j < NV; ++j) lm = min(lm, __float_as_uint(L[j]));
+    const float m = __uint_as_float(row_min16(lm));   // sgm.cu:55-66
+    const float P1 = __uint_as_float(P1u), mP2 = m + __uint_as_float(P2u);
+    // L(q, d0-1) and L(q, d0+NV) from the neighbouring lanes of the row; the row's edge lanes
+    // have no source lane and keep 0xFFFFFFFF, above every non-negative float's bit pattern
+    lo = (uint32_t)__builtin_amdgcn_update_dpp((int)lo, (int)__float_as_uint(L[NV - 1]), 0x111,
+                                               0xF, 0xF, false);   // row_shr:1
+    hi = (uint32_t)__builtin_amdgcn_update_dpp((int)hi, (int)__float_as_uint(L[0]), 0x101, 0xF,
+                                               0xF, false);        // row_shl:1
+    uint32_t q[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const uint32_t left = j > 0 ? __float_as_uint(L[j - 1]) : lo;
+      const uint32_t right = j + 1 < NV ? __float_as_uint(L[j + 1]) : hi;
+      // min(L(q,d), L(q,d-1)+P1, L(q,d+1)+P1, m+P2) (sgm.cu:85-93)
+      const float val = umin(umin(L[j], mP2), __uint_as_float(min(left, right)) + P1);
+      q[j] = min(kExcessMaxQ, (uint32_t)fmaf(val - m, a.eq_inv, 0.5f));
+      Lq[j] = fmaf(CC::plane(c, j), w, val) - m;   // sgm.cu:98
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) L[j] = Lq[j];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) qw[i] = q[2 * i] | (q[2 * i + 1] << 16);
+  };
+  auto run_uniform = [&](int base, const CostWords<NW> (&c)[B]) {
+    penalties(base);
+    uint32_t pk = (uint32_t)(pbase + base * pstep) * D * 2u;
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      uint32_t qw[NW];
+      step(c[i], row_bcast_at<B>(P1v, i), row_bcast_at<B>(P2v, i), qw);
+      store_words<NW, 2>(beout, qw, voff, pk);
+      pk = scalar_add(pk, pkstep);
+    }
+  };
+  auto run_own = [&](int base, const CostWords<NW> (&c)[B]) {
+    penalties(base);
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      if (base + i >= nmax) break;
+      uint32_t qw[NW];
+      step(c[i], row_bcast_at<B>(P1v, i), row_bcast_at<B>(P2v, i), qw);
+      if (base + i < nst) store_words<NW, 2>(beout, qw, own(base + i), 0u);
+    }
+  };
+
+  int base = 1;
+  if (base + B <= nmin) {
+    // blocks every row runs in full, two register buffers used ping-pong: the loads of the
+    // next block are in flight while the current one computes
+    load_uniform(base, c0);
+    while (base + 2 * B <= nmin) {
+      load_uniform(base + B, c1);
+      run_uniform(base, c0);
+      load_uniform(base + 2 * B, c0);
+      run_uniform(base + B, c1);
+      base += 2 * B;
+    }
+    if (base + B <= nmin) {
+      run_uniform(base, c0);
+      base += B;
+    }
+  }
+  // the rest (rows of unequal length, the last scanlines of a path), lane by lane
+  while (base < nmax) {
+    load_own(base, c0);
+    run_own(base, c0);
+    base += B;
+  }
+}
+
 }  // namespace pm

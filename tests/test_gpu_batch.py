@@ -71,17 +71,19 @@ def test_batch_equals_single_keyframe(uneven, cost_type, paths, storage):
     batch.run()
     torch.cuda.synchronize()
     _assert_equal(batch, expect)
-    # every launch shape gives the same bits
+    # every launch shape gives the same bits (sgm_rows: the compact paths four scanlines per
+    # wave for the 16-bit costs of 32, 64, 96 and 128 planes, or one scanline per wave)
     for rows in (2, 4, 8):
         for planes in ((4, 8) if rows == 8 else (8, 16)):
             for pref in (8, 16):
-                for bpc, split in ((0, 0), (4, 1), (0, 1), (4, 0)):
+                for bpc, split, sgm_rows in ((0, 0, 1), (4, 1, 0), (0, 1, 0), (4, 0, 1)):
                     for o in batch.outputs:
                         for v in o.values():
                             v.fill_(7)
                     batch.tuning = dict(cost_tile_rows=rows, cost_stage_planes=planes,
                                         sgm_prefetch=pref, cost_blocks_per_cu=bpc,
-                                        sgm_blocks_per_cu=bpc, cost_plane_split=split)
+                                        sgm_blocks_per_cu=bpc, cost_plane_split=split,
+                                        sgm_rows=sgm_rows)
                     batch.run()
                     torch.cuda.synchronize()
                     _assert_equal(batch, expect)
@@ -221,6 +223,8 @@ def test_batch_rejects_bad_input(uneven):
         b.tuning = dict(cost_tile_rows=8, cost_stage_planes=16)
     with pytest.raises(C.PmError):
         b.tuning = dict(cost_plane_split=2)
+    with pytest.raises(C.PmError):
+        b.tuning = dict(sgm_rows=2)
     with pytest.raises(C.PmError):
         b.launch()   # nothing captured
     with pytest.raises(C.PmError):   # the sweep form takes at most 128 planes
