@@ -52,14 +52,14 @@ COST_BYTES = {"f32": 4, "f16": 2, "bf16": 2}
 #     FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 in bytes (scales from a device copy of known size);
 COUNTER_PROFILE = "profiles/counters_752x480x64_s5_p8.md"
 SGM_PASS_TRAFFIC = {("plain", "f32", "exact"): 2.2033e9,
-                    ("batched", "f16", "compact"): 1.1427e9}
+                    ("batched", "f16", "compact"): 1.1583e9}
 #   * COST_VALU: the cost launch's SQ_INSTS_VALU (wave64 instructions) and the fraction of the
 #     chip's VALU issue slots they fill over the launch's duration in the profiled run (at the
 #     2.4 GHz peak clock).
 #   * UNEVEN_SGM_TRAFFIC: HBM bytes of the uneven batch's SGM pass (its paths and combine/filter
 #     launches) per replay, default keyframe set (UNEVEN_PROFILE).
 UNEVEN_PROFILE = "profiles/uneven_batch_16kf_f16.md"
-UNEVEN_SGM_TRAFFIC = 21.306e9
+UNEVEN_SGM_TRAFFIC = 21.210e9
 COST_VALU = {("plain", "f32", "exact"): (112.98e6, 0.819),
              ("batched", "f16", "compact"): (112.27e6, 0.936)}
 
