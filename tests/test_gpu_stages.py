@@ -441,6 +441,100 @@ def test_compact_sgm_against_exact(dev, small_scene, cost_type, paths, planes):
     assert torch.equal(dep_c, dep_d)
 
 
+_PATH_DX = (1, -1, 0, 0, 1, -1, 1, -1)
+_PATH_DY = (0, 0, 1, -1, 1, -1, -1, 1)
+
+
+def _scanline(W, H, dx, dy, i):
+    """Pixels of scanline i of direction (dx, dy), in step order (the lines of a direction:
+    rows or columns, or for the diagonals the lines from the left or right edge, then from the
+    top or bottom edge)."""
+    if dx != 0:
+        if i < H:
+            x, y = (0 if dx > 0 else W - 1), i
+        else:
+            j = i - H
+            x, y = (1 + j if dx > 0 else j), (0 if dy > 0 else H - 1)
+    else:
+        x, y = i, (0 if dy > 0 else H - 1)
+    pts = []
+    while 0 <= x < W and 0 <= y < H:
+        pts.append((x, y))
+        x, y = x + dx, y + dy
+    return pts
+
+
+def _replay_levels(c, ref, pts, dx, dy, knobs, step):
+    """The excess levels of one scanline, replayed in float32 with the operations of the GPU
+    recurrence in the same order (sgm.cu:75-98): m = min_d L; val = min(L(d), m + P2,
+    min(L(d-1), L(d+1)) + P1); L' = (C' + val) - m with C' = 0 at a pixel with a negative cost;
+    level = trunc(fl(E / step + 1/2)) with E = val - m, at most 65534; 0 at the first step."""
+    f32 = np.float32
+    P1n, P2n = f32(knobs["pi1"]), f32(knobs["pi2"])
+    P1s, P2s = f32(P1n * f32(knobs["sgm_q1"])), f32(P2n * f32(knobs["sgm_q2"]))
+    tau = int(np.ceil(knobs["tau_so"]))
+    inv = f32(1.0) / f32(step)
+    D = c.shape[2]
+    out = np.zeros((len(pts), D), np.uint16)
+    L = None
+    for k, (x, y) in enumerate(pts):
+        cc = c[y, x]
+        invalid = bool((cc < 0).any())
+        cw = np.zeros(D, f32) if invalid else cc
+        if k == 0:
+            L = cw.copy()
+            continue
+        m = L.min()
+        smooth = abs(int(ref[y, x]) - int(ref[y - dy, x - dx])) < tau
+        P1, P2 = (P1s, P2s) if smooth else (P1n, P2n)
+        nb = np.full(D, np.inf, f32)
+        nb[1:] = L[:-1]
+        nb[:-1] = np.minimum(nb[:-1], L[1:])
+        val = np.minimum(np.minimum(L, f32(m + P2)), (nb + P1).astype(f32))
+        e = (val - m).astype(f32)
+        q = (e.astype(np.float64) * np.float64(inv) + 0.5).astype(f32)
+        out[k] = np.minimum(65534, np.floor(q)).astype(np.uint16)
+        L = ((cw + val).astype(f32) - m).astype(f32)
+    return out
+
+
+@pytest.mark.parametrize("cost_type,planes", [("f16", 64), ("bf16", 96), ("f16", 128),
+                                              ("f16", 32), ("f32", 64), ("f16", 48)])
+def test_compact_levels_match_scanline_replay(dev, small_scene, cost_type, planes):
+    """The 16-bit excess levels stored by the compact paths launch, on five scanlines of each
+    of the 8 directions (first, last and three inside), equal a float32 replay of those
+    scanlines in numpy level for level, at every pixel.  16-bit costs with 32, 64, 96 or 128
+    planes run four scanlines per wave (NV = 2, 4, 6, 8 planes per lane); FP32 costs and 48
+    planes run one scanline per wave."""
+    s = small_scene
+    K, homs, _, _ = _small_inputs(s)
+    ds = (s["idep_max"] - s["idep_min"]) / (planes - 1)
+    ref = torch.from_numpy(s["ref"]).cuda()
+    quads = [dev.make_quad(torch.from_numpy(im).cuda()) for im in s["srcs"]]
+    cost = dev.calc_cost(ref, quads, homs, s["idep_min"], ds, planes, cost_type)
+    knobs = dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+    H, W = s["ref"].shape
+    ws = dev.sgm_excess_workspace(H, W, planes, 8)
+    dev.sgm_filter_excess(ref, cost, knobs, 8, s["idep_min"], ds, workspace=ws)
+    torch.cuda.synchronize()
+    E = ws.view(8, H, W, planes).cpu().numpy().view(np.uint16)
+    c = cost.float().cpu().numpy()
+    img = s["ref"]
+    step = _excess_step(knobs)
+    checked = 0
+    for p in range(8):
+        dx, dy = _PATH_DX[p], _PATH_DY[p]
+        nlines = W + H - 1 if dx and dy else (H if dx else W)
+        for i in sorted({0, nlines // 3, nlines // 2, 2 * nlines // 3 + 1, nlines - 1}):
+            pts = _scanline(W, H, dx, dy, i)
+            want = _replay_levels(c, img, pts, dx, dy, knobs, step)
+            got = np.stack([E[p, y, x] for x, y in pts])
+            bad = np.argwhere(got != want)
+            assert bad.size == 0, (p, i, bad[:4].tolist(), len(pts))
+            checked += len(pts)
+    _report(f"compact levels vs replay {cost_type} D={planes}", {"pixels": checked})
+
+
 def test_compact_sgm_zero_penalties_is_exact(dev, small_scene):
     """P1 = P2 = 0: every excess is 0 (L = C), so the compact aggregate is P C exactly."""
     s = small_scene

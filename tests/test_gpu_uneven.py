@@ -90,6 +90,30 @@ def test_uneven_graph_replay_equals_single_keyframes(uneven16, replayed):
     torch.cuda.empty_cache()
 
 
+def test_uneven_rows_kernel_equals_one_scanline_kernel(uneven16):
+    """The compact paths of the 1280x720 keyframes with 96 and 128 planes (8, 13) four
+    scanlines per wave (12- and 16-byte level stores per lane) and one scanline per wave give
+    the same aggregate and depth, bit for bit."""
+    from probabilistic_mapping_amd import pipeline
+    host, specs = uneven16
+    knobs = host[0]["knobs"]
+    batch = pipeline.Batch([specs[8], specs[13]], "f16", knobs, 8, "compact", write_agg=True)
+    try:
+        got = {}
+        for rows in (0, 1):
+            batch.tuning = dict(sgm_rows=rows)
+            batch.run()
+            torch.cuda.synchronize()
+            got[rows] = [(o["agg"].clone(), o["depth"].clone()) for o in batch.outputs]
+        for (a0, d0), (a1, d1) in zip(got[0], got[1]):
+            assert torch.equal(a0, a1)
+            assert torch.equal(d0, d1)
+    finally:
+        batch.close()
+    del got
+    torch.cuda.empty_cache()
+
+
 # 13: 1280x720, 128 planes, 8 sources (the largest); 8: 1280x720, 96 planes, 4 sources;
 # 0: 320x240, 96 planes, 2 sources
 @pytest.mark.parametrize("k", [13, 8, 0, 7, 10, 12])
