@@ -691,3 +691,46 @@ class BatchedSequence(KeyframeSequence):
         self.batch.run()
         for i in range(len(self.images)):
             self.fusion_step(i)
+
+    def run_overlapped(self) -> None:
+        """``run`` with group g+1's cost launch beside group g's SGM pass: the current stream
+        runs the groups' cost launches one after another, and a second stream runs each group's
+        SGM pass, filter and fusion steps once that group's cost is written (the cost kernel is
+        VALU-bound, the SGM pass bound by memory).  The current stream waits for the second one
+        at the end, so every result of the sequence is ready there; the results equal
+        ``run``'s."""
+        main = torch.cuda.current_stream()
+        if getattr(self, "_side", None) is None:
+            self._side = torch.cuda.Stream(device=self.device)
+        side = self._side
+        groups = getattr(self.batch, "batches", [self.batch])
+        self.reset()
+        side.wait_stream(main)
+        k = 0
+        for b in groups:
+            b.run_stage(0)
+            done = torch.cuda.Event()
+            done.record(main)
+            with torch.cuda.stream(side):
+                side.wait_event(done)
+                b.run_stage(1)
+                b.run_stage(2)
+                for i in range(k, k + len(b.outputs)):
+                    self.fusion_step(i)
+            k += len(b.outputs)
+        main.wait_stream(side)
+
+    def capture_overlapped(self) -> None:
+        """Records ``run_overlapped`` (both streams) into one hipGraph, after one eager run."""
+        cap = torch.cuda.Stream(device=self.device)
+        cap.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(cap):
+            self.run_overlapped()
+        torch.cuda.current_stream().wait_stream(cap)
+        self._graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self._graph, stream=cap):
+            self.run_overlapped()
+
+    def launch_overlapped(self) -> None:
+        """Replays the graph of ``capture_overlapped`` on the current stream."""
+        self._graph.replay()

@@ -304,6 +304,49 @@ def test_batched_sequence_groups_match_one_batch(trajectory_scene):
                     assert torch.equal(q.state[k], one.state[k]), (rep, k)
 
 
+def test_overlapped_sequence_equals_serial(trajectory_scene):
+    """The sequence with each group's cost launch beside the previous group's SGM pass and
+    fusion (two streams), run eagerly and as one captured hipGraph replayed three times with
+    eager work in between, gives the serial run's depths, fused depth and state bit for bit
+    after every run."""
+    from probabilistic_mapping_amd.pipeline import BatchedSequence
+    s = trajectory_scene
+    n = 12
+    args = (s["images"][:n], s["poses"][:n], [[j for j in src if j < n] or [1]
+                                               for src in s["sources"][:n]],
+            s["scene"].K, s["planes"], s["idep_min"], s["idep_max"], s["knobs"], s["paths"])
+    ref = BatchedSequence(*args, cost_type="f16", sgm_storage="compact", median=True,
+                          lookahead=True, group=4)
+    ovl = BatchedSequence(*args, cost_type="f16", sgm_storage="compact", median=True,
+                          lookahead=True, group=4)
+    assert len(ovl.batch.batches) == 3
+    ref.run()
+    torch.cuda.synchronize()
+    want_dep = [ref.depth_of(i).clone() for i in range(n)]
+    want_fused = ref.fused.clone()
+    want_state = {k: v.clone() for k, v in ref.state.items()}
+
+    def check(tag):
+        torch.cuda.synchronize()
+        for i in range(n):
+            assert torch.equal(ovl.depth_of(i), want_dep[i]), (tag, i)
+        assert torch.equal(ovl.fused, want_fused), tag
+        for k, v in want_state.items():
+            assert torch.equal(ovl.state[k], v), (tag, k)
+
+    ovl.run_overlapped()
+    check("eager")
+    ovl.capture_overlapped()
+    for rep in range(3):
+        for i in range(n):
+            ovl.depth_of(i).fill_(rep + 5)
+        ovl.fused.fill_(-1.0)
+        ovl.launch_overlapped()
+        check(f"replay {rep}")
+        ref.run()          # eager work between the replays
+        torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize("storage", ["exact", "compact"])
 def test_batched_sequence_with_median_matches_plain(trajectory_scene, storage):
     """The batched sequence's median -> fusion steps give the plain sequence's fused depth and
