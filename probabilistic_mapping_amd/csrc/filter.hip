@@ -201,14 +201,19 @@ __device__ __forceinline__ bool excess_agg4(const CT* __restrict__ cost,
     uint32_t s[4] = {0u, 0u, 0u, 0u};
     // the excess volumes are read exactly once: non-temporal loads.  For 8 and 4 paths all
     // loads are issued before the first is used (a loop over a run-time count waits for each
-    // one before issuing the next); the integer sum does not depend on the order.
+    // one before issuing the next); the integer sum does not depend on the order.  Each path
+    // volume is read through a buffer resource of its own (base and size in scalar registers),
+    // so a lane's only address is its 32-bit byte offset e * 2 (e < V < 2^30), shared by all
+    // paths, instead of a 64-bit address per path.
     auto add = [&](u32x2 u) {
       const uint32_t q[4] = {u.x & 0xFFFFu, u.x >> 16, u.y & 0xFFFFu, u.y >> 16};
 #pragma unroll
       for (int j = 0; j < 4; ++j) s[j] += q[j];
     };
+    const uint32_t eb = (uint32_t)e * 2u;
     auto ld = [&](int r) {
-      return __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(ebuf + (long)r * V + e));
+      const Buf b(ebuf + (long)r * V, (uint32_t)V * 2u);
+      return __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)eb, 0, 2);   // non-temporal
     };
     if (paths == 8) {
       u32x2 u[8];
