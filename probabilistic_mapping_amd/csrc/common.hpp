@@ -60,6 +60,13 @@ struct SgmArgs {
 // Compact per-path storage of the SGM excess E = L - C' (0 <= E <= P2 on every step): 16-bit
 // fixed point with step max(P2n, P2s) / kExcessMaxQ.
 constexpr uint32_t kExcessMaxQ = 0xFFFEu;
+// The level of an excess e >= 0: e / step rounded to the nearest integer (ties to even), at
+// most kExcessMaxQ, in the low 16 bits of the result (the high bits hold 0x4B00).  One fma puts
+// e * inv + 2^23 in a float whose unit in the last place is 1, so its mantissa holds the rounded
+// quotient (exact for quotients below 2^23); the clamp works on that float's bit pattern.
+__device__ __forceinline__ uint32_t excess_level_bits(float e, float inv) {
+  return min(0x4B000000u | kExcessMaxQ, __float_as_uint(fmaf(e, inv, 8388608.0f)));
+}
 __host__ __device__ inline float excess_step(float P2n, float P2s) {
   const float m = P2n > P2s ? P2n : P2s;
   return m > 0.0f ? m / (float)kExcessMaxQ : 0.0f;

@@ -283,7 +283,7 @@ __device__ __forceinline__ void sgm_scanline(const SgmArgs& a, int line,
       lo_[j] = (LMASK && invalid) ? kCostInf : Ln[j];
       // val >= m exactly (every candidate is >= m), so E >= 0; rounding to nearest.  At an
       // invalid pixel the level is ignored (the combine finds the pixel from its cost).
-      if (EQ) q[j] = min(kExcessMaxQ, (uint32_t)fmaf(val - m, a.eq_inv, 0.5f));
+      if (EQ) q[j] = excess_level_bits(val - m, a.eq_inv) & 0xFFFFu;
     }
     if (AGG) store_planes_f32<NV, 0>(bagg, out, act, pack, off, p * 4u);
     if (LOUT) store_planes_f32<NV, 2>(blout, lo_, act, pack, off, p * 4u);
@@ -450,6 +450,10 @@ __device__ __forceinline__ void store_words(const Buf& b, const uint32_t (&w)[NW
   }
 }
 
+// Whether any 16-bit half of the lane's cost words, plus 0x7FFF modulo 2^16, is at most limit.
+template <int NW>
+__device__ __forceinline__ bool any_below(const CostWords<NW>& c, uint32_t limit);
+
 // Plane j of a lane's cost words, as float, and whether any of the lane's planes is negative.
 template <typename CT>
 struct Cost16;
@@ -461,31 +465,40 @@ struct Cost16<__half> {
     __builtin_memcpy(&h, &c.w[j >> 1], 4);
     return (float)((j & 1) ? h.y : h.x);
   }
+  // Negative FP16 values (-0 and NaNs excluded, as for c < 0) are the bit patterns 0x8001 ..
+  // 0xFC00 (-inf); adding 0x7FFF modulo 2^16 maps exactly those to 0 .. 0x7BFF, so one packed
+  // add per word, a packed minimum and one compare test all of a lane's planes.
   template <int NW>
   static __device__ __forceinline__ bool any_negative(const CostWords<NW>& c) {
-    h16x2 m;
-    __builtin_memcpy(&m, &c.w[0], 4);
-#pragma unroll
-    for (int i = 1; i < NW; ++i) {
-      h16x2 h;
-      __builtin_memcpy(&h, &c.w[i], 4);
-      m = __builtin_elementwise_min(m, h);   // v_pk_min_f16; minNum ignores a NaN operand
-    }
-    return m.x < (_Float16)0 || m.y < (_Float16)0;
+    return any_below<NW>(c, 0x7BFFu);
   }
 };
+template <int NW>
+__device__ __forceinline__ bool any_below(const CostWords<NW>& c, uint32_t limit) {
+  typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+  const u16x2 bias = {0x7FFF, 0x7FFF};
+  u16x2 m;
+  __builtin_memcpy(&m, &c.w[0], 4);
+  m += bias;   // v_pk_add_u16, modulo 2^16
+#pragma unroll
+  for (int i = 1; i < NW; ++i) {
+    u16x2 h;
+    __builtin_memcpy(&h, &c.w[i], 4);
+    h += bias;
+    m = __builtin_elementwise_min(m, h);   // v_pk_min_u16
+  }
+  return (uint32_t)min(m.x, m.y) <= limit;
+}
 template <>
 struct Cost16<__hip_bfloat16> {
   template <int NW>
   static __device__ __forceinline__ float plane(const CostWords<NW>& c, int j) {
     return __uint_as_float((j & 1) ? (c.w[j >> 1] & 0xFFFF0000u) : (c.w[j >> 1] << 16));
   }
+  // negative BF16 values: 0x8001 .. 0xFF80 (-inf), mapped to 0 .. 0x7F7F as for FP16
   template <int NW>
   static __device__ __forceinline__ bool any_negative(const CostWords<NW>& c) {
-    bool neg = false;
-#pragma unroll
-    for (int j = 0; j < 2 * NW; ++j) neg = neg || plane<NW>(c, j) < 0.0f;
-    return neg;
+    return any_below<NW>(c, 0x7F7Fu);
   }
 };
 
@@ -600,13 +613,14 @@ __device__ __forceinline__ void sgm_rows_excess(const SgmArgs& a, int line0,
       const uint32_t right = j + 1 < NV ? __float_as_uint(L[j + 1]) : hi;
       // min(L(q,d), L(q,d-1)+P1, L(q,d+1)+P1, m+P2) (sgm.cu:85-93)
       const float val = umin(umin(L[j], mP2), __uint_as_float(min(left, right)) + P1);
-      q[j] = min(kExcessMaxQ, (uint32_t)fmaf(val - m, a.eq_inv, 0.5f));
+      q[j] = excess_level_bits(val - m, a.eq_inv);
       Lq[j] = fmaf(CC::plane(c, j), w, val) - m;   // sgm.cu:98
     }
 #pragma unroll
     for (int j = 0; j < NV; ++j) L[j] = Lq[j];
+    // the low halves of two level words side by side (v_perm_b32: bytes 1:0 of each)
 #pragma unroll
-    for (int i = 0; i < NW; ++i) qw[i] = q[2 * i] | (q[2 * i + 1] << 16);
+    for (int i = 0; i < NW; ++i) qw[i] = __builtin_amdgcn_perm(q[2 * i + 1], q[2 * i], 0x05040100u);
   };
   auto run_uniform = [&](int base, const CostWords<NW> (&c)[B]) {
     penalties(base);

@@ -468,7 +468,8 @@ def _replay_levels(c, ref, pts, dx, dy, knobs, step):
     """The excess levels of one scanline, replayed in float32 with the operations of the GPU
     recurrence in the same order (sgm.cu:75-98): m = min_d L; val = min(L(d), m + P2,
     min(L(d-1), L(d+1)) + P1); L' = (C' + val) - m with C' = 0 at a pixel with a negative cost;
-    level = trunc(fl(E / step + 1/2)) with E = val - m, at most 65534; 0 at the first step."""
+    level = E / step rounded to the nearest integer, ties to even (one rounding of the exact
+    quotient), with E = val - m, at most 65534; 0 at the first step."""
     f32 = np.float32
     P1n, P2n = f32(knobs["pi1"]), f32(knobs["pi2"])
     P1s, P2s = f32(P1n * f32(knobs["sgm_q1"])), f32(P2n * f32(knobs["sgm_q2"]))
@@ -492,8 +493,8 @@ def _replay_levels(c, ref, pts, dx, dy, knobs, step):
         nb[:-1] = np.minimum(nb[:-1], L[1:])
         val = np.minimum(np.minimum(L, f32(m + P2)), (nb + P1).astype(f32))
         e = (val - m).astype(f32)
-        q = (e.astype(np.float64) * np.float64(inv) + 0.5).astype(f32)
-        out[k] = np.minimum(65534, np.floor(q)).astype(np.uint16)
+        q = np.rint(e.astype(np.float64) * np.float64(inv))   # exact product, one rounding
+        out[k] = np.minimum(65534, q).astype(np.uint16)
         L = ((cw + val).astype(f32) - m).astype(f32)
     return out
 
