@@ -464,7 +464,7 @@ def _scanline(W, H, dx, dy, i):
     return pts
 
 
-def _replay_levels(c, ref, pts, dx, dy, knobs, step):
+def _replay_levels(c, ref, pts, dx, dy, knobs, step, want_l=False):
     """The excess levels of one scanline, replayed in float32 with the operations of the GPU
     recurrence in the same order (sgm.cu:75-98): m = min_d L; val = min(L(d), m + P2,
     min(L(d-1), L(d+1)) + P1); L' = (C' + val) - m with C' = 0 at a pixel with a negative cost;
@@ -474,9 +474,10 @@ def _replay_levels(c, ref, pts, dx, dy, knobs, step):
     P1n, P2n = f32(knobs["pi1"]), f32(knobs["pi2"])
     P1s, P2s = f32(P1n * f32(knobs["sgm_q1"])), f32(P2n * f32(knobs["sgm_q2"]))
     tau = int(np.ceil(knobs["tau_so"]))
-    inv = f32(1.0) / f32(step)
+    inv = f32(1.0) / f32(step) if step > 0 else f32(0.0)
     D = c.shape[2]
     out = np.zeros((len(pts), D), np.uint16)
+    lout = np.zeros((len(pts), D), f32)   # L, 0 at invalid pixels (the exact form's volume)
     L = None
     for k, (x, y) in enumerate(pts):
         cc = c[y, x]
@@ -484,6 +485,7 @@ def _replay_levels(c, ref, pts, dx, dy, knobs, step):
         cw = np.zeros(D, f32) if invalid else cc
         if k == 0:
             L = cw.copy()
+            lout[k] = 0.0 if invalid else L
             continue
         m = L.min()
         smooth = abs(int(ref[y, x]) - int(ref[y - dy, x - dx])) < tau
@@ -496,17 +498,21 @@ def _replay_levels(c, ref, pts, dx, dy, knobs, step):
         q = np.rint(e.astype(np.float64) * np.float64(inv))   # exact product, one rounding
         out[k] = np.minimum(65534, q).astype(np.uint16)
         L = ((cw + val).astype(f32) - m).astype(f32)
-    return out
+        lout[k] = 0.0 if invalid else L
+    return (out, lout) if want_l else out
 
 
-@pytest.mark.parametrize("cost_type,planes", [("f16", 64), ("bf16", 96), ("f16", 128),
-                                              ("f16", 32), ("f32", 64), ("f16", 48)])
-def test_compact_levels_match_scanline_replay(dev, small_scene, cost_type, planes):
+@pytest.mark.parametrize("cost_type,planes,paths", [("f16", 64, 8), ("bf16", 96, 8),
+                                                    ("f16", 128, 8), ("f16", 32, 8),
+                                                    ("f32", 64, 8), ("f16", 48, 8),
+                                                    ("bf16", 32, 4), ("bf16", 64, 4),
+                                                    ("bf16", 128, 8)])
+def test_compact_levels_match_scanline_replay(dev, small_scene, cost_type, planes, paths):
     """The 16-bit excess levels stored by the compact paths launch, on five scanlines of each
-    of the 8 directions (first, last and three inside), equal a float32 replay of those
-    scanlines in numpy level for level, at every pixel.  16-bit costs with 32, 64, 96 or 128
-    planes run four scanlines per wave (NV = 2, 4, 6, 8 planes per lane); FP32 costs and 48
-    planes run one scanline per wave."""
+    direction (first, last and three inside), equal a float32 replay of those scanlines in
+    numpy level for level, at every pixel.  16-bit costs with 32, 64, 96 or 128 planes run four
+    scanlines per wave (NV = 2, 4, 6, 8 planes per lane); FP32 costs and 48 planes run one
+    scanline per wave."""
     s = small_scene
     K, homs, _, _ = _small_inputs(s)
     ds = (s["idep_max"] - s["idep_min"]) / (planes - 1)
@@ -515,15 +521,15 @@ def test_compact_levels_match_scanline_replay(dev, small_scene, cost_type, plane
     cost = dev.calc_cost(ref, quads, homs, s["idep_min"], ds, planes, cost_type)
     knobs = dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
     H, W = s["ref"].shape
-    ws = dev.sgm_excess_workspace(H, W, planes, 8)
-    dev.sgm_filter_excess(ref, cost, knobs, 8, s["idep_min"], ds, workspace=ws)
+    ws = dev.sgm_excess_workspace(H, W, planes, paths)
+    dev.sgm_filter_excess(ref, cost, knobs, paths, s["idep_min"], ds, workspace=ws)
     torch.cuda.synchronize()
-    E = ws.view(8, H, W, planes).cpu().numpy().view(np.uint16)
+    E = ws.view(paths, H, W, planes).cpu().numpy().view(np.uint16)
     c = cost.float().cpu().numpy()
     img = s["ref"]
     step = _excess_step(knobs)
     checked = 0
-    for p in range(8):
+    for p in range(paths):
         dx, dy = _PATH_DX[p], _PATH_DY[p]
         nlines = W + H - 1 if dx and dy else (H if dx else W)
         for i in sorted({0, nlines // 3, nlines // 2, 2 * nlines // 3 + 1, nlines - 1}):
@@ -533,7 +539,40 @@ def test_compact_levels_match_scanline_replay(dev, small_scene, cost_type, plane
             bad = np.argwhere(got != want)
             assert bad.size == 0, (p, i, bad[:4].tolist(), len(pts))
             checked += len(pts)
-    _report(f"compact levels vs replay {cost_type} D={planes}", {"pixels": checked})
+    _report(f"compact levels vs replay {cost_type} D={planes} P={paths}", {"pixels": checked})
+
+
+@pytest.mark.parametrize("cost_type,planes", [("f32", 64), ("f16", 128), ("f32", 200)])
+def test_exact_paths_match_scanline_replay(dev, small_scene, cost_type, planes):
+    """The per-path FP32 L volumes of the exact all-paths launch (pm_sgm_filter_paths; 0 at
+    invalid pixels), on five scanlines of each of the 8 directions, equal the float32 replay of
+    the recurrence bit for bit: the plain form's SGM without any tolerance."""
+    s = small_scene
+    K, homs, _, _ = _small_inputs(s)
+    ds = (s["idep_max"] - s["idep_min"]) / (planes - 1)
+    ref = torch.from_numpy(s["ref"]).cuda()
+    quads = [dev.make_quad(torch.from_numpy(im).cuda()) for im in s["srcs"]]
+    cost = dev.calc_cost(ref, quads, homs, s["idep_min"], ds, planes, cost_type)
+    knobs = dict(pi1=8.0, pi2=24.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+    H, W = s["ref"].shape
+    ws = dev.sgm_workspace(H, W, planes, 8)
+    dev.sgm_filter_paths(ref, cost, knobs, 8, s["idep_min"], ds, workspace=ws)
+    torch.cuda.synchronize()
+    Lv = ws.view(8, H, W, planes).cpu().numpy()
+    c = cost.float().cpu().numpy()
+    img = s["ref"]
+    checked = 0
+    for p in range(8):
+        dx, dy = _PATH_DX[p], _PATH_DY[p]
+        nlines = W + H - 1 if dx and dy else (H if dx else W)
+        for i in sorted({0, nlines // 3, nlines // 2, 2 * nlines // 3 + 1, nlines - 1}):
+            pts = _scanline(W, H, dx, dy, i)
+            _, want = _replay_levels(c, img, pts, dx, dy, knobs, _excess_step(knobs), True)
+            got = np.stack([Lv[p, y, x] for x, y in pts])
+            bad = np.argwhere(got.view(np.uint32) != want.view(np.uint32))
+            assert bad.size == 0, (p, i, bad[:4].tolist(), len(pts))
+            checked += len(pts)
+    _report(f"exact paths vs replay {cost_type} D={planes}", {"pixels": checked})
 
 
 def test_compact_sgm_zero_penalties_is_exact(dev, small_scene):
