@@ -50,13 +50,17 @@ __global__ __launch_bounds__(256) void make_quad_kernel(const uint8_t* __restric
 }
 
 template <typename CT, int TY, int DC>
-__global__ __launch_bounds__(256) void cost_tile_kernel(const CostArgs a, CT* __restrict__ cost,
-                                                        int npg) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void cost_tile_kernel(const CostArgs a, CT* __restrict__ cost,
+                                                        int npg, int nblocks) {
   static_assert(cost_tile_max_sources(TY) >= PM_MAX_SOURCES, "tile too short for the sources");
   __shared__ float stage[4][TY * DC * 64];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  cost_tile<CT, TY, DC>(a, xcd_block(blockIdx.x, gridDim.x), cost, stage[wave], nullptr,
-                        nullptr, npg);
+  // One item per block (nblocks = the grid).  Written as a loop and held to 4 waves per SIMD,
+  // the 8 x 4 FP32 tile body takes 128 VGPRs with 9 spilled values, reloaded only in the
+  // per-chunk record setup; straight-line it took 168 (3 waves per SIMD), and held to 4 waves
+  // it spilled 32 into the inner loop.
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x)
+    cost_tile<CT, TY, DC>(a, xcd_block(b, nblocks), cost, stage[wave], nullptr, nullptr, npg);
 }
 
 // Batched form: the work items of every keyframe of a batch form one list (keyframe k owns
@@ -139,7 +143,8 @@ void launch_cost_single(const CostArgs& a, CT* cost, hipStream_t st) {
   constexpr int TY = SingleShape<CT>::TY, DC = SingleShape<CT>::DC;
   const long tiles = cost_tiles<TY>(a.W, a.H);
   const int npg = tiles >= kCostSplitTiles ? 1 : (a.D % 64 == 0 ? 4 : (a.D % 32 == 0 ? 2 : 1));
-  cost_tile_kernel<CT, TY, DC><<<(unsigned)(tiles * npg), 256, 0, st>>>(a, cost, npg);
+  const int nblocks = (int)(tiles * npg);
+  cost_tile_kernel<CT, TY, DC><<<(unsigned)nblocks, 256, 0, st>>>(a, cost, npg, nblocks);
 }
 }  // namespace pm
 
