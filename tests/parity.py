@@ -5,6 +5,10 @@ Bars (stated where they are asserted):
     depth pixel) must agree on >= 99.9 % of elements;
   * continuous values are compared as relative error on elements where both sides made the
     same discrete decision.
+
+It also holds a float32 replay of one SGM scanline (``replay_scanline``) that the GPU tests
+compare the stored per-path volumes with bit for bit; tests/test_oracle_sgm.py pins the replay
+against the oracle's aggregation.
 """
 from __future__ import annotations
 
@@ -186,3 +190,73 @@ def median_lipschitz(g_raw, o_raw, g_med, o_med, decided):
     chk = decided & (depth_category(om) == 2)
     viol = np.abs(gm[chk] - om[chk]) > wmax[chk] + 1e-6 * np.abs(om[chk])
     return {"checked": int(chk.sum()), "violations": int(viol.sum())}
+
+
+def excess_step(knobs):
+    """Level step of the compact SGM storage: max(P2, P2 q2) / 65534 in float32, so that an
+    excess up to the largest P2 of the knobs fits the 16-bit levels 0..65534."""
+    p2 = max(np.float32(knobs["pi2"]), np.float32(knobs["pi2"]) * np.float32(knobs["sgm_q2"]))
+    return float(np.float32(p2) / np.float32(65534.0))
+
+
+# The 8 path directions in aggregation order: L->R, R->L, T->B, B->T (sgm.cu:119-131), then
+# the diagonals.
+PATH_DX = (1, -1, 0, 0, 1, -1, 1, -1)
+PATH_DY = (0, 0, 1, -1, 1, -1, -1, 1)
+
+
+def scanline(W, H, dx, dy, i):
+    """Pixels of scanline i of direction (dx, dy), in step order (the lines of a direction:
+    rows or columns, or for the diagonals the lines from the left or right edge, then from the
+    top or bottom edge)."""
+    if dx != 0:
+        if i < H:
+            x, y = (0 if dx > 0 else W - 1), i
+        else:
+            j = i - H
+            x, y = (1 + j if dx > 0 else j), (0 if dy > 0 else H - 1)
+    else:
+        x, y = i, (0 if dy > 0 else H - 1)
+    pts = []
+    while 0 <= x < W and 0 <= y < H:
+        pts.append((x, y))
+        x, y = x + dx, y + dy
+    return pts
+
+
+def replay_scanline(c, ref, pts, dx, dy, knobs, step, want_l=False):
+    """The excess levels of one scanline, replayed in float32 with the operations of the GPU
+    recurrence in the same order (sgm.cu:75-98): m = min_d L; val = min(L(d), m + P2,
+    min(L(d-1), L(d+1)) + P1); L' = (C' + val) - m with C' = 0 at a pixel with a negative cost;
+    level = E / step rounded to the nearest integer, ties to even (one rounding of the exact
+    quotient), with E = val - m, at most 65534; 0 at the first step."""
+    f32 = np.float32
+    P1n, P2n = f32(knobs["pi1"]), f32(knobs["pi2"])
+    P1s, P2s = f32(P1n * f32(knobs["sgm_q1"])), f32(P2n * f32(knobs["sgm_q2"]))
+    tau = int(np.ceil(knobs["tau_so"]))
+    inv = f32(1.0) / f32(step) if step > 0 else f32(0.0)
+    D = c.shape[2]
+    out = np.zeros((len(pts), D), np.uint16)
+    lout = np.zeros((len(pts), D), f32)   # L, 0 at invalid pixels (the exact form's volume)
+    L = None
+    for k, (x, y) in enumerate(pts):
+        cc = c[y, x]
+        invalid = bool((cc < 0).any())
+        cw = np.zeros(D, f32) if invalid else cc
+        if k == 0:
+            L = cw.copy()
+            lout[k] = 0.0 if invalid else L
+            continue
+        m = L.min()
+        smooth = abs(int(ref[y, x]) - int(ref[y - dy, x - dx])) < tau
+        P1, P2 = (P1s, P2s) if smooth else (P1n, P2n)
+        nb = np.full(D, np.inf, f32)
+        nb[1:] = L[:-1]
+        nb[:-1] = np.minimum(nb[:-1], L[1:])
+        val = np.minimum(np.minimum(L, f32(m + P2)), (nb + P1).astype(f32))
+        e = (val - m).astype(f32)
+        q = np.rint(e.astype(np.float64) * np.float64(inv))   # exact product, one rounding
+        out[k] = np.minimum(65534, q).astype(np.uint16)
+        L = ((cw + val).astype(f32) - m).astype(f32)
+        lout[k] = 0.0 if invalid else L
+    return (out, lout) if want_l else out

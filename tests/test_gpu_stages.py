@@ -170,7 +170,7 @@ def test_ragged_sizes_cost_sgm_filter(dev, W, H, D):
     ex = agg_s.cpu().numpy().astype(np.float64)
     cx = agg_x.cpu().numpy().astype(np.float64)
     ulp = np.spacing(np.abs(ex).astype(np.float32)).astype(np.float64)
-    assert np.all(np.abs(cx - ex) <= 8 * _excess_step(knobs) / 2 + 4 * 8 * ulp)
+    assert np.all(np.abs(cx - ex) <= 8 * parity.excess_step(knobs) / 2 + 4 * 8 * ulp)
 
 
 def test_sgm_zero_penalties_is_path_count_times_cost(dev, small_scene):
@@ -401,11 +401,6 @@ def test_concurrent_paths_equal_sequential(dev, small_scene, cost_type, planes):
     assert torch.equal(dep_a, dep_c)
 
 
-def _excess_step(knobs):
-    p2 = max(np.float32(knobs["pi2"]), np.float32(knobs["pi2"]) * np.float32(knobs["sgm_q2"]))
-    return float(np.float32(p2) / np.float32(65534.0))
-
-
 @pytest.mark.parametrize("cost_type,paths,planes", [("f32", 8, 64), ("f16", 8, 64),
                                                     ("bf16", 4, 96), ("f16", 8, 200),
                                                     ("f32", 4, 5), ("f16", 4, 128)])
@@ -424,7 +419,7 @@ def test_compact_sgm_against_exact(dev, small_scene, cost_type, paths, planes):
     agg_e, dep_e = dev.sgm_filter_paths(ref, cost, knobs, paths, s["idep_min"], ds)
     agg_c, dep_c = dev.sgm_filter_excess(ref, cost, knobs, paths, s["idep_min"], ds)
     ge, gc = agg_e.cpu().numpy().astype(np.float64), agg_c.cpu().numpy().astype(np.float64)
-    step = _excess_step(knobs)
+    step = parity.excess_step(knobs)
     ulp = np.spacing(np.abs(ge).astype(np.float32)).astype(np.float64)
     bound = paths * step / 2 + 4 * paths * ulp
     err = np.abs(gc - ge)
@@ -439,67 +434,6 @@ def test_compact_sgm_against_exact(dev, small_scene, cost_type, paths, planes):
     # depth-only call gives the same depth
     _, dep_d = dev.sgm_filter_excess(ref, cost, knobs, paths, s["idep_min"], ds, write_agg=False)
     assert torch.equal(dep_c, dep_d)
-
-
-_PATH_DX = (1, -1, 0, 0, 1, -1, 1, -1)
-_PATH_DY = (0, 0, 1, -1, 1, -1, -1, 1)
-
-
-def _scanline(W, H, dx, dy, i):
-    """Pixels of scanline i of direction (dx, dy), in step order (the lines of a direction:
-    rows or columns, or for the diagonals the lines from the left or right edge, then from the
-    top or bottom edge)."""
-    if dx != 0:
-        if i < H:
-            x, y = (0 if dx > 0 else W - 1), i
-        else:
-            j = i - H
-            x, y = (1 + j if dx > 0 else j), (0 if dy > 0 else H - 1)
-    else:
-        x, y = i, (0 if dy > 0 else H - 1)
-    pts = []
-    while 0 <= x < W and 0 <= y < H:
-        pts.append((x, y))
-        x, y = x + dx, y + dy
-    return pts
-
-
-def _replay_levels(c, ref, pts, dx, dy, knobs, step, want_l=False):
-    """The excess levels of one scanline, replayed in float32 with the operations of the GPU
-    recurrence in the same order (sgm.cu:75-98): m = min_d L; val = min(L(d), m + P2,
-    min(L(d-1), L(d+1)) + P1); L' = (C' + val) - m with C' = 0 at a pixel with a negative cost;
-    level = E / step rounded to the nearest integer, ties to even (one rounding of the exact
-    quotient), with E = val - m, at most 65534; 0 at the first step."""
-    f32 = np.float32
-    P1n, P2n = f32(knobs["pi1"]), f32(knobs["pi2"])
-    P1s, P2s = f32(P1n * f32(knobs["sgm_q1"])), f32(P2n * f32(knobs["sgm_q2"]))
-    tau = int(np.ceil(knobs["tau_so"]))
-    inv = f32(1.0) / f32(step) if step > 0 else f32(0.0)
-    D = c.shape[2]
-    out = np.zeros((len(pts), D), np.uint16)
-    lout = np.zeros((len(pts), D), f32)   # L, 0 at invalid pixels (the exact form's volume)
-    L = None
-    for k, (x, y) in enumerate(pts):
-        cc = c[y, x]
-        invalid = bool((cc < 0).any())
-        cw = np.zeros(D, f32) if invalid else cc
-        if k == 0:
-            L = cw.copy()
-            lout[k] = 0.0 if invalid else L
-            continue
-        m = L.min()
-        smooth = abs(int(ref[y, x]) - int(ref[y - dy, x - dx])) < tau
-        P1, P2 = (P1s, P2s) if smooth else (P1n, P2n)
-        nb = np.full(D, np.inf, f32)
-        nb[1:] = L[:-1]
-        nb[:-1] = np.minimum(nb[:-1], L[1:])
-        val = np.minimum(np.minimum(L, f32(m + P2)), (nb + P1).astype(f32))
-        e = (val - m).astype(f32)
-        q = np.rint(e.astype(np.float64) * np.float64(inv))   # exact product, one rounding
-        out[k] = np.minimum(65534, q).astype(np.uint16)
-        L = ((cw + val).astype(f32) - m).astype(f32)
-        lout[k] = 0.0 if invalid else L
-    return (out, lout) if want_l else out
 
 
 @pytest.mark.parametrize("cost_type,planes,paths", [("f16", 64, 8), ("bf16", 96, 8),
@@ -527,14 +461,14 @@ def test_compact_levels_match_scanline_replay(dev, small_scene, cost_type, plane
     E = ws.view(paths, H, W, planes).cpu().numpy().view(np.uint16)
     c = cost.float().cpu().numpy()
     img = s["ref"]
-    step = _excess_step(knobs)
+    step = parity.excess_step(knobs)
     checked = 0
     for p in range(paths):
-        dx, dy = _PATH_DX[p], _PATH_DY[p]
+        dx, dy = parity.PATH_DX[p], parity.PATH_DY[p]
         nlines = W + H - 1 if dx and dy else (H if dx else W)
         for i in sorted({0, nlines // 3, nlines // 2, 2 * nlines // 3 + 1, nlines - 1}):
-            pts = _scanline(W, H, dx, dy, i)
-            want = _replay_levels(c, img, pts, dx, dy, knobs, step)
+            pts = parity.scanline(W, H, dx, dy, i)
+            want = parity.replay_scanline(c, img, pts, dx, dy, knobs, step)
             got = np.stack([E[p, y, x] for x, y in pts])
             bad = np.argwhere(got != want)
             assert bad.size == 0, (p, i, bad[:4].tolist(), len(pts))
@@ -563,11 +497,11 @@ def test_exact_paths_match_scanline_replay(dev, small_scene, cost_type, planes):
     img = s["ref"]
     checked = 0
     for p in range(8):
-        dx, dy = _PATH_DX[p], _PATH_DY[p]
+        dx, dy = parity.PATH_DX[p], parity.PATH_DY[p]
         nlines = W + H - 1 if dx and dy else (H if dx else W)
         for i in sorted({0, nlines // 3, nlines // 2, 2 * nlines // 3 + 1, nlines - 1}):
-            pts = _scanline(W, H, dx, dy, i)
-            _, want = _replay_levels(c, img, pts, dx, dy, knobs, _excess_step(knobs), True)
+            pts = parity.scanline(W, H, dx, dy, i)
+            _, want = parity.replay_scanline(c, img, pts, dx, dy, knobs, parity.excess_step(knobs), True)
             got = np.stack([Lv[p, y, x] for x, y in pts])
             bad = np.argwhere(got.view(np.uint32) != want.view(np.uint32))
             assert bad.size == 0, (p, i, bad[:4].tolist(), len(pts))

@@ -121,3 +121,41 @@ def test_scanline_enumeration_covers_every_pixel_once():
                            if not (0 <= x0 - dx < W and 0 <= y0 - dy < H))
             expect = W + H - 1 if (dx and dy) else (H if dx else W)
             assert n_starts == expect
+
+
+@pytest.mark.parametrize("shape,paths", [((13, 17, 9), 8), ((7, 11, 64), 8), ((9, 5, 5), 4)])
+def test_float32_scanline_replay_sums_to_oracle(shape, paths):
+    """The float32 scanline replay that the GPU tests compare stored path volumes with
+    (``parity.replay_scanline``), run over every scanline of every direction and summed per
+    pixel in path order, equals the float32 oracle's aggregate bit for bit and the FP64
+    oracle's to float rounding; invalid pixels are 0.  Each scanline of a direction is visited
+    once (``parity.scanline`` covers every pixel exactly once), and the per-pixel edge switch of
+    P1/P2 is exercised by a reference image with both small and large steps."""
+    from tests import parity
+    rng = np.random.default_rng(shape[0] * 100 + shape[2] + paths)
+    H, W, D = shape
+    ref = rng.integers(0, 12, size=(H, W), dtype=np.uint8) * np.uint8(rng.integers(1, 4))
+    ref[rng.random((H, W)) < 0.3] = 200
+    cost = rng.uniform(0.0, 40.0, size=(H, W, D)).astype(np.float32)
+    cost[rng.random((H, W)) < 0.1, 0] = -1.0
+    kn = dict(pi1=3.0, pi2=11.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+    invalid = cost.min(axis=2) < 0
+    total = np.zeros((H, W, D), np.float32)
+    step = parity.excess_step(kn)
+    for p in range(paths):
+        dx, dy = parity.PATH_DX[p], parity.PATH_DY[p]
+        nlines = W + H - 1 if dx and dy else (H if dx else W)
+        seen = np.zeros((H, W), np.int32)
+        for i in range(nlines):
+            pts = parity.scanline(W, H, dx, dy, i)
+            _, lv = parity.replay_scanline(cost, ref, pts, dx, dy, kn, step, True)
+            for k, (x, y) in enumerate(pts):
+                total[y, x] = (total[y, x] + lv[k]).astype(np.float32)
+                seen[y, x] += 1
+        assert np.all(seen == 1)
+    want32 = oracle.sgm(ref, cost, kn, paths, precision="f32")
+    assert np.array_equal(total.view(np.uint32), want32.view(np.uint32))
+    want64 = oracle.sgm(ref, cost.astype(np.float64), kn, paths, precision="f64")
+    assert np.all(total[invalid] == 0) and np.all(want64[invalid] == 0)
+    rel = np.abs(total - want64) / np.maximum(np.abs(want64), 1.0)
+    assert rel.max() <= 1e-5
