@@ -112,11 +112,6 @@ def test_batch_autotune_and_graph_replay(uneven, storage):
         _assert_equal(batch, expect)
 
 
-def _excess_step(knobs):
-    p2 = max(np.float32(knobs["pi2"]), np.float32(knobs["pi2"]) * np.float32(knobs["sgm_q2"]))
-    return float(np.float32(p2) / np.float32(65534.0))
-
-
 @pytest.mark.parametrize("cost_type,paths", [("f16", 8), ("f32", 8), ("bf16", 4), ("f16", 4)])
 def test_batch_sweep_against_exact(uneven, cost_type, paths):
     """Sweep SGM storage against the exact batch on the same keyframes (D = 32, 96, 128, 64:
@@ -133,7 +128,7 @@ def test_batch_sweep_against_exact(uneven, cost_type, paths):
     dn = pipeline.Batch(uneven, cost_type, KNOBS, paths, "sweep", write_agg=False)
     dn.run()
     torch.cuda.synchronize()
-    step = _excess_step(KNOBS)
+    step = parity.excess_step(KNOBS)
     n_down = 3 if paths == 8 else 1
     for i, (a, b, c) in enumerate(zip(ex.outputs, sw.outputs, dn.outputs)):
         assert torch.equal(a["cost"], b["cost"])
@@ -173,7 +168,7 @@ def test_batch_sweep_small_keyframes():
     sw = pipeline.Batch(specs, "f16", KNOBS, 8, "sweep")
     sw.run()
     torch.cuda.synchronize()
-    bound_step = (2 + 3) * _excess_step(KNOBS) / 2
+    bound_step = (2 + 3) * parity.excess_step(KNOBS) / 2
     for i, (a, b) in enumerate(zip(ex.outputs, sw.outputs)):
         ge = a["agg"].cpu().numpy().astype(np.float64)
         gs = b["agg"].cpu().numpy().astype(np.float64)
@@ -182,6 +177,47 @@ def test_batch_sweep_small_keyframes():
         st = parity.depth_parity(b["depth"].cpu().numpy(), a["depth"].cpu().numpy())
         n = ge.shape[0] * ge.shape[1]
         assert st["category_mismatch"] * n <= max(3, 1e-3 * n), (i, st)   # a few ties
+
+
+@pytest.mark.parametrize("cost_type", ["f16", "bf16"])
+def test_batch_compact_ragged_keyframes(cost_type):
+    """Compact storage with 16-bit costs on keyframes of 1 to 9 rows or 1 to 5 columns (paths
+    with fewer scanlines than one 16-scanline item, diagonals from length 1) beside two of
+    ordinary size: the batch, with the paths four scanlines per wave and one, equals the
+    single-keyframe calls bit for bit.  Keyframes under 4 rows or columns have no valid cost
+    (the 9-term pattern needs the neighbouring rows and columns), so they run the invalid-pixel
+    route; the others mix valid and invalid pixels."""
+    from probabilistic_mapping_amd import pipeline, synth
+    specs = []
+    for seed, (W, H, D) in enumerate([(40, 3, 32), (1, 24, 32), (33, 1, 64), (130, 7, 96),
+                                      (5, 9, 128), (2, 2, 32), (19, 6, 128), (90, 41, 64),
+                                      (47, 70, 96)]):
+        sc = synth.trajectory_scene(seed + 61, width=W, height=H, n_frames=4, n_sources=3,
+                                    planes=D, fx=max(0.612 * W, 4.0))
+        K = sc["scene"].K
+        imgs = [torch.from_numpy(im).cuda() for im in sc["images"]]
+        quads = [pipeline.make_quad(im) for im in imgs]
+        homs = [np_homography(K, *sc["poses"][1], *sc["poses"][j]) for j in (0, 2, 3)]
+        specs.append({"ref": imgs[1], "quads": [quads[j] for j in (0, 2, 3)],
+                      "homs": homs, "planes": D, "idep_min": sc["idep_min"],
+                      "dep_sample": (sc["idep_max"] - sc["idep_min"]) / (D - 1)})
+    expect = [_single(pipeline, kf, cost_type, 8, "compact") for kf in specs]
+    valid = [float((e["cost"].float().amin(dim=2) >= 0).float().mean()) for e in expect]
+    _report(f"ragged batch {cost_type} valid pixel fractions", {"valid": valid})
+    assert max(valid) > 0.3
+    batch = pipeline.Batch(specs, cost_type, KNOBS, 8, "compact")
+    try:
+        for sgm_rows in (1, 0):
+            for pref in (8, 16):
+                for o in batch.outputs:
+                    for v in o.values():
+                        v.fill_(5)
+                batch.tuning = dict(sgm_rows=sgm_rows, sgm_prefetch=pref)
+                batch.run()
+                torch.cuda.synchronize()
+                _assert_equal(batch, expect)
+    finally:
+        batch.close()
 
 
 def test_batch_sweep_graph_replay(uneven):

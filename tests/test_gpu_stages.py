@@ -476,6 +476,44 @@ def test_compact_levels_match_scanline_replay(dev, small_scene, cost_type, plane
     _report(f"compact levels vs replay {cost_type} D={planes} P={paths}", {"pixels": checked})
 
 
+@pytest.mark.parametrize("W,H,planes,cost_type", [(40, 3, 32, "f16"), (70, 17, 64, "bf16"),
+                                                  (130, 7, 96, "f16"), (5, 9, 128, "f16"),
+                                                  (1, 24, 32, "f16"), (33, 1, 64, "bf16"),
+                                                  (2, 2, 32, "f16"), (19, 6, 128, "bf16")])
+def test_ragged_compact_levels_match_scanline_replay(dev, W, H, planes, cost_type):
+    """Four scanlines per wave at ragged sizes: fewer scanlines in a path than one wave's four
+    (H = 1, 2, 3 or W = 1, 2), diagonals of every length from 1, and groups whose last rows lie
+    past the path's last scanline.  Seeded 16-bit costs with 10 % invalid pixels and a
+    reference image with both smooth and edge steps; the stored levels of every scanline of
+    all 8 directions equal the float32 replay."""
+    rng = np.random.default_rng(W * 1000 + H * 10 + planes)
+    tdt = torch.float16 if cost_type == "f16" else torch.bfloat16
+    c = torch.from_numpy(rng.uniform(0.0, 60.0, size=(H, W, planes)).astype(np.float32))
+    inval = rng.random((H, W)) < 0.1
+    c[torch.from_numpy(inval), 0] = -1.0
+    cost = c.to(tdt).cuda()
+    img = (rng.integers(0, 3, size=(H, W)) * 3 + (rng.random((H, W)) < 0.3) * 90).astype(np.uint8)
+    ref = torch.from_numpy(img).cuda()
+    knobs = dict(pi1=3.0, pi2=11.0, tau_so=4.0, sgm_q1=2.0, sgm_q2=3.0, var_scale=1.02)
+    ws = dev.sgm_excess_workspace(H, W, planes, 8)
+    dev.sgm_filter_excess(ref, cost, knobs, 8, 0.125, 0.03, workspace=ws)
+    torch.cuda.synchronize()
+    E = ws.view(8, H, W, planes).cpu().numpy().view(np.uint16)
+    cf = cost.float().cpu().numpy()
+    step = parity.excess_step(knobs)
+    for p in range(8):
+        dx, dy = parity.PATH_DX[p], parity.PATH_DY[p]
+        nlines = W + H - 1 if dx and dy else (H if dx else W)
+        for i in range(nlines):
+            pts = parity.scanline(W, H, dx, dy, i)
+            want = parity.replay_scanline(cf, img, pts, dx, dy, knobs, step)
+            got = np.stack([E[p, y, x] for x, y in pts])
+            bad = np.argwhere(got != want)
+            assert bad.size == 0, (p, i, bad[:4].tolist(), len(pts))
+    _report(f"ragged compact levels vs replay {cost_type} {W}x{H}x{planes}",
+            {"pixels": 8 * W * H, "invalid": int(inval.sum())})
+
+
 @pytest.mark.parametrize("cost_type,planes", [("f32", 64), ("f16", 128), ("f32", 200)])
 def test_exact_paths_match_scanline_replay(dev, small_scene, cost_type, planes):
     """The per-path FP32 L volumes of the exact all-paths launch (pm_sgm_filter_paths; 0 at
