@@ -325,7 +325,9 @@ typedef enum pm_sgm_storage {
   PM_SGM_SWEEP = 2
 } pm_sgm_storage;
 
-/* Copies the keyframe table and work lists to the device (allocates; call once). */
+/* Copies the keyframe table and work lists to the device (allocates; call once).  The batch
+ * works on its keyframes largest first (W H D, then source count); each keyframe's results go
+ * to its own buffers, so the caller's order only names keyframes in error messages. */
 pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count, pm_cost_type cost_type,
                           pm_sgm_params params, float var_scale, pm_sgm_storage storage,
                           pm_batch** out);

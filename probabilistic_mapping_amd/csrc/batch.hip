@@ -255,10 +255,22 @@ extern "C" pm_status pm_batch_create(const pm_keyframe* kfs, int32_t count,
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
       cus > 0)
     b->cus = cus;
+  // Keyframes are held largest first (cost volume W H D, then source count; equal ones in the
+  // caller's order), so the persistent launches start their longest items first and end with
+  // the short ones.  Each keyframe carries its own output buffers, so the order is internal.
+  std::vector<int> order(count);
+  for (int k = 0; k < count; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int i, int j) {
+    const uint64_t vi = (uint64_t)std::max(kfs[i].width, 0) * (uint64_t)std::max(kfs[i].height, 0) *
+                        (uint64_t)std::max(kfs[i].planes, 0);
+    const uint64_t vj = (uint64_t)std::max(kfs[j].width, 0) * (uint64_t)std::max(kfs[j].height, 0) *
+                        (uint64_t)std::max(kfs[j].planes, 0);
+    return vi != vj ? vi > vj : kfs[i].num_sources > kfs[j].num_sources;
+  });
   b->host.resize(count);
   for (int k = 0; k < count; ++k) {
-    const pm_keyframe& in = kfs[k];
-    const std::string who = "pm_batch_create: keyframe " + std::to_string(k) + ": ";
+    const pm_keyframe& in = kfs[order[k]];
+    const std::string who = "pm_batch_create: keyframe " + std::to_string(order[k]) + ": ";
     if (in.width <= 0 || in.height <= 0 || in.width > 65535 || in.height > 65535 ||
         (long)in.width * in.height >= (1L << 29)) {
       delete b;
